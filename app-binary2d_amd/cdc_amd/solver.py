@@ -1,0 +1,147 @@
+"""Host-side handle over the C ABI: BlockData + State + scheme::advance.
+
+`Solver.advance(fold)` is scheme::advance (src/scheme.rs:323-349) on the GPU;
+a hydro error raises `HydroError` (src/physics.rs:26-71) and leaves the
+solver's state as it was before the call.  Arrays use the reference layout:
+conserved [block][i][j][v], ist [block][7][nv], oec [block][7][4].
+"""
+import ctypes
+from ctypes import byref, c_double, c_int64, c_void_p
+
+import numpy as np
+
+from . import abi
+
+
+class CdcError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+class HydroError(CdcError):
+    """HydroError: NegativeDensity / NegativePressure / OrbitalEvolutionError."""
+
+    def __init__(self, err):
+        self.kind = {abi.CDC_ERR_NEGATIVE_DENSITY: "NegativeDensity",
+                     abi.CDC_ERR_NEGATIVE_PRESSURE: "NegativePressure",
+                     abi.CDC_ERR_UNBOUND_ORBIT: "OrbitalEvolutionError"}.get(err.code, "Unknown")
+        self.value = err.value
+        self.position = (err.x, err.y)
+        self.time = err.time
+        self.binary = [(b.mass, b.position_x, b.position_y, b.velocity_x, b.velocity_y) for b in err.binary] \
+            if err.has_binary else None
+        super().__init__(err.code, err.message.decode(errors="replace"))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(c_void_p)
+
+
+def _check(lib, rc):
+    if rc != abi.CDC_OK:
+        raise CdcError(rc, lib.cdc_last_error_message().decode(errors="replace"))
+
+
+def orbital_state_from_time(physics, t):
+    lib = abi.load()
+    out = (abi.CPointMass * 2)()
+    c = physics.to_c()
+    _check(lib, lib.cdc_orbital_state_from_time(byref(c), float(t), out))
+    return np.array([[p.mass, p.position_x, p.position_y, p.velocity_x, p.velocity_y] for p in out])
+
+
+def seeded_cavity(mesh, hydro, cavity_radius=2.5, cavity_exponent=12.0, density_floor=1e-4,
+                  noise_amplitude=1e-3, softening_length=0.05, mach_number=10.0, seed=1):
+    """The seeded cavity initial model (no reference counterpart), per cell."""
+    lib = abi.load()
+    m = abi.CCavityModel(cavity_radius, cavity_exponent, density_floor, noise_amplitude,
+                         softening_length, mach_number, seed)
+    nb, bs = mesh.num_blocks, mesh.block_size
+    out = np.zeros((nb * nb, bs, bs, hydro.nv), dtype=np.float64)
+    cm, ch = mesh.to_c(), hydro.to_c()
+    _check(lib, lib.cdc_model_seeded_cavity(byref(cm), byref(ch), byref(m), _ptr(out)))
+    return out
+
+
+class Solver:
+    """One mesh, hydro system and physics on one GPU."""
+
+    def __init__(self, mesh, hydro, physics, initial_conserved):
+        self.lib = abi.load()
+        abi.check_layout(self.lib)
+        self.mesh, self.hydro, self.physics = mesh, hydro, physics
+        self.nv = hydro.nv
+        self.nblk = mesh.num_blocks ** 2
+        shape = (self.nblk, mesh.block_size, mesh.block_size, self.nv)
+        init = np.ascontiguousarray(initial_conserved, dtype=np.float64).reshape(shape)
+        self._cm, self._ch, self._cp = mesh.to_c(), hydro.to_c(), physics.to_c()
+        h = c_void_p()
+        _check(self.lib, self.lib.cdc_solver_create(byref(self._cm), byref(self._ch), byref(self._cp),
+                                                   _ptr(init), byref(h)))
+        self.handle = h
+        self.shape = shape
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.cdc_solver_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_physics(self, physics):
+        self.physics = physics
+        self._cp = physics.to_c()
+        _check(self.lib, self.lib.cdc_solver_set_physics(self.handle, byref(self._cp)))
+
+    def set_state(self, conserved, time=0.0, iteration=(0, 1), ist=None, oec=None):
+        u = np.ascontiguousarray(conserved, dtype=np.float64).reshape(self.shape)
+        i = None if ist is None else np.ascontiguousarray(ist, dtype=np.float64).reshape(self.nblk, 7, self.nv)
+        e = None if oec is None else np.ascontiguousarray(oec, dtype=np.float64).reshape(self.nblk, 7, 4)
+        _check(self.lib, self.lib.cdc_solver_set_state(self.handle, float(time), int(iteration[0]),
+                                                      int(iteration[1]), _ptr(u),
+                                                      None if i is None else _ptr(i),
+                                                      None if e is None else _ptr(e)))
+
+    def get_state(self):
+        """Returns dict(time, iteration=(num, den), conserved, ist, oec)."""
+        u = np.empty(self.shape, dtype=np.float64)
+        ist = np.empty((self.nblk, 7, self.nv), dtype=np.float64)
+        oec = np.empty((self.nblk, 7, 4), dtype=np.float64)
+        t, n, d = c_double(), c_int64(), c_int64()
+        _check(self.lib, self.lib.cdc_solver_get_state(self.handle, byref(t), byref(n), byref(d),
+                                                      _ptr(u), _ptr(ist), _ptr(oec)))
+        return {"time": t.value, "iteration": (n.value, d.value), "conserved": u, "ist": ist, "oec": oec}
+
+    def advance(self, fold=1):
+        """scheme::advance: returns dt; raises HydroError (state unchanged)."""
+        dt = c_double()
+        err = abi.CError()
+        rc = self.lib.cdc_solver_advance(self.handle, int(fold), byref(dt), byref(err))
+        if rc in (abi.CDC_ERR_NEGATIVE_DENSITY, abi.CDC_ERR_NEGATIVE_PRESSURE, abi.CDC_ERR_UNBOUND_ORBIT):
+            raise HydroError(err)
+        _check(self.lib, rc)
+        return dt.value
+
+    def max_signal_speed(self):
+        a = c_double()
+        _check(self.lib, self.lib.cdc_solver_max_signal_speed(self.handle, byref(a)))
+        return a.value
+
+    def set_timing(self, enable=True):
+        _check(self.lib, self.lib.cdc_solver_set_timing(self.handle, int(enable)))
+
+    def get_timing(self):
+        n, ms, cells = c_int64(), c_double(), c_double()
+        _check(self.lib, self.lib.cdc_solver_get_timing(self.handle, byref(n), byref(ms), byref(cells)))
+        return n.value, ms.value, cells.value

@@ -1,0 +1,89 @@
+// cdc_internal.h -- launch interface between the host driver (cdc_solver.cpp)
+// and the gfx950 kernels (cdc_kernels.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#include "../../include/cdc_solver.h"
+#include "cdc_orbit.h"
+
+namespace cdc {
+
+// Per-tile error record of the fused stage kernel: the first failing cell of
+// the tile in block-local C order (cell = i * bs + j).
+struct TileErr {
+    int32_t code;
+    int32_t cell;
+    double value;
+};
+
+// Physics constants of a stage launch, folded once on the host.
+struct StageConsts {
+    double dx, dy, idx, idy;
+    double rs2;                 // softening_length^2
+    double inv_mach2;           // isothermal: 1 / Mach^2
+    double gm, gm1, inv_gm1;    // Euler gamma, gamma - 1, 1 / (gamma - 1)
+    double plm, pdt;            // PLM theta, plm_density_trigger (0 when unset)
+    double visc;                // nu (constant_nu) or alpha / sqrt(gamma) (alpha)
+    double lam;                 // lambda_multiplier
+    double eps_dx;              // 1e-12 * dx (alpha model)
+    double domain_radius, inv_buffer_scale, buffer_coef;  // 0.5 * buffer_rate * Omega_outer
+    double sink_s2, inv_sink_s2, sink_rate;
+    double fake_threshold, fake_rate;
+    double cooling_coefficient, cooling_inv_beta;
+    double cooling_f;           // T^4 slow-start factor exp(-2 pi / (t + dt)) or 1
+    double mach_ceiling_coef;   // 2 / gm / (gm - 1) / mach_ceiling^2
+    double density_floor, pressure_floor, density_floor_pressure;
+    double mtot;
+    int32_t viscosity_model, sink_model, one_body, has_mach_ceiling;
+    int32_t has_density_floor, has_pressure_floor, density_index, cooling_slow_start;
+    int32_t cooling_model, has_pdt;
+};
+
+struct StageArgs {
+    const double *uc;      // stage input, SoA [nv][N][N]
+    const double *u0;      // initial (background) state, SoA
+    const double *ub;      // RK step base, SoA (read when read_base)
+    double *uo;            // stage output, SoA
+    double *tile_sums;     // [ntiles][NSUM + 1] (last slot: max signal speed)
+    TileErr *tile_err;     // [ntiles]
+    int32_t *err_flag;     // stage id of the first failing stage, 0 if none
+    int32_t N, nb, bs, ntiles;
+    int32_t stage_id, read_base, want_amax, pad_;
+    double R, L;           // domain radius, block length
+    double time, dt, c1, c0;
+    Orbit orb;
+    StageConsts k;
+};
+
+struct FinalizeArgs {
+    const double *tile_sums;
+    const double *ist_c, *ist_b;   // [nblk][7][nv]
+    const double *oec_c, *oec_b;   // [nblk][7][4]
+    double *ist_o, *oec_o;
+    int32_t *perr;                 // [nblk]
+    int32_t *err_flag;
+    int32_t nblk, tiles_per_block, stage_id, read_base;
+    double c1, c0, area;           // RK weights, dx * dy
+    Orbit orb;
+    double el[4];
+};
+
+// Tile shape chosen for a block size; see DESIGN.md for the LDS budget.
+struct TileShape {
+    int tx, ty, nt;
+};
+TileShape tile_shape_for(int bs, int nv);
+
+int nsum_for(int nv);
+
+hipError_t launch_stage(int nv, const TileShape &ts, const StageArgs &a, hipStream_t s);
+hipError_t launch_finalize(int nv, const FinalizeArgs &a, hipStream_t s);
+hipError_t launch_max_speed(int nv, const double *u, int64_t ncell, double gm, double *partial,
+                            int nparts, hipStream_t s);
+hipError_t launch_aos_to_soa(int nv, int nb, int bs, const double *aos, double *soa, hipStream_t s);
+hipError_t launch_soa_to_aos(int nv, int nb, int bs, const double *soa, double *aos, hipStream_t s);
+hipError_t launch_set_int(int32_t *p, int32_t v, hipStream_t s);
+
+}  // namespace cdc

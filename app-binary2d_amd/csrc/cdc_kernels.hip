@@ -1,0 +1,706 @@
+// cdc_kernels.hip -- gfx950 kernels of the per-block Runge-Kutta update.
+//
+// One fused FP64 kernel per RK stage (stage_kernel) does, per tile of
+// TX x TY cells inside one reference block:
+//   phase 1  conserved -> primitive on the tile plus its 2-zone halo, halo
+//            cells read straight from the neighbouring blocks' HBM with
+//            periodic wrap (guard extension, src/scheme.rs:126-127), into LDS;
+//            first failing cell recorded (try_block_primitive,
+//            src/scheme.rs:185-199)
+//   phase 2  PLM gradients gx, gy on the 1-zone ring (src/scheme.rs:224-225)
+//   phase 3  x- and y-face fluxes: HLLE + viscous stress, constant-nu or
+//            alpha viscosity, Plummer potential at the face
+//            (src/scheme.rs:230-268, src/physics.rs:584-607, 776-803)
+//   phase 4  flux divergence, source terms (gravity, sinks, buffer, fake mass,
+//            cooling), the low-storage RK combine with the step base, and the
+//            per-tile sums of the itemized source terms
+//            (src/scheme.rs:273-316, src/state.rs:183-200)
+// finalize_kernel then sums the tiles of each block in a fixed order, scales
+// by dx*dy, applies the orbital-element perturbation and the RK combine of
+// the block's integrated_source_terms / orbital_elements_change.
+#include <hip/hip_runtime.h>
+#include <limits.h>
+#include <math.h>
+
+#include "cdc_internal.h"
+
+namespace cdc {
+
+// ---------------------------------------------------------------------------
+// Compact list of the itemized-source components that can be nonzero; the
+// others (grav mass, iso cooling, Euler fake_mass) are identically zero.
+
+template <int NV> struct SumMap;
+template <> struct SumMap<3> {
+    static constexpr int n = 16;
+    __host__ __device__ static constexpr int item(int s) {
+        return s < 3 ? CDC_ITEM_SINK1 : s < 6 ? CDC_ITEM_SINK2 : s < 8 ? CDC_ITEM_GRAV1
+             : s < 10 ? CDC_ITEM_GRAV2 : s < 13 ? CDC_ITEM_BUFFER : CDC_ITEM_FAKE_MASS;
+    }
+    __host__ __device__ static constexpr int comp(int s) {
+        return s < 3 ? s : s < 6 ? s - 3 : s < 8 ? s - 5 : s < 10 ? s - 7 : s < 13 ? s - 10 : s - 13;
+    }
+};
+template <> struct SumMap<4> {
+    static constexpr int n = 22;
+    __host__ __device__ static constexpr int item(int s) {
+        return s < 4 ? CDC_ITEM_SINK1 : s < 8 ? CDC_ITEM_SINK2 : s < 11 ? CDC_ITEM_GRAV1
+             : s < 14 ? CDC_ITEM_GRAV2 : s < 18 ? CDC_ITEM_BUFFER : CDC_ITEM_COOLING;
+    }
+    __host__ __device__ static constexpr int comp(int s) {
+        return s < 4 ? s : s < 8 ? s - 4 : s < 11 ? s - 7 : s < 14 ? s - 10 : s < 18 ? s - 14 : s - 18;
+    }
+};
+
+int nsum_for(int nv) { return nv == 4 ? SumMap<4>::n : SumMap<3>::n; }
+
+namespace dev {
+
+// Block geometry exactly as Mesh::block_vertices / cell_centers
+// (src/mesh.rs:45-84): linspace(start, start + L, bs + 1), centres as
+// adjacent means.  No contraction, so the coordinates match the CPU's bits.
+struct Axis {
+    double a, step;
+    __device__ __forceinline__ double v(int k) const {
+#pragma clang fp contract(off)
+        return a + step * (double)k;
+    }
+    __device__ __forceinline__ double c(int k) const {
+#pragma clang fp contract(off)
+        return (v(k) + v(k + 1)) / 2.0;
+    }
+};
+
+__device__ __forceinline__ Axis block_axis(double R, double L, int bi, int bs) {
+#pragma clang fp contract(off)
+    Axis ax;
+    ax.a = -R + (double)bi * L;
+    const double b = ax.a + L;
+    ax.step = (b - ax.a) / (double)bs;
+    return ax;
+}
+
+__device__ __forceinline__ double minmod3(double a, double b, double c) {
+    const double mn = fmin(a, fmin(b, c));
+    const double mx = fmax(a, fmax(b, c));
+    return mn > 0.0 ? mn : (mx < 0.0 ? mx : 0.0);
+}
+
+// godunov-core plm_gradient: minmod(theta (b - a), (c - a) / 2, theta (c - b))
+__device__ __forceinline__ double plm(double th, double a, double b, double c) {
+    return minmod3(th * (b - a), 0.5 * (c - a), th * (c - b));
+}
+
+// try_to_primitive: iso src/physics.rs:489-503, Euler src/physics.rs:631-660
+template <int NV>
+__device__ __forceinline__ int try_to_primitive(const StageConsts &k, const double *u, double *p,
+                                                double &val) {
+    if constexpr (NV == 3) {
+        double d = u[0], sx = u[1], sy = u[2];
+        if (k.has_density_floor && d < k.density_floor) {
+            sx = 0.0; sy = 0.0; d = k.density_floor;
+        }
+        const double id = 1.0 / d;
+        p[0] = d; p[1] = sx * id; p[2] = sy * id;
+        if (d < 0.0) { val = d; return CDC_ERR_NEGATIVE_DENSITY; }
+        return 0;
+    } else {
+        const double d = u[0];
+        const double id = 1.0 / d;
+        const double vx = u[1] * id, vy = u[2] * id;
+        double pg = (u[3] - 0.5 * d * (vx * vx + vy * vy)) * k.gm1;
+        p[0] = d; p[1] = vx; p[2] = vy; p[3] = pg;
+        if (k.has_density_floor && d < k.density_floor) {
+            p[0] = k.density_floor; p[1] = 0.0; p[2] = 0.0; p[3] = k.density_floor_pressure;
+        }
+        if (p[0] < 0.0) { val = p[0]; return CDC_ERR_NEGATIVE_DENSITY; }
+        if (k.has_pressure_floor && p[3] < k.pressure_floor) p[3] = k.pressure_floor;
+        if (p[3] <= 0.0) { val = p[3]; return CDC_ERR_NEGATIVE_PRESSURE; }
+        return 0;
+    }
+}
+
+// Hydrodynamics::max_signal_speed of a conserved state: iso
+// max(|vx|, |vy|) (src/physics.rs:521-526), Euler max(|vx|, |vy|) + c_s.
+template <int NV>
+__device__ __forceinline__ double max_signal_speed(const double *u, double gm) {
+    if constexpr (NV == 3) {
+        return fmax(fabs(u[1] / u[0]), fabs(u[2] / u[0]));
+    } else {
+        const double d = u[0];
+        const double vx = u[1] / d, vy = u[2] / d;
+        const double p = (u[3] - 0.5 * d * (vx * vx + vy * vy)) * (gm - 1.0);
+        return fmax(fabs(vx), fabs(vy)) + sqrt(gm * p / d);
+    }
+}
+
+// One face: reconstruction, viscous stress, HLLE.  axis 0 = x, 1 = y.
+// eL/eR index the primitive tile, rL/rR the gradient ring.
+template <int NV, int EXY, int RXY>
+__device__ __forceinline__ void face_flux(const StageArgs &A, int axis, const double *P,
+                                          const double *GX, const double *GY, int eL, int eR,
+                                          int rL, int rR, double fxp, double fyp, double *F) {
+    const StageConsts &k = A.k;
+    const double *GN = axis == 0 ? GX : GY;
+    double pL[NV], pR[NV], pl[NV], pr[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        pL[v] = P[v * EXY + eL];
+        pR[v] = P[v * EXY + eR];
+        pl[v] = pL[v] + GN[v * RXY + rL] * 0.5;
+        pr[v] = pR[v] - GN[v * RXY + rR] * 0.5;
+    }
+
+    // Plummer potential at the face (isothermal sound speed, alpha model)
+    const Orbit &o = A.orb;
+    const double d1x = fxp - o.x1, d1y = fyp - o.y1;
+    const double d2x = fxp - o.x2, d2y = fyp - o.y2;
+    const double r1s = d1x * d1x + d1y * d1y;
+    const double r2s = d2x * d2x + d2y * d2y;
+    double cs2f = 0.0;
+    if constexpr (NV == 3) {
+        const double phi = -o.m1 / sqrt(r1s + k.rs2) - o.m2 / sqrt(r2s + k.rs2);
+        cs2f = -phi * k.inv_mach2;
+    }
+
+    // viscous stress tau = (sigma_L + sigma_R) / 2 (src/physics.rs:322-340)
+    double tau_x = 0.0, tau_y = 0.0;
+    const bool visc = k.viscosity_model != CDC_VISC_CONSTANT_NU || k.visc != 0.0;
+    if (visc) {
+        double nuL, nuR;
+        if (k.viscosity_model == CDC_VISC_CONSTANT_NU) {
+            nuL = nuR = k.visc;
+        } else {
+            const double q1 = sqrt(r1s) + k.eps_dx;
+            const double q2 = sqrt(r2s) + k.eps_dx;
+            const double itwofreq = 1.0 / sqrt(o.m1 / (q1 * q1 * q1) + o.m2 / (q2 * q2 * q2));
+            if constexpr (NV == 3) {
+                nuL = nuR = k.visc * cs2f * itwofreq;
+            } else {
+                nuL = k.visc * (k.gm * pL[3] / pL[0]) * itwofreq;
+                nuR = k.visc * (k.gm * pR[3] / pR[0]) * itwofreq;
+            }
+        }
+        const double gxvxL = GX[1 * RXY + rL] * k.idx, gxvyL = GX[2 * RXY + rL] * k.idx;
+        const double gyvxL = GY[1 * RXY + rL] * k.idy, gyvyL = GY[2 * RXY + rL] * k.idy;
+        const double gxvxR = GX[1 * RXY + rR] * k.idx, gxvyR = GX[2 * RXY + rR] * k.idx;
+        const double gyvxR = GY[1 * RXY + rR] * k.idy, gyvyR = GY[2 * RXY + rR] * k.idy;
+        const double offL = gxvyL + gyvxL, offR = gxvyR + gyvxR;
+        const double divL = gxvxL + gyvyL, divR = gxvxR + gyvyR;
+        if (axis == 0) {
+            const double sxxL = pL[0] * (nuL * (4.0 / 3.0 * gxvxL - 2.0 / 3.0 * gyvyL) + k.lam * nuL * divL);
+            const double sxxR = pR[0] * (nuR * (4.0 / 3.0 * gxvxR - 2.0 / 3.0 * gyvyR) + k.lam * nuR * divR);
+            tau_x = 0.5 * (sxxL + sxxR);
+            tau_y = 0.5 * (pL[0] * (nuL * offL) + pR[0] * (nuR * offR));
+        } else {
+            const double syyL = pL[0] * (nuL * (-2.0 / 3.0 * gxvxL + 4.0 / 3.0 * gyvyL) + k.lam * nuL * divL);
+            const double syyR = pR[0] * (nuR * (-2.0 / 3.0 * gxvxR + 4.0 / 3.0 * gyvyR) + k.lam * nuR * divR);
+            tau_x = 0.5 * (pL[0] * (nuL * offL) + pR[0] * (nuR * offR));
+            tau_y = 0.5 * (syyL + syyR);
+        }
+    }
+
+    // HLLE (Toro ch. 10), outer wave speeds v_n -/+ c_s
+    const double vnl = pl[1 + axis], vnr = pr[1 + axis];
+    double ul[NV], ur[NV], fl[NV], fr[NV], csl, csr;
+    ul[0] = pl[0]; ul[1] = pl[0] * pl[1]; ul[2] = pl[0] * pl[2];
+    ur[0] = pr[0]; ur[1] = pr[0] * pr[1]; ur[2] = pr[0] * pr[2];
+    double ppl, ppr;
+    if constexpr (NV == 3) {
+        csl = csr = sqrt(cs2f);
+        ppl = pl[0] * cs2f;
+        ppr = pr[0] * cs2f;
+    } else {
+        ul[3] = 0.5 * pl[0] * (pl[1] * pl[1] + pl[2] * pl[2]) + pl[3] * k.inv_gm1;
+        ur[3] = 0.5 * pr[0] * (pr[1] * pr[1] + pr[2] * pr[2]) + pr[3] * k.inv_gm1;
+        csl = sqrt(k.gm * pl[3] / pl[0]);
+        csr = sqrt(k.gm * pr[3] / pr[0]);
+        ppl = pl[3];
+        ppr = pr[3];
+        fl[3] = (ul[3] + pl[3]) * vnl;
+        fr[3] = (ur[3] + pr[3]) * vnr;
+    }
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        fl[v] = ul[v] * vnl;
+        fr[v] = ur[v] * vnr;
+    }
+    fl[1 + axis] += ppl;
+    fr[1 + axis] += ppr;
+    const double ap = fmax(fmax(vnl + csl, vnr + csr), 0.0);
+    const double am = fmin(fmin(vnl - csl, vnr - csr), 0.0);
+    const double inv = 1.0 / (ap - am);
+    const double apam = ap * am;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) F[v] = (fl[v] * ap - fr[v] * am - (ul[v] - ur[v]) * apam) * inv;
+    F[1] -= tau_x;
+    F[2] -= tau_y;
+    if constexpr (NV == 4) {
+        const double vx = 0.5 * (pL[1] + pR[1]);
+        const double vy = 0.5 * (pL[2] + pR[2]);
+        F[3] -= tau_x * vx + tau_y * vy;
+    }
+}
+
+// Plummer acceleration of one body (kepler-two-body restated)
+__device__ __forceinline__ void plummer(double m, double bx, double by, double x, double y,
+                                        double rs2, double &ax, double &ay) {
+    const double dx = x - bx, dy = y - by;
+    const double t = dx * dx + dy * dy + rs2;
+    const double mag = -m / (t * sqrt(t));
+    ax = mag * dx;
+    ay = mag * dy;
+}
+
+// Isothermal / Euler source_terms (src/physics.rs:528-582, 683-774) with
+// Physics::source_terms (src/physics.rs:423-457).  it[item][v].
+template <int NV>
+__device__ __forceinline__ void cell_sources(const StageArgs &A, const double *u, const double *u0,
+                                             double x, double y, double it[7][NV]) {
+    const StageConsts &k = A.k;
+    const Orbit &o = A.orb;
+    const double dt = A.dt;
+    const double sd = u[0];
+    double ax1, ay1, ax2, ay2;
+    plummer(o.m1, o.x1, o.y1, x, y, k.rs2, ax1, ay1);
+    plummer(o.m2, o.x2, o.y2, x, y, k.rs2, ax2, ay2);
+    const double fx1 = sd * ax1, fy1 = sd * ay1, fx2 = sd * ax2, fy2 = sd * ay2;
+
+    double sr1 = 0.0, sr2 = 0.0;
+    {
+        const double dx1 = x - o.x1, dy1 = y - o.y1, dx2 = x - o.x2, dy2 = y - o.y2;
+        const double q1 = dx1 * dx1 + dy1 * dy1, q2 = dx2 * dx2 + dy2 * dy2;
+        if (q1 < k.sink_s2) { const double w = 1.0 - q1 * k.inv_sink_s2; sr1 = k.sink_rate * (w * w); }
+        if (q2 < k.sink_s2) { const double w = 1.0 - q2 * k.inv_sink_s2; sr2 = k.sink_rate * (w * w); }
+    }
+    const double r = sqrt(x * x + y * y);
+    const double brate = k.buffer_coef * (1.0 + tanh((r - k.domain_radius) * k.inv_buffer_scale));
+
+    // primitive velocities (no floors, as the reference's to_primitive)
+    const double id = 1.0 / sd;
+    const double vx = u[1] * id, vy = u[2] * id;
+
+    // sinks
+    double s1[NV], s2[NV];
+    if (k.sink_model == CDC_SINK_IMPARTIAL) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { s1[v] = u[v] * -sr1; s2[v] = u[v] * -sr2; }
+    } else {
+        const double vx1 = k.one_body ? 0.0 : o.vx1, vy1 = k.one_body ? 0.0 : o.vy1;
+        const double vx2 = k.one_body ? 0.0 : o.vx2, vy2 = k.one_body ? 0.0 : o.vy2;
+        const double ex1 = x - o.x1, ey1 = y - o.y1, ex2 = x - o.x2, ey2 = y - o.y2;
+        const double ir1 = 1.0 / sqrt(ex1 * ex1 + ey1 * ey1);
+        const double ir2 = 1.0 / sqrt(ex2 * ex2 + ey2 * ey2);
+        const double r1x = ex1 * ir1, r1y = ey1 * ir1, r2x = ex2 * ir2, r2y = ey2 * ir2;
+        const double dv1 = (vx - vx1) * r1x + (vy - vy1) * r1y;
+        const double dv2 = (vx - vx2) * r2x + (vy - vy2) * r2y;
+        const double vx1c = dv1 * r1x + vx1, vy1c = dv1 * r1y + vy1;
+        const double vx2c = dv2 * r2x + vx2, vy2c = dv2 * r2y + vy2;
+        s1[0] = sd * -sr1; s1[1] = (sd * vx1c) * -sr1; s1[2] = (sd * vy1c) * -sr1;
+        s2[0] = sd * -sr2; s2[1] = (sd * vx2c) * -sr2; s2[2] = (sd * vy2c) * -sr2;
+        if constexpr (NV == 4) {
+            const double e0 = ((u[3] - 0.5 * sd * (vx * vx + vy * vy)) * k.gm1) * (id * k.inv_gm1);
+            s1[3] = (sd * e0 + 0.5 * sd * (vx1c * vx1c + vy1c * vy1c)) * -sr1;
+            s2[3] = (sd * e0 + 0.5 * sd * (vx2c * vx2c + vy2c * vy2c)) * -sr2;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        it[CDC_ITEM_SINK1][v] = s1[v] * dt;
+        it[CDC_ITEM_SINK2][v] = s2[v] * dt;
+        it[CDC_ITEM_BUFFER][v] = (u[v] - u0[v]) * (-dt * brate);
+    }
+    it[CDC_ITEM_GRAV1][0] = 0.0; it[CDC_ITEM_GRAV1][1] = fx1 * dt; it[CDC_ITEM_GRAV1][2] = fy1 * dt;
+    it[CDC_ITEM_GRAV2][0] = 0.0; it[CDC_ITEM_GRAV2][1] = fx2 * dt; it[CDC_ITEM_GRAV2][2] = fy2 * dt;
+
+    if constexpr (NV == 3) {
+#pragma unroll
+        for (int v = 0; v < 3; ++v) it[CDC_ITEM_COOLING][v] = 0.0;
+        if (u[0] < u0[0] * k.fake_threshold) {
+#pragma unroll
+            for (int v = 0; v < 3; ++v) it[CDC_ITEM_FAKE_MASS][v] = 0.0;
+        } else {
+#pragma unroll
+            for (int v = 0; v < 3; ++v) it[CDC_ITEM_FAKE_MASS][v] = u0[v] * k.fake_rate * dt;
+        }
+    } else {
+        it[CDC_ITEM_GRAV1][3] = (fx1 * vx + fy1 * vy) * dt;
+        it[CDC_ITEM_GRAV2][3] = (fx2 * vx + fy2 * vy) * dt;
+        const double p0 = (u[3] - 0.5 * sd * (vx * vx + vy * vy)) * k.gm1;
+        const double e0 = p0 * (id * k.inv_gm1);
+        double ec;
+        if (k.cooling_model == CDC_COOLING_BETA) {
+            const double r2 = x * x + y * y + k.rs2;
+            const double omega = sqrt(k.mtot / (r2 * sqrt(r2)));
+            const double b0 = u0[0], ib0 = 1.0 / b0;
+            const double bvx = u0[1] * ib0, bvy = u0[2] * ib0;
+            const double bp = (u0[3] - 0.5 * b0 * (bvx * bvx + bvy * bvy)) * k.gm1;
+            const double et = bp * (ib0 * k.inv_gm1);
+            ec = et + (e0 - et) * exp(-dt * omega * k.cooling_inv_beta);
+        } else {
+            const double sdi = k.density_index == 1 ? sd : (k.density_index == 2 ? sd * sd : pow(sd, (double)k.density_index));
+            const double a = k.cooling_coefficient / sdi * k.cooling_f;
+            ec = e0 * pow(1.0 + 3.0 * a * (e0 * e0 * e0) * dt, -1.0 / 3.0);
+        }
+        if (k.has_mach_ceiling) ec = fmax(ec, 0.5 * (vx * vx + vy * vy) * k.mach_ceiling_coef);
+        // uc = to_conserved(Sigma, vx, vy, Sigma ec (gamma - 1)); cooling = uc - u
+        const double pc = sd * ec * k.gm1;
+        it[CDC_ITEM_COOLING][0] = sd - u[0];
+        it[CDC_ITEM_COOLING][1] = sd * vx - u[1];
+        it[CDC_ITEM_COOLING][2] = sd * vy - u[2];
+        it[CDC_ITEM_COOLING][3] = (0.5 * sd * (vx * vx + vy * vy) + pc * k.inv_gm1) - u[3];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) it[CDC_ITEM_FAKE_MASS][v] = 0.0;
+    }
+}
+
+}  // namespace dev
+
+// ---------------------------------------------------------------------------
+
+template <int NV, int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) stage_kernel(const StageArgs A) {
+    constexpr int EX = TX + 4, EY = TY + 4, EXY = EX * EY;
+    constexpr int RX = TX + 2, RY = TY + 2, RXY = RX * RY;
+    constexpr int NFX = (TX + 1) * TY, NFY = TX * (TY + 1), NF = NFX + NFY;
+    constexpr int KF = (NF + NT - 1) / NT;
+    constexpr int CPT = TX * TY / NT;
+    constexpr int NS = SumMap<NV>::n, NR = NS + 1;
+    constexpr int CH = 16, CL = NT / CH;
+    constexpr int LDS_A = NV * EXY + 2 * NV * RXY;
+    constexpr int LDS_R = NR * (NT + CH);
+    constexpr int LDS = LDS_A > LDS_R ? LDS_A : LDS_R;
+    static_assert(NV * NF <= 2 * NV * RXY, "flux staging must fit the gradient region");
+    static_assert((TX * TY) % NT == 0 && NT % CH == 0, "tile / thread mismatch");
+    __shared__ double lds[LDS];
+    __shared__ int s_key;
+
+    const int ef = *(volatile const int32_t *)A.err_flag;
+    if (ef != 0 && ef < A.stage_id) return;
+
+    const int tid = threadIdx.x;
+    const int wg = blockIdx.x;
+    const int tile = (A.ntiles % 8 == 0) ? (wg % 8) * (A.ntiles / 8) + wg / 8 : wg;
+    const int bs = A.bs, N = A.N;
+    const int tpby = bs / TY, tpb = (bs / TX) * tpby;
+    const int b = tile / tpb, lt = tile - b * tpb;
+    const int bi = b / A.nb, bj = b - (b / A.nb) * A.nb;
+    const int li0 = (lt / tpby) * TX, lj0 = (lt - (lt / tpby) * tpby) * TY;
+    const int gi0 = bi * bs + li0, gj0 = bj * bs + lj0;
+    const size_t NN = (size_t)N * N;
+    const dev::Axis axx = dev::block_axis(A.R, A.L, bi, bs);
+    const dev::Axis axy = dev::block_axis(A.R, A.L, bj, bs);
+
+    double *P = lds;
+    double *GX = lds + NV * EXY;
+    double *GY = GX + NV * RXY;
+
+    // ---- phase 1: primitives on the tile + 2-zone halo --------------------
+    int key = INT_MAX, kcode = 0;
+    double kval = 0.0;
+    if (tid == 0) s_key = INT_MAX;
+    for (int e = tid; e < EXY; e += NT) {
+        const int ei = e / EY, ej = e - (e / EY) * EY;
+        int gi = gi0 - 2 + ei, gj = gj0 - 2 + ej;
+        gi = gi < 0 ? gi + N : (gi >= N ? gi - N : gi);
+        gj = gj < 0 ? gj + N : (gj >= N ? gj - N : gj);
+        const size_t g = (size_t)gi * N + gj;
+        double u[NV], p[NV], val = 0.0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) u[v] = A.uc[v * NN + g];
+        const int code = dev::try_to_primitive<NV>(A.k, u, p, val);
+        if (code && ei >= 2 && ei < TX + 2 && ej >= 2 && ej < TY + 2) {
+            const int c = (li0 + ei - 2) * bs + (lj0 + ej - 2);
+            if (c < key) { key = c; kcode = code; kval = val; }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) P[v * EXY + e] = p[v];
+    }
+    __syncthreads();
+
+    // ---- phase 2: PLM gradients on the 1-zone ring --------------------------
+    for (int r = tid; r < RXY; r += NT) {
+        const int ri = r / RY, rj = r - (r / RY) * RY;
+        const int ec = (ri + 1) * EY + (rj + 1);
+        double thx = A.k.plm, thy = A.k.plm;
+        if (A.k.has_pdt) {
+            const double pdt = A.k.pdt;
+            if (P[ec - EY] < pdt || P[ec] < pdt || P[ec + EY] < pdt) thx = 0.0;
+            if (P[ec - 1] < pdt || P[ec] < pdt || P[ec + 1] < pdt) thy = 0.0;
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double *q = P + v * EXY + ec;
+            GX[v * RXY + r] = dev::plm(thx, q[-EY], q[0], q[EY]);
+            GY[v * RXY + r] = dev::plm(thy, q[-1], q[0], q[1]);
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 3: face fluxes ----------------------------------------------
+    double fl[KF][NV];
+#pragma unroll
+    for (int q = 0; q < KF; ++q) {
+        const int f = tid + q * NT;
+        if (f < NFX) {
+            const int fi = f / TY, fj = f - (f / TY) * TY;
+            const int eL = (fi + 1) * EY + fj + 2;
+            const int rL = fi * RY + fj + 1;
+            dev::face_flux<NV, EXY, RXY>(A, 0, P, GX, GY, eL, eL + EY, rL, rL + RY,
+                                         axx.v(li0 + fi), axy.c(lj0 + fj), fl[q]);
+        } else if (f < NF) {
+            const int g = f - NFX;
+            const int fi = g / (TY + 1), fj = g - (g / (TY + 1)) * (TY + 1);
+            const int eL = (fi + 2) * EY + fj + 1;
+            const int rL = (fi + 1) * RY + fj;
+            dev::face_flux<NV, EXY, RXY>(A, 1, P, GX, GY, eL, eL + 1, rL, rL + 1,
+                                         axx.c(li0 + fi), axy.v(lj0 + fj), fl[q]);
+        }
+    }
+    __syncthreads();
+    double *FX = GX;
+    double *FY = GX + NV * NFX;
+#pragma unroll
+    for (int q = 0; q < KF; ++q) {
+        const int f = tid + q * NT;
+        if (f < NFX) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) FX[v * NFX + f] = fl[q][v];
+        } else if (f < NF) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) FY[v * NFY + f - NFX] = fl[q][v];
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 4: update, sources, RK combine, tile sums ---------------------
+    double acc[NR];
+#pragma unroll
+    for (int s = 0; s < NR; ++s) acc[s] = 0.0;
+    const double ndt = -A.dt;
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int c = tid + q * NT;
+        const int ci = c / TY, cj = c - (c / TY) * TY;
+        const size_t g = (size_t)(gi0 + ci) * N + (gj0 + cj);
+        double uc[NV], u0[NV], ub[NV], du[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            uc[v] = A.uc[v * NN + g];
+            u0[v] = A.u0[v * NN + g];
+            ub[v] = A.read_base ? A.ub[v * NN + g] : 0.0;
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double *fx = FX + v * NFX + ci * TY + cj;
+            const double *fy = FY + v * NFY + ci * (TY + 1) + cj;
+            du[v] = ((fx[TY] - fx[0]) * A.k.idx + (fy[1] - fy[0]) * A.k.idy) * ndt;
+        }
+        double it[7][NV];
+        dev::cell_sources<NV>(A, uc, u0, axx.c(li0 + ci), axy.c(lj0 + cj), it);
+        double out[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double total = it[CDC_ITEM_SINK1][v] + it[CDC_ITEM_SINK2][v] + it[CDC_ITEM_GRAV1][v] +
+                           it[CDC_ITEM_GRAV2][v] + it[CDC_ITEM_BUFFER][v];
+            if constexpr (NV == 4) total = total + it[CDC_ITEM_COOLING][v];
+            else total = total + it[CDC_ITEM_FAKE_MASS][v];
+            const double u1 = uc[v] + du[v] + total;
+            out[v] = A.read_base ? u1 * A.c1 + ub[v] * A.c0 : u1;
+            A.uo[v * NN + g] = out[v];
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) acc[s] += it[SumMap<NV>::item(s)][SumMap<NV>::comp(s)];
+        if (A.want_amax) acc[NS] = fmax(acc[NS], dev::max_signal_speed<NV>(out, A.k.gm));
+    }
+
+    // ---- deterministic tile reductions --------------------------------------
+    if (key != INT_MAX) atomicMin(&s_key, key);
+    __syncthreads();
+    double *red = lds;
+    double *red2 = lds + NR * NT;
+#pragma unroll
+    for (int s = 0; s < NR; ++s) red[s * NT + tid] = acc[s];
+    __syncthreads();
+    for (int w = tid; w < NR * CH; w += NT) {
+        const int s = w / CH, c = w - (w / CH) * CH;
+        const double *q = red + s * NT + c * CL;
+        double t = q[0];
+        if (s == NS) {
+            for (int j = 1; j < CL; ++j) t = fmax(t, q[j]);
+        } else {
+            for (int j = 1; j < CL; ++j) t += q[j];
+        }
+        red2[s * CH + c] = t;
+    }
+    __syncthreads();
+    for (int s = tid; s < NR; s += NT) {
+        const double *q = red2 + s * CH;
+        double t = q[0];
+        if (s == NS) {
+            for (int j = 1; j < CH; ++j) t = fmax(t, q[j]);
+        } else {
+            for (int j = 1; j < CH; ++j) t += q[j];
+        }
+        A.tile_sums[(size_t)tile * NR + s] = t;
+    }
+    const int sk = s_key;
+    if (sk == INT_MAX) {
+        if (tid == 0) A.tile_err[tile].code = 0;
+    } else if (key == sk) {
+        TileErr te;
+        te.code = kcode;
+        te.cell = key;
+        te.value = kval;
+        A.tile_err[tile] = te;
+        atomicCAS(A.err_flag, 0, A.stage_id);
+    }
+}
+
+// One thread per block: fixed-order tile sum, dx*dy, perturbation, RK combine
+// of integrated_source_terms and orbital_elements_change
+// (src/scheme.rs:307-315, src/state.rs:195-197).
+template <int NV>
+__global__ void __launch_bounds__(64) finalize_kernel(const FinalizeArgs F) {
+    constexpr int NS = SumMap<NV>::n, NR = NS + 1;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= F.nblk) return;
+    const int ef = *(volatile const int32_t *)F.err_flag;
+    if (ef != 0 && ef < F.stage_id) return;
+    double cs[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) cs[s] = 0.0;
+    for (int t = 0; t < F.tiles_per_block; ++t) {
+        const double *q = F.tile_sums + ((size_t)b * F.tiles_per_block + t) * NR;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) cs[s] += q[s];
+    }
+    double ds[7 * NV];
+#pragma unroll
+    for (int k = 0; k < 7 * NV; ++k) ds[k] = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) ds[SumMap<NV>::item(s) * NV + SumMap<NV>::comp(s)] = cs[s] * F.area;
+    double de[7 * 4];
+    if (!perturbation(F.orb, F.el, ds, NV, de)) {
+        F.perr[b] = CDC_ERR_UNBOUND_ORBIT;
+        atomicCAS(F.err_flag, 0, F.stage_id);
+        return;
+    }
+    F.perr[b] = 0;
+    const size_t oi = (size_t)b * 7 * NV, oe = (size_t)b * 7 * 4;
+    for (int k = 0; k < 7 * NV; ++k) {
+        const double s1 = F.ist_c[oi + k] + ds[k];
+        F.ist_o[oi + k] = F.read_base ? s1 * F.c1 + F.ist_b[oi + k] * F.c0 : s1;
+    }
+    for (int k = 0; k < 7 * 4; ++k) {
+        const double s1 = F.oec_c[oe + k] + de[k];
+        F.oec_o[oe + k] = F.read_base ? s1 * F.c1 + F.oec_b[oe + k] * F.c0 : s1;
+    }
+}
+
+template <int NV>
+__global__ void __launch_bounds__(256) max_speed_kernel(const double *u, int64_t ncell, double gm,
+                                                        double *partial) {
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * 256) {
+        double w[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) w[v] = u[v * ncell + c];
+        m = fmax(m, dev::max_signal_speed<NV>(w, gm));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Boundary layout [block][i][j][v] <-> device layout [v][gi][gj]
+__global__ void __launch_bounds__(256) aos_to_soa_kernel(int nv, int nb, int bs, const double *aos,
+                                                         double *soa) {
+    const int64_t N = (int64_t)nb * bs, NN = N * N;
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= NN) return;
+    const int64_t bsq = (int64_t)bs * bs;
+    const int64_t b = c / bsq, l = c - b * bsq;
+    const int64_t i = l / bs, j = l - i * bs;
+    const int64_t gi = (b / nb) * bs + i, gj = (b % nb) * bs + j;
+    for (int v = 0; v < nv; ++v) soa[v * NN + gi * N + gj] = aos[c * nv + v];
+}
+
+__global__ void __launch_bounds__(256) soa_to_aos_kernel(int nv, int nb, int bs, const double *soa,
+                                                         double *aos) {
+    const int64_t N = (int64_t)nb * bs, NN = N * N;
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= NN) return;
+    const int64_t bsq = (int64_t)bs * bs;
+    const int64_t b = c / bsq, l = c - b * bsq;
+    const int64_t i = l / bs, j = l - i * bs;
+    const int64_t gi = (b / nb) * bs + i, gj = (b % nb) * bs + j;
+    for (int v = 0; v < nv; ++v) aos[c * nv + v] = soa[v * NN + gi * N + gj];
+}
+
+__global__ void set_int_kernel(int32_t *p, int32_t v) { *p = v; }
+
+// ---------------------------------------------------------------------------
+// Host launchers
+
+TileShape tile_shape_for(int bs, int nv) {
+    (void)nv;
+    if (bs % 32 == 0) return {16, 32, 256};
+    if (bs % 16 == 0) return {16, 16, 256};
+    return {8, 8, 64};
+}
+
+template <int NV, int TX, int TY, int NT>
+static hipError_t launch_stage_t(const StageArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL((stage_kernel<NV, TX, TY, NT>), dim3(a.ntiles), dim3(NT), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage(int nv, const TileShape &ts, const StageArgs &a, hipStream_t s) {
+    if (nv == 3) {
+        if (ts.tx == 16 && ts.ty == 32) return launch_stage_t<3, 16, 32, 256>(a, s);
+        if (ts.tx == 16 && ts.ty == 16) return launch_stage_t<3, 16, 16, 256>(a, s);
+        return launch_stage_t<3, 8, 8, 64>(a, s);
+    }
+    if (ts.tx == 16 && ts.ty == 32) return launch_stage_t<4, 16, 32, 256>(a, s);
+    if (ts.tx == 16 && ts.ty == 16) return launch_stage_t<4, 16, 16, 256>(a, s);
+    return launch_stage_t<4, 8, 8, 64>(a, s);
+}
+
+hipError_t launch_finalize(int nv, const FinalizeArgs &a, hipStream_t s) {
+    const int grid = (a.nblk + 63) / 64;
+    if (nv == 3) hipLaunchKernelGGL(finalize_kernel<3>, dim3(grid), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL(finalize_kernel<4>, dim3(grid), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_max_speed(int nv, const double *u, int64_t ncell, double gm, double *partial,
+                            int nparts, hipStream_t s) {
+    if (nv == 3) hipLaunchKernelGGL(max_speed_kernel<3>, dim3(nparts), dim3(256), 0, s, u, ncell, gm, partial);
+    else hipLaunchKernelGGL(max_speed_kernel<4>, dim3(nparts), dim3(256), 0, s, u, ncell, gm, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_aos_to_soa(int nv, int nb, int bs, const double *aos, double *soa, hipStream_t s) {
+    const int64_t n = (int64_t)nb * bs * nb * bs;
+    hipLaunchKernelGGL(aos_to_soa_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nv, nb, bs, aos, soa);
+    return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos(int nv, int nb, int bs, const double *soa, double *aos, hipStream_t s) {
+    const int64_t n = (int64_t)nb * bs * nb * bs;
+    hipLaunchKernelGGL(soa_to_aos_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nv, nb, bs, soa, aos);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_int(int32_t *p, int32_t v, hipStream_t s) {
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, p, v);
+    return hipGetLastError();
+}
+
+}  // namespace cdc

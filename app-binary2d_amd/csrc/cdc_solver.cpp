@@ -1,0 +1,671 @@
+// cdc_solver.cpp -- host side of the C ABI (include/cdc_solver.h).
+//
+// Mirrors scheme::advance (src/scheme.rs:323-349) and the RK driver it runs:
+// dt from the global max signal speed, then `fold` steps of RK1/2/3 where
+// each stage is one fused HIP kernel (try_advance_rk, src/scheme.rs:88-173,
+// with the RK combine of src/state.rs:183-219 folded in) followed by a small
+// per-block finalize kernel.  Stage times and the Rational64 iteration are
+// carried on the host exactly as State::weighted_average computes them, and
+// the binary's orbital state for each stage is evaluated on the host.
+//
+// Transactional like the reference's clone-and-replace (src/main.rs:159-160):
+// an advance works only in scratch buffers and commits by swapping an index,
+// so a failing call leaves the committed state untouched.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/cdc_solver.h"
+#include "cdc_internal.h"
+
+namespace {
+
+thread_local std::string g_msg;
+
+int set_msg(int code, const std::string &m) {
+    g_msg = m;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return set_msg(CDC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int NBUF = 4;
+constexpr double ORBITAL_PERIOD = 2.0 * 3.14159265358979323846;
+
+int64_t gcd64(int64_t a, int64_t b) {
+    if (a < 0) a = -a;
+    if (b < 0) b = -b;
+    while (b) { int64_t t = a % b; a = b; b = t; }
+    return a ? a : 1;
+}
+
+void rat_norm(int64_t &n, int64_t &d) {
+    int64_t g = gcd64(n, d);
+    n /= g; d /= g;
+    if (d < 0) { n = -n; d = -d; }
+}
+
+// Rational64: s1 * (1 - b) + s0 * b (src/state.rs:215)
+void rat_wavg(int64_t n1, int64_t d1, int64_t n0, int64_t d0, int64_t bn, int64_t bd, int64_t &on,
+              int64_t &od) {
+    int64_t an = (bd - bn) * n1, ad = bd * d1;
+    int64_t cn = bn * n0, cd = bd * d0;
+    rat_norm(an, ad);
+    rat_norm(cn, cd);
+    on = an * cd + cn * ad;
+    od = ad * cd;
+    rat_norm(on, od);
+}
+
+double softening_length(const cdc_physics_t &p) {
+    return p.has_softening_length ? p.softening_length : p.sink_radius;  // src/physics.rs:399-405
+}
+
+}  // namespace
+
+struct cdc_solver {
+    cdc_mesh_t mesh;
+    cdc_hydro_t hydro;
+    cdc_physics_t phys;
+    int nv, nb, bs, N, nblk, ntiles, tpb, nsum;
+    cdc::TileShape ts;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    double *d_init = nullptr;
+    double *d_u[NBUF] = {};
+    double *d_ist[NBUF] = {};
+    double *d_oec[NBUF] = {};
+    double *d_tile_sums = nullptr;
+    cdc::TileErr *d_tile_err = nullptr;
+    int32_t *d_perr = nullptr;
+    int32_t *d_err_flag = nullptr;
+    double *d_stage = nullptr;   // AoS staging
+    double *d_amax = nullptr;    // max-speed partials
+    int namax = 0;
+
+    int cur = 0;                 // committed buffer
+    double time = 0.0;
+    int64_t num = 0, den = 1;
+    bool amax_valid = false;     // tile_sums[:, nsum] hold the committed state's max speed
+
+    bool timing = false;
+    int64_t launches = 0;
+    double total_ms = 0.0;
+    std::vector<hipEvent_t> ev;
+
+    size_t ncell() const { return (size_t)N * N; }
+    size_t nstate() const { return ncell() * nv; }
+};
+
+namespace {
+
+int validate(const cdc_mesh_t *m, const cdc_hydro_t *h, const cdc_physics_t *p) {
+    if (!m || !h || !p) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null configuration pointer");
+    if (m->num_blocks < 1) return set_msg(CDC_ERR_INVALID_ARGUMENT, "num_blocks must be >= 1");
+    if (m->block_size < 8 || m->block_size % 8 != 0)
+        return set_msg(CDC_ERR_INVALID_ARGUMENT, "block_size must be a positive multiple of 8");
+    if ((int64_t)m->num_blocks * m->block_size > 65536)
+        return set_msg(CDC_ERR_INVALID_ARGUMENT, "num_blocks * block_size must be <= 65536");
+    if (!(m->domain_radius > 0.0)) return set_msg(CDC_ERR_INVALID_ARGUMENT, "domain_radius must be > 0");
+    if (h->kind != CDC_ISOTHERMAL && h->kind != CDC_EULER)
+        return set_msg(CDC_ERR_INVALID_ARGUMENT, "hydro.kind must be isothermal or euler");
+    if (h->kind == CDC_EULER && !(h->gamma_law_index > 1.0))
+        return set_msg(CDC_ERR_INVALID_ARGUMENT, "gamma_law_index must be > 1");
+    if (h->kind == CDC_EULER && h->cooling_model == CDC_COOLING_BETA && !(h->cooling_beta > 0.0))
+        return set_msg(CDC_ERR_INVALID_ARGUMENT, "cooling_beta must be > 0");
+    if (p->rk_order < 1 || p->rk_order > 3) return set_msg(CDC_ERR_INVALID_ARGUMENT, "rk_order must be 1, 2 or 3");
+    if (p->viscosity_model != CDC_VISC_CONSTANT_NU && p->viscosity_model != CDC_VISC_ALPHA)
+        return set_msg(CDC_ERR_INVALID_ARGUMENT, "unknown viscosity model");
+    if (p->sink_model != CDC_SINK_IMPARTIAL && p->sink_model != CDC_SINK_TORQUE_FREE)
+        return set_msg(CDC_ERR_INVALID_ARGUMENT, "unknown sink model");
+    return CDC_OK;
+}
+
+cdc::StageConsts make_consts(const cdc_solver *s, double time, double dt) {
+    const cdc_physics_t &p = s->phys;
+    const cdc_hydro_t &h = s->hydro;
+    cdc::StageConsts k;
+    memset(&k, 0, sizeof(k));
+    const double L = 2.0 * s->mesh.domain_radius / (double)s->nb;
+    k.dx = L / (double)s->bs;
+    k.dy = k.dx;
+    k.idx = 1.0 / k.dx;
+    k.idy = 1.0 / k.dy;
+    const double rs = softening_length(p);
+    k.rs2 = rs * rs;
+    k.inv_mach2 = h.kind == CDC_ISOTHERMAL ? 1.0 / (h.mach_number * h.mach_number) : 0.0;
+    const double gm = h.kind == CDC_EULER ? h.gamma_law_index : 1.0;
+    k.gm = gm;
+    k.gm1 = gm - 1.0;
+    k.inv_gm1 = h.kind == CDC_EULER ? 1.0 / (gm - 1.0) : 0.0;
+    k.plm = p.plm;
+    k.has_pdt = p.has_plm_density_trigger;
+    k.pdt = p.has_plm_density_trigger ? p.plm_density_trigger : 0.0;
+    k.viscosity_model = p.viscosity_model;
+    k.visc = p.viscosity_model == CDC_VISC_ALPHA ? p.viscosity / sqrt(gm) : p.viscosity;
+    k.lam = p.lambda_multiplier;
+    k.eps_dx = 1e-12 * k.dx;
+    double el[4];
+    cdc::orbital_elements(p, el);
+    const double R = s->mesh.domain_radius;
+    k.mtot = el[1];
+    k.domain_radius = R;
+    k.inv_buffer_scale = 1.0 / p.buffer_scale;
+    k.buffer_coef = 0.5 * p.buffer_rate * sqrt(el[1] / (R * R * R));
+    k.sink_s2 = p.sink_radius * p.sink_radius;
+    k.inv_sink_s2 = 1.0 / k.sink_s2;
+    k.sink_rate = p.sink_rate;
+    k.fake_threshold = p.fake_mass_threshold;
+    k.fake_rate = p.fake_mass_rate;
+    k.cooling_coefficient = h.cooling_coefficient;
+    k.cooling_inv_beta = h.cooling_model == CDC_COOLING_BETA ? 1.0 / h.cooling_beta : 0.0;
+    k.cooling_f = h.cooling_slow_start ? exp(-ORBITAL_PERIOD / (time + dt)) : 1.0;
+    k.has_mach_ceiling = p.has_mach_ceiling;
+    k.mach_ceiling_coef = p.has_mach_ceiling ? 2.0 / gm / (gm - 1.0) / (p.mach_ceiling * p.mach_ceiling) : 0.0;
+    k.has_density_floor = h.has_density_floor;
+    k.density_floor = h.density_floor;
+    k.density_floor_pressure = (h.kind == CDC_EULER && h.has_density_floor) ? pow(h.density_floor, gm) : 0.0;
+    k.has_pressure_floor = h.kind == CDC_EULER ? h.has_pressure_floor : 0;
+    k.pressure_floor = h.pressure_floor;
+    k.density_index = h.density_index;
+    k.cooling_slow_start = h.cooling_slow_start;
+    k.cooling_model = h.cooling_model;
+    k.sink_model = p.sink_model;
+    k.one_body = p.one_body;
+    return k;
+}
+
+void free_all(cdc_solver *s) {
+    if (!s) return;
+    (void)hipFree(s->d_init);
+    for (int i = 0; i < NBUF; ++i) {
+        (void)hipFree(s->d_u[i]);
+        (void)hipFree(s->d_ist[i]);
+        (void)hipFree(s->d_oec[i]);
+    }
+    (void)hipFree(s->d_tile_sums);
+    (void)hipFree(s->d_tile_err);
+    (void)hipFree(s->d_perr);
+    (void)hipFree(s->d_err_flag);
+    (void)hipFree(s->d_stage);
+    (void)hipFree(s->d_amax);
+    for (auto e : s->ev) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+// Host-side cell centre, as Mesh::cell_centers (src/mesh.rs:67-72)
+void cell_centre(const cdc_solver *s, int b, int cell, double *x, double *y) {
+    const double R = s->mesh.domain_radius;
+    const double L = 2.0 * R / (double)s->nb;
+    const int bi = b / s->nb, bj = b % s->nb;
+    const int i = cell / s->bs, j = cell % s->bs;
+    auto axis = [&](int blk, int k) {
+        const double a = -R + (double)blk * L;
+        const double e = a + L;
+        const double step = (e - a) / (double)s->bs;
+        const double v0 = a + step * (double)k;
+        const double v1 = a + step * (double)(k + 1);
+        return (v0 + v1) / 2.0;
+    };
+    *x = axis(bi, i);
+    *y = axis(bj, j);
+}
+
+int upload_conserved(cdc_solver *s, const double *aos, double *soa) {
+    HIP_TRY(hipMemcpyAsync(s->d_stage, aos, s->nstate() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(cdc::launch_aos_to_soa(s->nv, s->nb, s->bs, s->d_stage, soa, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return CDC_OK;
+}
+
+// The state that one RK stage reads or writes: buffer index plus the time
+// and iteration carried with it.
+struct Slot {
+    int idx;
+    double time;
+    int64_t num, den;
+};
+
+struct StageRec {
+    double time;      // stage input time (orbital state of the failing stage)
+};
+
+}  // namespace
+
+extern "C" {
+
+int32_t cdc_abi_version(void) { return CDC_ABI_VERSION; }
+
+int32_t cdc_struct_size(int32_t which) {
+    switch (which) {
+        case 0: return (int32_t)sizeof(cdc_mesh_t);
+        case 1: return (int32_t)sizeof(cdc_hydro_t);
+        case 2: return (int32_t)sizeof(cdc_physics_t);
+        case 3: return (int32_t)sizeof(cdc_error_t);
+        case 4: return (int32_t)sizeof(cdc_cavity_model_t);
+        default: return -1;
+    }
+}
+
+int32_t cdc_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+const char *cdc_error_string(int32_t code) {
+    switch (code) {
+        case CDC_OK: return "ok";
+        case CDC_ERR_NEGATIVE_DENSITY: return "negative surface density";
+        case CDC_ERR_NEGATIVE_PRESSURE: return "negative gas pressure";
+        case CDC_ERR_UNBOUND_ORBIT: return "unbound orbital state";
+        case CDC_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case CDC_ERR_HIP: return "HIP runtime error";
+        case CDC_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case CDC_ERR_NO_DEVICE: return "no HIP device";
+        default: return "unknown error";
+    }
+}
+
+const char *cdc_last_error_message(void) { return g_msg.c_str(); }
+
+int32_t cdc_orbital_state_from_time(const cdc_physics_t *physics, double time, cdc_point_mass_t out[2]) {
+    if (!physics || !out) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null pointer");
+    double el[4];
+    cdc::orbital_elements(*physics, el);
+    cdc::Orbit o = cdc::orbital_state(el, time);
+    out[0] = {o.m1, o.x1, o.y1, o.vx1, o.vy1};
+    out[1] = {o.m2, o.x2, o.y2, o.vx2, o.vy2};
+    return CDC_OK;
+}
+
+int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, const cdc_physics_t *physics,
+                          const double *initial_conserved, cdc_solver_t **out) {
+    if (!out) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    int rc = validate(mesh, hydro, physics);
+    if (rc) return rc;
+    if (!initial_conserved) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null initial_conserved");
+    if (cdc_device_count() < 1) return set_msg(CDC_ERR_NO_DEVICE, "no HIP device visible");
+
+    cdc_solver *s = new cdc_solver();
+    s->mesh = *mesh;
+    s->hydro = *hydro;
+    s->phys = *physics;
+    s->nv = hydro->kind == CDC_EULER ? 4 : 3;
+    s->nb = mesh->num_blocks;
+    s->bs = mesh->block_size;
+    s->N = s->nb * s->bs;
+    s->nblk = s->nb * s->nb;
+    s->ts = cdc::tile_shape_for(s->bs, s->nv);
+    s->tpb = (s->bs / s->ts.tx) * (s->bs / s->ts.ty);
+    s->ntiles = s->nblk * s->tpb;
+    s->nsum = cdc::nsum_for(s->nv);
+    (void)hipGetDevice(&s->device);
+
+    auto fail = [&](int code, const std::string &m) {
+        free_all(s);
+        delete s;
+        return set_msg(code, m);
+    };
+#define ALLOC(ptr, bytes)                                                                   \
+    do {                                                                                    \
+        if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess) {                            \
+            (void)hipGetLastError();                                                        \
+            return fail(CDC_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes"); \
+        }                                                                                   \
+    } while (0)
+
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(CDC_ERR_HIP, "hipStreamCreate failed");
+    const size_t sb = s->nstate() * sizeof(double);
+    ALLOC(s->d_init, sb);
+    ALLOC(s->d_stage, sb);
+    for (int i = 0; i < NBUF; ++i) {
+        ALLOC(s->d_u[i], sb);
+        ALLOC(s->d_ist[i], (size_t)s->nblk * 7 * s->nv * sizeof(double));
+        ALLOC(s->d_oec[i], (size_t)s->nblk * 7 * 4 * sizeof(double));
+    }
+    ALLOC(s->d_tile_sums, (size_t)s->ntiles * (s->nsum + 1) * sizeof(double));
+    ALLOC(s->d_tile_err, (size_t)s->ntiles * sizeof(cdc::TileErr));
+    ALLOC(s->d_perr, (size_t)s->nblk * sizeof(int32_t));
+    ALLOC(s->d_err_flag, sizeof(int32_t));
+    s->namax = 1024;
+    ALLOC(s->d_amax, (size_t)s->namax * sizeof(double));
+#undef ALLOC
+
+    rc = upload_conserved(s, initial_conserved, s->d_init);
+    if (rc) return fail(rc, g_msg);
+    if (hipMemcpyAsync(s->d_u[0], s->d_init, sb, hipMemcpyDeviceToDevice, s->stream) != hipSuccess ||
+        hipMemsetAsync(s->d_ist[0], 0, (size_t)s->nblk * 7 * s->nv * sizeof(double), s->stream) != hipSuccess ||
+        hipMemsetAsync(s->d_oec[0], 0, (size_t)s->nblk * 7 * 4 * sizeof(double), s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return fail(CDC_ERR_HIP, "initial state upload failed");
+    s->cur = 0;
+    s->time = 0.0;
+    s->num = 0;
+    s->den = 1;
+    *out = s;
+    return CDC_OK;
+}
+
+void cdc_solver_destroy(cdc_solver_t *s) {
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_all(s);
+    delete s;
+}
+
+int32_t cdc_solver_set_physics(cdc_solver_t *s, const cdc_physics_t *physics) {
+    if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
+    int rc = validate(&s->mesh, &s->hydro, physics);
+    if (rc) return rc;
+    s->phys = *physics;
+    return CDC_OK;
+}
+
+int32_t cdc_solver_set_state(cdc_solver_t *s, double time, int64_t num, int64_t den, const double *conserved,
+                             const double *ist, const double *oec) {
+    if (!s || !conserved) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null pointer");
+    if (den == 0) return set_msg(CDC_ERR_INVALID_ARGUMENT, "iteration denominator is zero");
+    int rc = upload_conserved(s, conserved, s->d_u[s->cur]);
+    if (rc) return rc;
+    const size_t ni = (size_t)s->nblk * 7 * s->nv * sizeof(double);
+    const size_t ne = (size_t)s->nblk * 7 * 4 * sizeof(double);
+    if (ist) HIP_TRY(hipMemcpyAsync(s->d_ist[s->cur], ist, ni, hipMemcpyHostToDevice, s->stream));
+    else HIP_TRY(hipMemsetAsync(s->d_ist[s->cur], 0, ni, s->stream));
+    if (oec) HIP_TRY(hipMemcpyAsync(s->d_oec[s->cur], oec, ne, hipMemcpyHostToDevice, s->stream));
+    else HIP_TRY(hipMemsetAsync(s->d_oec[s->cur], 0, ne, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->time = time;
+    s->num = num;
+    s->den = den;
+    rat_norm(s->num, s->den);
+    s->amax_valid = false;
+    return CDC_OK;
+}
+
+int32_t cdc_solver_get_state(cdc_solver_t *s, double *time, int64_t *num, int64_t *den, double *conserved,
+                             double *ist, double *oec) {
+    if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
+    if (time) *time = s->time;
+    if (num) *num = s->num;
+    if (den) *den = s->den;
+    if (conserved) {
+        HIP_TRY(cdc::launch_soa_to_aos(s->nv, s->nb, s->bs, s->d_u[s->cur], s->d_stage, s->stream));
+        HIP_TRY(hipMemcpyAsync(conserved, s->d_stage, s->nstate() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    }
+    if (ist)
+        HIP_TRY(hipMemcpyAsync(ist, s->d_ist[s->cur], (size_t)s->nblk * 7 * s->nv * sizeof(double),
+                               hipMemcpyDeviceToHost, s->stream));
+    if (oec)
+        HIP_TRY(hipMemcpyAsync(oec, s->d_oec[s->cur], (size_t)s->nblk * 7 * 4 * sizeof(double),
+                               hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return CDC_OK;
+}
+
+static int compute_amax(cdc_solver *s, double *out) {
+    std::vector<double> h;
+    if (s->amax_valid) {
+        // max signal speed of the committed state, left by the last stage
+        const int nr = s->nsum + 1;
+        std::vector<double> sums((size_t)s->ntiles * nr);
+        HIP_TRY(hipMemcpyAsync(sums.data(), s->d_tile_sums, sums.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        double a = 0.0;
+        for (int t = 0; t < s->ntiles; ++t) a = std::max(sums[(size_t)t * nr + s->nsum], a);
+        *out = a;
+        return CDC_OK;
+    }
+    HIP_TRY(cdc::launch_max_speed(s->nv, s->d_u[s->cur], (int64_t)s->ncell(),
+                                  s->hydro.kind == CDC_EULER ? s->hydro.gamma_law_index : 1.0, s->d_amax, s->namax,
+                                  s->stream));
+    h.resize(s->namax);
+    HIP_TRY(hipMemcpyAsync(h.data(), s->d_amax, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    double a = 0.0;
+    for (double v : h) a = std::max(v, a);
+    *out = a;
+    return CDC_OK;
+}
+
+int32_t cdc_solver_max_signal_speed(cdc_solver_t *s, double *out) {
+    if (!s || !out) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null pointer");
+    return compute_amax(s, out);
+}
+
+int32_t cdc_solver_set_timing(cdc_solver_t *s, int32_t enable) {
+    if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->timing = enable != 0;
+    s->launches = 0;
+    s->total_ms = 0.0;
+    return CDC_OK;
+}
+
+int32_t cdc_solver_get_timing(cdc_solver_t *s, int64_t *launches, double *total_ms, double *cells_per_launch) {
+    if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
+    if (launches) *launches = s->launches;
+    if (total_ms) *total_ms = s->total_ms;
+    if (cells_per_launch) *cells_per_launch = (double)s->ncell();
+    return CDC_OK;
+}
+
+int32_t cdc_solver_advance(cdc_solver_t *s, int32_t fold, double *dt_out, cdc_error_t *err) {
+    if (err) memset(err, 0, sizeof(*err));
+    if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
+    if (fold < 0) return set_msg(CDC_ERR_INVALID_ARGUMENT, "fold must be >= 0");
+    const cdc_physics_t &ph = s->phys;
+
+    // dt = cfl * dx / max signal speed (src/scheme.rs:336-340)
+    double amax = 0.0;
+    int rc = compute_amax(s, &amax);
+    if (rc) return rc;
+    const double L = 2.0 * s->mesh.domain_radius / (double)s->nb;
+    const double dx = L / (double)s->bs;
+    double dt = ph.cfl * dx / amax;
+    if (ph.has_dt_min) dt = std::max(dt, ph.dt_min);
+    if (dt_out) *dt_out = dt;
+    if (fold == 0) return CDC_OK;
+
+    HIP_TRY(cdc::launch_set_int(s->d_err_flag, 0, s->stream));
+
+    double el[4];
+    cdc::orbital_elements(ph, el);
+    const int rk = ph.rk_order;
+    std::vector<StageRec> stages;
+    stages.reserve((size_t)fold * rk);
+
+    if (s->timing && s->ev.size() < (size_t)2 * fold * rk) {
+        const size_t need = (size_t)2 * fold * rk;
+        while (s->ev.size() < need) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            s->ev.push_back(e);
+        }
+    }
+    int nev = 0;
+
+    auto pick = [&](int a, int b, int c) {
+        for (int i = 0; i < NBUF; ++i)
+            if (i != a && i != b && i != c) return i;
+        return -1;
+    };
+
+    // one stage: out = L(in) combined with base when read_base
+    auto stage = [&](const Slot &in, const Slot &base, bool read_base, int64_t bn, int64_t bd, bool last,
+                     Slot &out) -> int {
+        const int sid = (int)stages.size() + 1;
+        stages.push_back({in.time});
+        out.idx = pick(in.idx, base.idx, s->cur);
+        const double bf = read_base ? (double)bn / (double)bd : 0.0;
+        const double c1 = -bf + 1.0;
+        cdc::Orbit orb = cdc::orbital_state(el, in.time);
+
+        cdc::StageArgs a;
+        memset(&a, 0, sizeof(a));
+        a.uc = s->d_u[in.idx];
+        a.u0 = s->d_init;
+        a.ub = s->d_u[base.idx];
+        a.uo = s->d_u[out.idx];
+        a.tile_sums = s->d_tile_sums;
+        a.tile_err = s->d_tile_err;
+        a.err_flag = s->d_err_flag;
+        a.N = s->N;
+        a.nb = s->nb;
+        a.bs = s->bs;
+        a.ntiles = s->ntiles;
+        a.stage_id = sid;
+        a.read_base = read_base ? 1 : 0;
+        a.want_amax = last ? 1 : 0;
+        a.R = s->mesh.domain_radius;
+        a.L = L;
+        a.time = in.time;
+        a.dt = dt;
+        a.c1 = c1;
+        a.c0 = bf;
+        a.orb = orb;
+        a.k = make_consts(s, in.time, dt);
+        if (s->timing) HIP_TRY(hipEventRecord(s->ev[nev], s->stream));
+        HIP_TRY(cdc::launch_stage(s->nv, s->ts, a, s->stream));
+        if (s->timing) HIP_TRY(hipEventRecord(s->ev[nev + 1], s->stream));
+        if (s->timing) nev += 2;
+
+        cdc::FinalizeArgs f;
+        memset(&f, 0, sizeof(f));
+        f.tile_sums = s->d_tile_sums;
+        f.ist_c = s->d_ist[in.idx];
+        f.ist_b = s->d_ist[base.idx];
+        f.oec_c = s->d_oec[in.idx];
+        f.oec_b = s->d_oec[base.idx];
+        f.ist_o = s->d_ist[out.idx];
+        f.oec_o = s->d_oec[out.idx];
+        f.perr = s->d_perr;
+        f.err_flag = s->d_err_flag;
+        f.nblk = s->nblk;
+        f.tiles_per_block = s->tpb;
+        f.stage_id = sid;
+        f.read_base = read_base ? 1 : 0;
+        f.c1 = c1;
+        f.c0 = bf;
+        f.area = a.k.dx * a.k.dy;
+        f.orb = orb;
+        for (int i = 0; i < 4; ++i) f.el[i] = el[i];
+        HIP_TRY(cdc::launch_finalize(s->nv, f, s->stream));
+
+        // time and iteration of the stage result, then of the combine
+        double t1 = in.time + dt;
+        int64_t n1 = in.num + in.den, d1 = in.den;
+        rat_norm(n1, d1);
+        if (read_base) {
+            out.time = t1 * c1 + base.time * bf;
+            rat_wavg(n1, d1, base.num, base.den, bn, bd, out.num, out.den);
+        } else {
+            out.time = t1;
+            out.num = n1;
+            out.den = d1;
+        }
+        return CDC_OK;
+    };
+
+    Slot cur{s->cur, s->time, s->num, s->den};
+    for (int f = 0; f < fold; ++f) {
+        const bool lastf = f == fold - 1;
+        Slot s1, s2, s3;
+        if (rk == 1) {
+            if ((rc = stage(cur, cur, false, 0, 1, lastf, s1))) return rc;
+            cur = s1;
+        } else if (rk == 2) {
+            if ((rc = stage(cur, cur, false, 0, 1, false, s1))) return rc;
+            if ((rc = stage(s1, cur, true, 1, 2, lastf, s2))) return rc;
+            cur = s2;
+        } else {
+            if ((rc = stage(cur, cur, false, 0, 1, false, s1))) return rc;
+            if ((rc = stage(s1, cur, true, 3, 4, false, s2))) return rc;
+            if ((rc = stage(s2, cur, true, 1, 3, lastf, s3))) return rc;
+            cur = s3;
+        }
+    }
+
+    int32_t flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, s->d_err_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->timing) {
+        for (int i = 0; i < nev; i += 2) {
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]));
+            s->total_ms += ms;
+            s->launches += 1;
+        }
+    }
+
+    if (flag == 0) {
+        s->cur = cur.idx;
+        s->time = cur.time;
+        s->num = cur.num;
+        s->den = cur.den;
+        s->amax_valid = true;
+        return CDC_OK;
+    }
+
+    // Resolve the error as join_all over blocks would (src/scheme.rs:161-165):
+    // block b fails with the first failing 3x3 neighbour's P-phase error
+    // (try_join_3by3 order), else with its own perturbation error.
+    s->amax_valid = false;
+    std::vector<cdc::TileErr> te((size_t)s->ntiles);
+    std::vector<int32_t> pe((size_t)s->nblk);
+    HIP_TRY(hipMemcpy(te.data(), s->d_tile_err, te.size() * sizeof(cdc::TileErr), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(pe.data(), s->d_perr, pe.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<cdc::TileErr> berr((size_t)s->nblk, cdc::TileErr{0, 0, 0.0});
+    for (int b = 0; b < s->nblk; ++b)
+        for (int t = 0; t < s->tpb; ++t) {
+            const cdc::TileErr &e = te[(size_t)b * s->tpb + t];
+            if (e.code && (!berr[b].code || e.cell < berr[b].cell)) berr[b] = e;
+        }
+    const double stage_time = stages[(size_t)flag - 1].time;
+    cdc::Orbit orb = cdc::orbital_state(el, stage_time);
+    for (int b = 0; b < s->nblk; ++b) {
+        const int bi = b / s->nb, bj = b % s->nb;
+        int code = 0, nbk = -1;
+        for (int r = 0; r < 3 && !code; ++r)
+            for (int c = 0; c < 3 && !code; ++c) {
+                const int n = ((bi + s->nb + r - 1) % s->nb) * s->nb + (bj + s->nb + c - 1) % s->nb;
+                if (berr[n].code) { code = berr[n].code; nbk = n; }
+            }
+        if (!code && pe[b]) code = pe[b];
+        if (!code) continue;
+        if (err) {
+            err->code = code;
+            err->time = stage_time;
+            err->has_binary = 1;
+            err->binary[0] = {orb.m1, orb.x1, orb.y1, orb.vx1, orb.vy1};
+            err->binary[1] = {orb.m2, orb.x2, orb.y2, orb.vx2, orb.vy2};
+            if (nbk >= 0) {
+                err->value = berr[nbk].value;
+                cell_centre(s, nbk, berr[nbk].cell, &err->x, &err->y);
+            }
+            snprintf(err->message, sizeof(err->message), "%s %.4e at position (%.4f %.4f)", cdc_error_string(code),
+                     err->value, err->x, err->y);
+        }
+        return set_msg(code, cdc_error_string(code));
+    }
+    return set_msg(CDC_ERR_HIP, "error flag set without a failing block");
+}
+
+}  // extern "C"

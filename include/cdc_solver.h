@@ -1,0 +1,254 @@
+/*
+ * cdc_solver.h -- C ABI of the MI355X-native per-block Runge-Kutta update of
+ * the circumbinary disk code (CDC, clemson-cal/app-binary2d).
+ *
+ * The entry points below replace the reference's hot path
+ *
+ *     pub fn advance<H>(state, hydro, mesh, physics, fold, &mut timestep,
+ *                       block_data, runtime) -> Result<State, HydroError>
+ *                                                       (src/scheme.rs:323-349)
+ *
+ * and the data it moves (State / BlockState, src/state.rs:27-62).  A Rust
+ * caller binds them with an `extern "C"` block; the binding is spelled out in
+ * INTEGRATION.md.
+ *
+ * Data layout at the boundary (identical to the reference's in-memory arrays,
+ * so a checkpoint converts with a memcpy):
+ *
+ *   conserved : [block b][i][j][v]   doubles, b in Mesh::block_indexes() order
+ *               (i-major, src/mesh.rs:94-99), i = x index, j = y index
+ *               (ndarray C order, src/state.rs:46), v < nv
+ *               (nv = 3 isothermal, 4 Euler)
+ *   ist       : [block b][item k][v]  integrated_source_terms, the 7 items in
+ *               ItemizedChange field order (src/state.rs:28-36):
+ *               sink1, sink2, grav1, grav2, buffer, cooling, fake_mass
+ *   oec       : [block b][item k][4]  orbital_elements_change, each an
+ *               OrbitalElements tuple (a, M, q, e) (src/physics.rs:411-417)
+ *
+ * Every call is synchronous with respect to the host.  A handle owns one HIP
+ * stream and its device buffers; a handle is not re-entrant, distinct handles
+ * may be used from distinct threads.  Host buffers are caller-owned and are
+ * copied.
+ */
+#ifndef CDC_SOLVER_H
+#define CDC_SOLVER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CDC_ABI_VERSION 1
+#define CDC_NUM_ITEMS 7
+
+/* ItemizedChange field order, src/state.rs:28-36 */
+enum {
+    CDC_ITEM_SINK1 = 0,
+    CDC_ITEM_SINK2 = 1,
+    CDC_ITEM_GRAV1 = 2,
+    CDC_ITEM_GRAV2 = 3,
+    CDC_ITEM_BUFFER = 4,
+    CDC_ITEM_COOLING = 5,
+    CDC_ITEM_FAKE_MASS = 6
+};
+
+/* Return codes.  1..3 mirror HydroErrorType (src/physics.rs:26-36). */
+enum {
+    CDC_OK = 0,
+    CDC_ERR_NEGATIVE_DENSITY = 1,
+    CDC_ERR_NEGATIVE_PRESSURE = 2,
+    CDC_ERR_UNBOUND_ORBIT = 3,
+    CDC_ERR_INVALID_ARGUMENT = 100,
+    CDC_ERR_HIP = 101,
+    CDC_ERR_OUT_OF_MEMORY = 102,
+    CDC_ERR_NO_DEVICE = 103
+};
+
+enum { CDC_ISOTHERMAL = 0, CDC_EULER = 1 };                 /* AnyHydro, src/app.rs:28-33 */
+enum { CDC_VISC_CONSTANT_NU = 0, CDC_VISC_ALPHA = 1 };      /* ViscosityModel, src/physics.rs:100-103 */
+enum { CDC_SINK_IMPARTIAL = 0, CDC_SINK_TORQUE_FREE = 1 };  /* SinkModel, src/physics.rs:112-115 */
+/* Cooling law of the Euler system.  T4 is the reference's
+ * (src/physics.rs:714-739); BETA (t_cool = beta / Omega, relaxing the
+ * specific internal energy to that of the initial state) has no reference
+ * counterpart. */
+enum { CDC_COOLING_T4 = 0, CDC_COOLING_BETA = 1 };
+
+/* Mesh, src/mesh.rs:19-37 */
+typedef struct cdc_mesh {
+    int32_t num_blocks;
+    int32_t block_size;
+    double  domain_radius;
+} cdc_mesh_t;
+
+/* AnyHydro: Isothermal (src/physics.rs:258-265) | Euler (src/physics.rs:273-303).
+ * Option<f64> fields are a has_/value pair. */
+typedef struct cdc_hydro {
+    int32_t kind;                 /* CDC_ISOTHERMAL | CDC_EULER */
+    int32_t density_index;        /* Euler; reference default 2 */
+    double  mach_number;          /* Isothermal */
+    double  gamma_law_index;      /* Euler */
+    double  cooling_coefficient;  /* Euler, T^4 law */
+    int32_t cooling_slow_start;   /* Euler, bool */
+    int32_t has_density_floor;
+    double  density_floor;
+    int32_t has_pressure_floor;   /* Euler */
+    int32_t cooling_model;        /* CDC_COOLING_T4 | CDC_COOLING_BETA */
+    double  pressure_floor;
+    double  cooling_beta;         /* CDC_COOLING_BETA only */
+} cdc_hydro_t;
+
+/* Physics, src/physics.rs:123-217.  `viscosity_model`/`viscosity` hold the
+ * resolved Physics::viscosity_model() (src/physics.rs:222-230): the caller
+ * applies the precedence of the deprecated `nu` / `alpha` fields. */
+typedef struct cdc_physics {
+    double  buffer_rate;
+    double  buffer_scale;
+    double  binary_eccentricity;
+    double  binary_mass_ratio;
+    double  lambda_multiplier;
+    int32_t viscosity_model;      /* CDC_VISC_* */
+    int32_t sink_model;           /* CDC_SINK_* */
+    double  viscosity;            /* nu (constant_nu) or alpha */
+    double  cfl;
+    double  plm;
+    int32_t has_plm_density_trigger;
+    int32_t one_body;             /* bool */
+    double  plm_density_trigger;
+    double  fake_mass_threshold;
+    double  fake_mass_rate;
+    int32_t has_mach_ceiling;
+    int32_t rk_order;             /* 1, 2 or 3 */
+    double  mach_ceiling;
+    double  sink_radius;
+    double  sink_rate;
+    int32_t has_softening_length;
+    int32_t has_dt_min;
+    double  softening_length;
+    double  dt_min;
+} cdc_physics_t;
+
+/* kepler_two_body::PointMass field order (mass, x, y, vx, vy) */
+typedef struct cdc_point_mass {
+    double mass;
+    double position_x;
+    double position_y;
+    double velocity_x;
+    double velocity_y;
+} cdc_point_mass_t;
+
+/* HydroError, src/physics.rs:48-71: error type + value, position, and the
+ * binary's orbital state at the failing stage's time. */
+typedef struct cdc_error {
+    int32_t code;                 /* CDC_ERR_* (0 when no error) */
+    int32_t has_binary;
+    double  value;                /* NegativeDensity / NegativePressure payload */
+    double  x;
+    double  y;
+    double  time;                 /* start time of the failing RK stage */
+    cdc_point_mass_t binary[2];
+    char    message[256];
+} cdc_error_t;
+
+typedef struct cdc_solver cdc_solver_t;
+
+/* ---- library ------------------------------------------------------------ */
+
+/* ABI version (CDC_ABI_VERSION). */
+int32_t cdc_abi_version(void);
+
+/* sizeof of each boundary struct, so that bindings can verify their layout. */
+int32_t cdc_struct_size(int32_t which);  /* 0 mesh, 1 hydro, 2 physics, 3 error */
+
+/* Number of visible HIP devices (0 when none). */
+int32_t cdc_device_count(void);
+
+/* Static text for a return code. */
+const char *cdc_error_string(int32_t code);
+
+/* Message of the last failing call on this thread. */
+const char *cdc_last_error_message(void);
+
+/* ---- solver handle: BlockData + State + advance ------------------------- */
+
+/*
+ * Create a solver.  `initial_conserved` ([block][i][j][v], see above) plays the
+ * role of BlockData::initial_conserved (src/scheme.rs:36-58, used by the buffer
+ * and fake-mass sources); it also becomes the initial state at time 0,
+ * iteration 0/1 with zero ist/oec (State::from_model, src/state.rs:99-110).
+ */
+int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro,
+                          const cdc_physics_t *physics,
+                          const double *initial_conserved, cdc_solver_t **out);
+
+void cdc_solver_destroy(cdc_solver_t *solver);
+
+/* Replace the physics used by later advance calls (the safe-mode swap of
+ * src/main.rs:131-147 passes a modified Physics per call). */
+int32_t cdc_solver_set_physics(cdc_solver_t *solver, const cdc_physics_t *physics);
+
+/* Upload a State (src/state.rs:57-62).  ist / oec may be NULL (zeros). */
+int32_t cdc_solver_set_state(cdc_solver_t *solver, double time,
+                             int64_t iteration_num, int64_t iteration_den,
+                             const double *conserved, const double *ist,
+                             const double *oec);
+
+/* Download the current State.  Any output pointer may be NULL. */
+int32_t cdc_solver_get_state(cdc_solver_t *solver, double *time,
+                             int64_t *iteration_num, int64_t *iteration_den,
+                             double *conserved, double *ist, double *oec);
+
+/*
+ * scheme::advance (src/scheme.rs:323-349): dt = cfl * dx / max signal speed
+ * (raised to dt_min when set), written to *dt_out, then `fold` RK steps of
+ * that dt.  On a hydro error the call returns its code, fills *err (may be
+ * NULL) and leaves the handle's state exactly as it was before the call.
+ */
+int32_t cdc_solver_advance(cdc_solver_t *solver, int32_t fold, double *dt_out,
+                           cdc_error_t *err);
+
+/* State::max_signal_speed (src/state.rs:121-133) of the current state. */
+int32_t cdc_solver_max_signal_speed(cdc_solver_t *solver, double *out);
+
+/* Per-launch timing of the fused stage kernel with HIP events on the
+ * handle's stream.  enable=1 clears and starts accumulating. */
+int32_t cdc_solver_set_timing(cdc_solver_t *solver, int32_t enable);
+int32_t cdc_solver_get_timing(cdc_solver_t *solver, int64_t *launches,
+                              double *total_ms, double *cells_per_launch);
+
+/* ---- host utilities (no GPU) -------------------------------------------- */
+
+/* Physics::orbital_state_from_time (src/physics.rs:419-421). */
+int32_t cdc_orbital_state_from_time(const cdc_physics_t *physics, double time,
+                                    cdc_point_mass_t out[2]);
+
+/*
+ * Seeded cavity disk: an initial model with no reference counterpart, an
+ * InitialModel (src/traits.rs:105-116) written out per cell.
+ *   Sigma(r) = max(floor, exp(-(r_cav / r)^xi)) * (1 + amp * n_sigma)
+ *   v_phi    = sqrt(M r^2 / (r^2 + r_s^2)^(3/2))   (about the centre of mass)
+ *   p        = Sigma v_phi^2 / (gamma M_a^2) * (1 + amp * n_p)      (Euler)
+ * with n_sigma, n_p uniform in [-1, 1) drawn by a counter-based generator
+ * (SplitMix64) from (seed, stream, global cell index), so the state is a pure
+ * function of the seed and the global grid: block-decomposition invariant and
+ * reproducible on restart.
+ */
+typedef struct cdc_cavity_model {
+    double   cavity_radius;      /* r_cav, 2.5 */
+    double   cavity_exponent;    /* xi, 12 */
+    double   density_floor;      /* 1e-4 */
+    double   noise_amplitude;    /* 1e-3 */
+    double   softening_length;   /* r_s in v_phi */
+    double   mach_number;        /* M_a for the Euler pressure */
+    uint64_t seed;
+} cdc_cavity_model_t;
+
+int32_t cdc_model_seeded_cavity(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro,
+                                const cdc_cavity_model_t *model,
+                                double *conserved_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CDC_SOLVER_H */
