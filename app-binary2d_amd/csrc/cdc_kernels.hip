@@ -645,6 +645,20 @@ __global__ void __launch_bounds__(256) soa_to_aos_kernel(int nv, int nb, int bs,
 
 __global__ void set_int_kernel(int32_t *p, int32_t v) { *p = v; }
 
+// max over a strided column (the per-tile max signal speed slot), one block
+__global__ void __launch_bounds__(256) column_max_kernel(const double *x, int n, int stride, double *out) {
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, x[(size_t)i * stride]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 
@@ -695,6 +709,11 @@ hipError_t launch_aos_to_soa(int nv, int nb, int bs, const double *aos, double *
 hipError_t launch_soa_to_aos(int nv, int nb, int bs, const double *soa, double *aos, hipStream_t s) {
     const int64_t n = (int64_t)nb * bs * nb * bs;
     hipLaunchKernelGGL(soa_to_aos_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nv, nb, bs, soa, aos);
+    return hipGetLastError();
+}
+
+hipError_t launch_column_max(const double *x, int n, int stride, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(column_max_kernel, dim3(1), dim3(256), 0, s, x, n, stride, out);
     return hipGetLastError();
 }
 

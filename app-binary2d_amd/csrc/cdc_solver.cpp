@@ -424,12 +424,10 @@ static int compute_amax(cdc_solver *s, double *out) {
     if (s->amax_valid) {
         // max signal speed of the committed state, left by the last stage
         const int nr = s->nsum + 1;
-        std::vector<double> sums((size_t)s->ntiles * nr);
-        HIP_TRY(hipMemcpyAsync(sums.data(), s->d_tile_sums, sums.size() * sizeof(double), hipMemcpyDeviceToHost,
-                               s->stream));
-        HIP_TRY(hipStreamSynchronize(s->stream));
         double a = 0.0;
-        for (int t = 0; t < s->ntiles; ++t) a = std::max(sums[(size_t)t * nr + s->nsum], a);
+        HIP_TRY(cdc::launch_column_max(s->d_tile_sums + s->nsum, s->ntiles, nr, s->d_amax, s->stream));
+        HIP_TRY(hipMemcpyAsync(&a, s->d_amax, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
         *out = a;
         return CDC_OK;
     }

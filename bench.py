@@ -1,0 +1,122 @@
+"""Benchmark: FP64 zone-updates/s of the per-block RK update on one MI355X.
+
+Workload (default euler2048): gamma-law gas on 2048^2 cells (32x32 blocks of
+64^2), RK3, alpha viscosity, eccentric (e = 0.5) unequal (q = 0.5) binary,
+beta cooling; seeded cavity initial state, resident in HBM before timing.
+A step is one full RK step of the whole mesh (one zone-update per cell, as
+the reference's Mzps counts it).  Steps run through the C ABI as
+cdc_solver_advance(fold) calls; every call returns after its work is done.
+
+Also reported: the roofline of the fused stage kernel (algorithmic HBM bytes
+per launch over its HIP-event-timed average duration) and the CPU oracle on
+the same workload for a bounded sample, timed on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "app-binary2d_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_per_step(nv, rk):
+    """HBM bytes per cell per full RK step: stage 1 reads U and U_init and
+    writes U*; later stages also read the step base."""
+    return 8 * nv * (3 + 4 * (rk - 1))
+
+
+def cpu_baseline(w, u0, seconds, threads):
+    from oracle import oracle as O
+    O.set_threads(threads)
+    m, h, p = w["mesh"].to_c(), w["hydro"].to_c(), w["physics"].to_c()
+    o = O.Oracle(m, h, p, u0, w["hydro"].nv)
+    o.advance(1)  # first touch of the work buffers
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.advance(1)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    o.close()
+    zones = w["mesh"].total_zones()
+    return {"value": n * zones / el, "unit": "zone-updates/s", "cores": O.get_threads(), "kind": "port",
+            "sample": f"{n} RK steps of the same workload ({el:.1f} s), C oracle, OpenMP over blocks"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--fold", type=int, default=10)
+    ap.add_argument("--workload", default="euler2048")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              f"(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    from cdc_amd import Solver, workloads
+    w = workloads.get(args.workload)
+    u0 = workloads.initial_state(w)
+    mesh, hydro, ph = w["mesh"], w["hydro"], w["physics"]
+    zones = mesh.total_zones()
+    fold = max(1, min(args.fold, args.steps))
+    steps = (args.steps // fold) * fold
+    g = Solver(mesh, hydro, ph, u0)
+    done = 0
+    while done < args.warmup:
+        g.advance(min(fold, args.warmup - done))
+        done += min(fold, args.warmup - done)
+
+    g.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps // fold):
+        g.advance(fold)
+    t1 = time.perf_counter()
+    launches, kernel_ms, cells = g.get_timing()
+    wall = t1 - t0
+    rk = ph.rk_substeps()
+    bytes_total = algorithmic_bytes_per_step(hydro.nv, rk) * zones * steps
+    achieved = bytes_total / (kernel_ms * 1e-3) / 1e9
+    line = {
+        "metric": "f64 zone-updates/s @2048²; learned 512² vs plain 2048² wall/orbit @≤2% Ṁ err",
+        "value": steps * zones / wall,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded cavity disk, seed %d)" % w["seed"],
+        "config": {"workload": args.workload, "grid": f"{mesh.num_blocks * mesh.block_size}^2",
+                   "blocks": f"{mesh.num_blocks}x{mesh.num_blocks} of {mesh.block_size}^2",
+                   "hydro": type(hydro).__name__.lower(), "rk": ph.rk_order, "fold": fold},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "stage_kernel", "launches": launches,
+                     "avg_launch_ms": kernel_ms / max(launches, 1),
+                     "bytes_per_launch": bytes_total / max(launches, 1)},
+    }
+    g.close()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(w, u0, args.cpu_seconds, args.cpu_threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,245 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bars: conserved state per step within 1e-12 (field-scaled relative,
+`helpers.field_err`), after one orbit within 1e-9; dt and time within
+1e-12; integrated source terms within 1e-10 per item (the accretion series
+within 1e-9); orbital-element changes within 1e-12 absolute (they are
+differences of O(1) elements); errors with the same code, value and
+position, and the state left as it was.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cdc_amd import HydroError, Solver, abi, workloads
+from oracle import oracle as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    lib = abi.load()
+    assert lib.cdc_device_count() >= 1, "no HIP device: the HIP path must run on a GPU"
+    O.set_threads(min(16, O.get_threads()))
+
+
+def pair(mesh, hydro, ph, u0):
+    return Solver(mesh, hydro, ph, u0), H.oracle_for(mesh, hydro, ph, u0)
+
+
+def check_states(g, o, tol=1e-12, ist_tol=1e-10):
+    sg, so = g.get_state(), o.get_state()
+    assert sg["iteration"] == so["iteration"]
+    assert sg["time"] == pytest.approx(so["time"], rel=1e-12, abs=1e-300)
+    err = H.field_err(sg["conserved"], so["conserved"])
+    assert err <= tol, f"state error {err:.3e}"
+    keep = [0, 1, 2, 3, 4, 6]
+    ie = H.items_err(sg["ist"][:, keep], so["ist"][:, keep])
+    assert ie <= ist_tol, f"itemized source error {ie:.3e}"
+    if g.nv == 4:
+        ce, cr = sg["ist"][:, 5, 3], so["ist"][:, 5, 3]
+        assert np.max(np.abs(ce - cr)) <= 1e-8 * max(np.max(np.abs(cr)), 1e-300)
+    oe = np.max(np.abs(sg["oec"] - so["oec"]))
+    assert oe <= 1e-12, f"orbital element change error {oe:.3e}"
+    return err
+
+
+CASES = [
+    # kind, nb, bs, physics overrides
+    ("iso", 2, 8, dict(rk_order="RK2")),
+    ("iso", 2, 16, dict(rk_order="RK3", viscosity={"alpha": 0.1}, sink_model="torque_free",
+                        binary_eccentricity=0.4, binary_mass_ratio=0.3)),
+    ("iso", 1, 32, dict(rk_order="RK1", lambda_multiplier=0.7, fake_mass_rate=1e-3, fake_mass_threshold=0.5)),
+    ("iso", 2, 64, dict(rk_order="RK2", plm_density_trigger=1e-6, one_body=True)),
+    ("euler", 2, 16, dict(rk_order="RK3", viscosity={"alpha": 0.1}, binary_eccentricity=0.5,
+                          binary_mass_ratio=0.5)),
+    ("euler", 2, 32, dict(rk_order="RK2", sink_model="torque_free", mach_ceiling=5.0, lambda_multiplier=0.3)),
+    ("euler_beta", 2, 8, dict(rk_order="RK3", viscosity={"constant_nu": 2e-3})),
+    ("euler_beta", 1, 64, dict(rk_order="RK1", viscosity={"alpha": 0.05}, sink_model="torque_free")),
+]
+
+
+@pytest.mark.parametrize("kind,nb,bs,kw", CASES, ids=[f"{c[0]}-{c[1]}x{c[2]}-{i}" for i, c in enumerate(CASES)])
+def test_small_configurations_match_oracle_every_step(kind, nb, bs, kw):
+    mesh, hydro, ph = H.small(kind, nb=nb, bs=bs, R=4.0, **kw)
+    u0 = H.initial(mesh, hydro, ph)
+    g, o = pair(mesh, hydro, ph, u0)
+    for step in range(6):
+        fold = 1 if step < 4 else 3
+        dg, do = g.advance(fold), o.advance(fold)
+        assert dg == pytest.approx(do, rel=1e-12)
+        check_states(g, o)
+
+
+def test_floors_and_slow_start_cooling():
+    mesh, hydro, ph = H.small("euler", nb=2, bs=16, R=4.0, rk_order="RK2")
+    hydro.density_floor = 2e-7
+    hydro.pressure_floor = 1e-12
+    hydro.cooling_slow_start = True
+    hydro.cooling_coefficient = 1e-6
+    u0 = H.initial(mesh, hydro, ph)
+    g, o = pair(mesh, hydro, ph, u0)
+    for _ in range(4):
+        assert g.advance(1) == pytest.approx(o.advance(1), rel=1e-12)
+        check_states(g, o)
+
+
+def accretion_series(solver_state):
+    return solver_state["ist"][:, 0, 0].sum(), solver_state["ist"][:, 1, 0].sum()
+
+
+def test_iso256_one_orbit():
+    """The isothermal 256^2 configuration stepped for one orbit in lockstep
+    with the oracle: per-step state <= 1e-12 for the first steps, <= 1e-9 at
+    the end, and the accretion and torque time series <= 1e-9."""
+    w = workloads.get("iso256")
+    u0 = workloads.initial_state(w)
+    g, o = pair(w["mesh"], w["hydro"], w["physics"], u0)
+    t_end = 2 * math.pi * w["orbits"]
+    step = 0
+    mg, mo, tg, to_ = [], [], [], []
+    while g.get_state()["time"] < t_end:
+        dg, do = g.advance(1), o.advance(1)
+        assert dg == pytest.approx(do, rel=1e-9)
+        sg, so = g.get_state(), o.get_state()
+        if step < 10:
+            check_states(g, o)
+        mg.append(accretion_series(sg)); mo.append(accretion_series(so))
+        tg.append(sg["ist"][:, 2:4, 1:3].sum(axis=0)); to_.append(so["ist"][:, 2:4, 1:3].sum(axis=0))
+        step += 1
+    sg, so = g.get_state(), o.get_state()
+    assert H.field_err(sg["conserved"], so["conserved"]) <= 1e-9
+    mg, mo = np.array(mg), np.array(mo)
+    assert np.max(np.abs(mg - mo)) <= 1e-9 * np.max(np.abs(mo))
+    tg, to_ = np.array(tg), np.array(to_)
+    assert np.max(np.abs(tg - to_)) <= 1e-9 * np.max(np.abs(to_))
+    assert step > 100
+
+
+def test_iso1024_steps():
+    w = workloads.get("iso1024")
+    u0 = workloads.initial_state(w)
+    g, o = pair(w["mesh"], w["hydro"], w["physics"], u0)
+    for _ in range(8):
+        assert g.advance(1) == pytest.approx(o.advance(1), rel=1e-12)
+        check_states(g, o)
+
+
+def test_euler2048_steps_and_mass_budget():
+    w = workloads.get("euler2048")
+    u0 = workloads.initial_state(w)
+    g, o = pair(w["mesh"], w["hydro"], w["physics"], u0)
+    for _ in range(2):
+        assert g.advance(1) == pytest.approx(o.advance(1), rel=1e-12)
+        check_states(g, o)
+    o.close()
+    # full 100-step run on the GPU alone: mass changes only through the sources
+    for _ in range(98):
+        g.advance(1)
+    s = g.get_state()
+    dA = w["mesh"].cell_spacing() ** 2
+    dm = (s["conserved"][..., 0].sum() - u0[..., 0].sum()) * dA
+    src = s["ist"][:, :, 0].sum()
+    assert abs(dm - src) <= 1e-11 * u0[..., 0].sum() * dA
+    assert s["iteration"] == (100, 1)
+
+
+def test_negative_density_error_matches_oracle_and_rolls_back():
+    mesh, hydro, ph = H.small("iso", nb=3, bs=16, R=4.0)
+    u0 = H.initial(mesh, hydro, ph)
+    g, o = pair(mesh, hydro, ph, u0)
+    g.advance(1); o.advance(1)
+    bad = g.get_state()["conserved"].copy()
+    bad[4, 3, 5, 0] = -1e-9
+    bad[8, 15, 15, 0] = -2e-9
+    bad[8, 1, 2, 0] = -3e-9
+    g.set_state(bad, time=0.25, iteration=(7, 2))
+    o.set_state(bad, time=0.25, iteration=(7, 2))
+    with pytest.raises(HydroError) as eg:
+        g.advance(2)
+    with pytest.raises(O.OracleError) as eo:
+        o.advance(2)
+    import ctypes
+    ref = abi.CError.from_buffer_copy(eo.value.err[:ctypes.sizeof(abi.CError)])
+    e = eg.value
+    assert e.kind == "NegativeDensity" and ref.code == abi.CDC_ERR_NEGATIVE_DENSITY
+    assert e.value == ref.value == -3e-9
+    assert e.position == (ref.x, ref.y)
+    assert e.time == ref.time == 0.25
+    s = g.get_state()
+    assert s["time"] == 0.25 and s["iteration"] == (7, 2)
+    np.testing.assert_array_equal(s["conserved"], bad)
+    # the solver keeps working once the state is repaired
+    good = bad.copy()
+    good[..., 0] = np.abs(good[..., 0])
+    g.set_state(good)
+    o.set_state(good)
+    assert g.advance(1) == pytest.approx(o.advance(1), rel=1e-12)
+    check_states(g, o)
+
+
+def test_negative_pressure_error():
+    mesh, hydro, ph = H.small("euler", nb=2, bs=16, R=4.0)
+    u0 = H.initial(mesh, hydro, ph)
+    bad = u0.copy()
+    bad[3, 4, 9, 3] = 0.0          # E = 0 with nonzero momentum: p < 0
+    g, o = pair(mesh, hydro, ph, bad)
+    with pytest.raises(HydroError) as eg:
+        g.advance(1)
+    with pytest.raises(O.OracleError) as eo:
+        o.advance(1)
+    import ctypes
+    ref = abi.CError.from_buffer_copy(eo.value.err[:ctypes.sizeof(abi.CError)])
+    assert eg.value.kind == "NegativePressure" and ref.code == abi.CDC_ERR_NEGATIVE_PRESSURE
+    assert eg.value.position == (ref.x, ref.y)
+    assert eg.value.value == pytest.approx(ref.value, rel=1e-12)
+
+
+def test_unbound_orbit_error():
+    # a full-density disk on a coarse mesh: the per-block momentum exchange
+    # unbinds the perturbed binary (OrbitalEvolutionError)
+    mesh, hydro, ph = H.small("iso", nb=2, bs=8, R=4.0, rk_order="RK2", binary_eccentricity=0.3,
+                              binary_mass_ratio=0.6)
+    u0 = H.initial(mesh, hydro, ph, cavity=1.5, scale=1.0)
+    g, o = pair(mesh, hydro, ph, u0)
+    codes = []
+    for solver in (g, o):
+        try:
+            for _ in range(4):
+                solver.advance(1)
+            codes.append(0)
+        except HydroError as e:
+            codes.append(e.code)
+        except O.OracleError as e:
+            codes.append(e.code)
+    assert codes[0] == codes[1] == abi.CDC_ERR_UNBOUND_ORBIT
+    check_states(g, o)
+
+
+def test_physics_swap_between_calls():
+    """The safe-mode swap of src/main.rs:131-147 hands a different Physics to
+    the next advance."""
+    mesh, hydro, ph = H.small("iso", nb=2, bs=16, R=4.0)
+    u0 = H.initial(mesh, hydro, ph)
+    g, o = pair(mesh, hydro, ph, u0)
+    g.advance(1); o.advance(1)
+    safe = H.deep(ph)
+    safe.cfl, safe.plm, safe.rk_order = 0.1, 1.0, "RK1"
+    g.set_physics(safe)
+    o.set_physics(safe.to_c())
+    for _ in range(2):
+        assert g.advance(1) == pytest.approx(o.advance(1), rel=1e-12)
+        check_states(g, o)
+
+
+def test_timing_counts_stage_launches():
+    mesh, hydro, ph = H.small("euler", nb=2, bs=32, R=4.0, rk_order="RK3")
+    u0 = H.initial(mesh, hydro, ph)
+    g = Solver(mesh, hydro, ph, u0)
+    g.set_timing(True)
+    g.advance(2)
+    n, ms, cells = g.get_timing()
+    assert n == 6 and ms > 0 and cells == 64 * 64
