@@ -30,7 +30,7 @@ def pair(mesh, hydro, ph, u0):
     return Solver(mesh, hydro, ph, u0), H.oracle_for(mesh, hydro, ph, u0)
 
 
-def check_states(g, o, tol=1e-12, ist_tol=1e-10):
+def check_states(g, o, tol=1e-12, ist_tol=1e-10, one_body=False):
     sg, so = g.get_state(), o.get_state()
     assert sg["iteration"] == so["iteration"]
     assert sg["time"] == pytest.approx(so["time"], rel=1e-12, abs=1e-300)
@@ -42,8 +42,14 @@ def check_states(g, o, tol=1e-12, ist_tol=1e-10):
     if g.nv == 4:
         ce, cr = sg["ist"][:, 5, 3], so["ist"][:, 5, 3]
         assert np.max(np.abs(ce - cr)) <= 1e-8 * max(np.max(np.abs(cr)), 1e-300)
-    oe = np.max(np.abs(sg["oec"] - so["oec"]))
-    assert oe <= 1e-12, f"orbital element change error {oe:.3e}"
+    d = np.abs(sg["oec"] - so["oec"])
+    # the binary's elements are O(1): absolute bar.  one_body's elements are
+    # (1e-12, 1, 1, 0) with a relative speed of 1e6, where a momentum kick is
+    # resolved only to ulp(m v) ~ 1e-10: bar relative to the change itself.
+    bar = 1e-12 + (1e-6 * np.abs(so["oec"]) if one_body else 0.0)
+    k = np.unravel_index(np.argmax(d - bar), d.shape)
+    assert np.all(d <= bar), (f"orbital element change error {d[k]:.3e} at block {k[0]} item {k[1]} "
+                              f"element {k[2]}: gpu {sg['oec'][k]:.17e} oracle {so['oec'][k]:.17e}")
     return err
 
 
@@ -71,7 +77,7 @@ def test_small_configurations_match_oracle_every_step(kind, nb, bs, kw):
         fold = 1 if step < 4 else 3
         dg, do = g.advance(fold), o.advance(fold)
         assert dg == pytest.approx(do, rel=1e-12)
-        check_states(g, o)
+        check_states(g, o, one_body=ph.one_body)
 
 
 def test_floors_and_slow_start_cooling():
