@@ -113,6 +113,8 @@ class CCavityModel(ctypes.Structure):
         ("noise_amplitude", c_double),
         ("softening_length", c_double),
         ("mach_number", c_double),
+        ("taper_radius", c_double),
+        ("taper_width", c_double),
         ("seed", c_uint64),
     ]
 

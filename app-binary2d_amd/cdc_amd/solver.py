@@ -52,11 +52,12 @@ def orbital_state_from_time(physics, t):
 
 
 def seeded_cavity(mesh, hydro, cavity_radius=2.5, cavity_exponent=12.0, density_floor=1e-4,
-                  noise_amplitude=1e-3, softening_length=0.05, mach_number=10.0, seed=1):
+                  noise_amplitude=1e-3, softening_length=0.05, mach_number=10.0, seed=1,
+                  taper_radius=0.0, taper_width=0.25):
     """The seeded cavity initial model (no reference counterpart), per cell."""
     lib = abi.load()
     m = abi.CCavityModel(cavity_radius, cavity_exponent, density_floor, noise_amplitude,
-                         softening_length, mach_number, seed)
+                         softening_length, mach_number, taper_radius, taper_width, seed)
     nb, bs = mesh.num_blocks, mesh.block_size
     out = np.zeros((nb * nb, bs, bs, hydro.nv), dtype=np.float64)
     cm, ch = mesh.to_c(), hydro.to_c()

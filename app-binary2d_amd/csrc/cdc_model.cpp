@@ -58,7 +58,8 @@ extern "C" int32_t cdc_model_seeded_cavity(const cdc_mesh_t *mesh, const cdc_hyd
                     if (sd < m->density_floor) sd = m->density_floor;
                     sd = sd * (1.0 + m->noise_amplitude * noise(m->seed, 0, gidx));
                     const double t = r2 + rs2;
-                    const double vp = sqrt(r2 / (t * sqrt(t)));
+                    double vp = sqrt(r2 / (t * sqrt(t)));
+                    if (m->taper_radius > 0.0) vp = vp / (1.0 + exp((r - m->taper_radius) / m->taper_width));
                     const double vx = r > 0.0 ? -vp * y / r : 0.0;
                     const double vy = r > 0.0 ? vp * x / r : 0.0;
                     double *u = out + (((int64_t)(bi * nb + bj) * bs + i) * bs + j) * nv;
@@ -66,7 +67,8 @@ extern "C" int32_t cdc_model_seeded_cavity(const cdc_mesh_t *mesh, const cdc_hyd
                     u[1] = sd * vx;
                     u[2] = sd * vy;
                     if (nv == 4) {
-                        double p = sd * vp * vp / (gm * m->mach_number * m->mach_number);
+                        // cs^2 = -phi / Ma^2 of the softened unit mass at the centre
+                        double p = sd * (1.0 / sqrt(t)) / (gm * m->mach_number * m->mach_number);
                         p = p * (1.0 + m->noise_amplitude * noise(m->seed, 1, gidx));
                         u[3] = 0.5 * sd * (vx * vx + vy * vy) + p / (gm - 1.0);
                     }

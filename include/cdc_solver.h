@@ -227,7 +227,10 @@ int32_t cdc_orbital_state_from_time(const cdc_physics_t *physics, double time,
  * InitialModel (src/traits.rs:105-116) written out per cell.
  *   Sigma(r) = max(floor, exp(-(r_cav / r)^xi)) * (1 + amp * n_sigma)
  *   v_phi    = sqrt(M r^2 / (r^2 + r_s^2)^(3/2))   (about the centre of mass)
- *   p        = Sigma v_phi^2 / (gamma M_a^2) * (1 + amp * n_p)      (Euler)
+ *   p        = Sigma / sqrt(r^2 + r_s^2) / (gamma M_a^2) * (1 + amp * n_p)   (Euler:
+ *              c_s^2 = -phi / M_a^2 of the softened unit mass, the isothermal law)
+ * and, when taper_radius > 0, v_phi multiplied by 1/(1 + exp((r - r_t)/w)) so
+ * that the flow vanishes at the periodic edges (inside the buffer zone),
  * with n_sigma, n_p uniform in [-1, 1) drawn by a counter-based generator
  * (SplitMix64) from (seed, stream, global cell index), so the state is a pure
  * function of the seed and the global grid: block-decomposition invariant and
@@ -240,6 +243,8 @@ typedef struct cdc_cavity_model {
     double   noise_amplitude;    /* 1e-3 */
     double   softening_length;   /* r_s in v_phi */
     double   mach_number;        /* M_a for the Euler pressure */
+    double   taper_radius;       /* v_phi *= 1/(1 + exp((r - r_t)/w)); 0 = no taper */
+    double   taper_width;        /* w */
     uint64_t seed;
 } cdc_cavity_model_t;
 

