@@ -28,46 +28,49 @@ struct StageConsts {
     double visc;                // nu (constant_nu) or alpha / sqrt(gamma) (alpha)
     double lam;                 // lambda_multiplier
     double eps_dx;              // 1e-12 * dx (alpha model)
-    double domain_radius, inv_buffer_scale, buffer_coef;  // 0.5 * buffer_rate * Omega_outer
     double sink_s2, inv_sink_s2, sink_rate;
     double fake_threshold, fake_rate;
     double cooling_coefficient, cooling_inv_beta;
     double cooling_f;           // T^4 slow-start factor exp(-2 pi / (t + dt)) or 1
     double mach_ceiling_coef;   // 2 / gm / (gm - 1) / mach_ceiling^2
     double density_floor, pressure_floor, density_floor_pressure;
-    double mtot;
     int32_t viscosity_model, sink_model, one_body, has_mach_ceiling;
     int32_t has_density_floor, has_pressure_floor, density_index, cooling_slow_start;
     int32_t cooling_model, has_pdt;
+};
+
+// Static per-cell planes, SoA [plane][gi][gj], written once per physics:
+// plane 0 = buffer-zone rate 0.5 rate (1 + tanh((r - R)/scale)) Omega_outer
+// (Physics::source_terms, src/physics.rs:443-446); plane 1 (beta cooling
+// only) = Omega = sqrt(M / (r^2 + r_s^2)^(3/2)).
+struct AuxArgs {
+    double *aux;
+    int32_t N, nb, bs, naux;
+    double R, L, buffer_rate, buffer_scale, omega_outer, rs2, mtot;
 };
 
 struct StageArgs {
     const double *uc;      // stage input, SoA [nv][N][N]
     const double *u0;      // initial (background) state, SoA
     const double *ub;      // RK step base, SoA (read when read_base)
+    const double *aux;     // static per-cell planes
     double *uo;            // stage output, SoA
     double *tile_sums;     // [ntiles][NSUM + 1] (last slot: max signal speed)
     TileErr *tile_err;     // [ntiles]
     int32_t *err_flag;     // stage id of the first failing stage, 0 if none
-    int32_t N, nb, bs, ntiles;
-    int32_t stage_id, read_base, want_amax, pad_;
-    double R, L;           // domain radius, block length
-    double time, dt, c1, c0;
-    Orbit orb;
-    StageConsts k;
-};
-
-struct FinalizeArgs {
-    const double *tile_sums;
+    int32_t *blk_count;    // [nblk] tiles of the block done this stage (reset by the last)
+    // per-block integrated_source_terms / orbital_elements_change
     const double *ist_c, *ist_b;   // [nblk][7][nv]
     const double *oec_c, *oec_b;   // [nblk][7][4]
     double *ist_o, *oec_o;
-    int32_t *perr;                 // [nblk]
-    int32_t *err_flag;
-    int32_t nblk, tiles_per_block, stage_id, read_base;
-    double c1, c0, area;           // RK weights, dx * dy
+    int32_t *perr;                 // [nblk] perturbation error of the block
+    int32_t N, nb, bs, ntiles;
+    int32_t stage_id, read_base, want_amax, reuse_prim;
+    double R, L;           // domain radius, block length
+    double time, dt, c1, c0, area;
+    double el[4];          // Physics::orbital_elements
     Orbit orb;
-    double el[4];
+    StageConsts k;
 };
 
 // Tile shape chosen for a block size; see DESIGN.md for the LDS budget.
@@ -79,7 +82,7 @@ TileShape tile_shape_for(int bs, int nv);
 int nsum_for(int nv);
 
 hipError_t launch_stage(int nv, const TileShape &ts, const StageArgs &a, hipStream_t s);
-hipError_t launch_finalize(int nv, const FinalizeArgs &a, hipStream_t s);
+hipError_t launch_aux(const AuxArgs &a, hipStream_t s);
 hipError_t launch_max_speed(int nv, const double *u, int64_t ncell, double gm, double *partial,
                             int nparts, hipStream_t s);
 hipError_t launch_aos_to_soa(int nv, int nb, int bs, const double *aos, double *soa, hipStream_t s);

@@ -15,12 +15,19 @@
 //            cooling), the low-storage RK combine with the step base, and the
 //            per-tile sums of the itemized source terms
 //            (src/scheme.rs:273-316, src/state.rs:183-200)
-// finalize_kernel then sums the tiles of each block in a fixed order, scales
-// by dx*dy, applies the orbital-element perturbation and the RK combine of
-// the block's integrated_source_terms / orbital_elements_change.
+//   tail     the last tile of each block to finish sums the block's tiles in
+//            a fixed order, scales by dx*dy, applies the orbital-element
+//            perturbation and the RK combine of the block's
+//            integrated_source_terms / orbital_elements_change
+//            (src/scheme.rs:307-315, src/state.rs:195-197)
+//
+// FP64 reciprocals and square roots use the hardware estimates (v_rcp_f64,
+// v_rsq_f64) refined by Newton-Raphson steps to within an ulp or so of the
+// IEEE results, at a third of the instructions of the IEEE expansions.
 #include <hip/hip_runtime.h>
 #include <limits.h>
 #include <math.h>
+#include <stdint.h>
 
 #include "cdc_internal.h"
 
@@ -55,6 +62,42 @@ template <> struct SumMap<4> {
 int nsum_for(int nv) { return nv == 4 ? SumMap<4>::n : SumMap<3>::n; }
 
 namespace dev {
+
+// ---- refined hardware estimates -------------------------------------------
+
+__device__ __forceinline__ double frcp(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-x, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+
+// 1 / sqrt(x), x > 0: coupled Newton iteration on (g ~ sqrt x, h ~ 1/(2 sqrt x))
+__device__ __forceinline__ double frsq(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double h = 0.5 * y, g = x * y;
+    double r = __builtin_fma(-h, g, 0.5);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(g, r, g);
+    r = __builtin_fma(-h, g, 0.5);
+    h = __builtin_fma(h, r, h);
+    return h + h;
+}
+
+// sqrt(x), x >= 0
+__device__ __forceinline__ double fsqrt(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double h = 0.5 * y, g = x * y;
+    double r = __builtin_fma(-h, g, 0.5);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(g, r, g);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return x > 0.0 ? g : x;
+}
 
 // Block geometry exactly as Mesh::block_vertices / cell_centers
 // (src/mesh.rs:45-84): linspace(start, start + L, bs + 1), centres as
@@ -91,27 +134,29 @@ __device__ __forceinline__ double plm(double th, double a, double b, double c) {
     return minmod3(th * (b - a), 0.5 * (c - a), th * (c - b));
 }
 
-// try_to_primitive: iso src/physics.rs:489-503, Euler src/physics.rs:631-660
+// try_to_primitive: iso src/physics.rs:489-503, Euler src/physics.rs:631-660.
+// Also returns 1/Sigma of the recovered state in `id`.
 template <int NV>
 __device__ __forceinline__ int try_to_primitive(const StageConsts &k, const double *u, double *p,
-                                                double &val) {
+                                                double &val, double &id) {
     if constexpr (NV == 3) {
         double d = u[0], sx = u[1], sy = u[2];
         if (k.has_density_floor && d < k.density_floor) {
             sx = 0.0; sy = 0.0; d = k.density_floor;
         }
-        const double id = 1.0 / d;
+        id = frcp(d);
         p[0] = d; p[1] = sx * id; p[2] = sy * id;
         if (d < 0.0) { val = d; return CDC_ERR_NEGATIVE_DENSITY; }
         return 0;
     } else {
         const double d = u[0];
-        const double id = 1.0 / d;
+        id = frcp(d);
         const double vx = u[1] * id, vy = u[2] * id;
-        double pg = (u[3] - 0.5 * d * (vx * vx + vy * vy)) * k.gm1;
-        p[0] = d; p[1] = vx; p[2] = vy; p[3] = pg;
+        p[0] = d; p[1] = vx; p[2] = vy;
+        p[3] = (u[3] - 0.5 * d * (vx * vx + vy * vy)) * k.gm1;
         if (k.has_density_floor && d < k.density_floor) {
             p[0] = k.density_floor; p[1] = 0.0; p[2] = 0.0; p[3] = k.density_floor_pressure;
+            id = frcp(k.density_floor);
         }
         if (p[0] < 0.0) { val = p[0]; return CDC_ERR_NEGATIVE_DENSITY; }
         if (k.has_pressure_floor && p[3] < k.pressure_floor) p[3] = k.pressure_floor;
@@ -122,6 +167,7 @@ __device__ __forceinline__ int try_to_primitive(const StageConsts &k, const doub
 
 // Hydrodynamics::max_signal_speed of a conserved state: iso
 // max(|vx|, |vy|) (src/physics.rs:521-526), Euler max(|vx|, |vy|) + c_s.
+// IEEE division and sqrt: this value sets dt.
 template <int NV>
 __device__ __forceinline__ double max_signal_speed(const double *u, double gm) {
     if constexpr (NV == 3) {
@@ -135,11 +181,12 @@ __device__ __forceinline__ double max_signal_speed(const double *u, double gm) {
 }
 
 // One face: reconstruction, viscous stress, HLLE.  axis 0 = x, 1 = y.
-// eL/eR index the primitive tile, rL/rR the gradient ring.
+// eL/eR index the primitive tile (and the cell sound-speed plane C), rL/rR
+// the gradient ring.
 template <int NV, int EXY, int RXY>
-__device__ __forceinline__ void face_flux(const StageArgs &A, int axis, const double *P,
-                                          const double *GX, const double *GY, int eL, int eR,
-                                          int rL, int rR, double fxp, double fyp, double *F) {
+__device__ __forceinline__ void face_flux(const StageArgs &A, int axis, const double *P, const double *C,
+                                          const double *GX, const double *GY, int eL, int eR, int rL,
+                                          int rR, double fxp, double fyp, double *F) {
     const StageConsts &k = A.k;
     const double *GN = axis == 0 ? GX : GY;
     double pL[NV], pR[NV], pl[NV], pr[NV];
@@ -151,34 +198,32 @@ __device__ __forceinline__ void face_flux(const StageArgs &A, int axis, const do
         pr[v] = pR[v] - GN[v * RXY + rR] * 0.5;
     }
 
-    // Plummer potential at the face (isothermal sound speed, alpha model)
+    // Plummer potential at the face: isothermal c_s^2 = -phi / M^2
     const Orbit &o = A.orb;
     const double d1x = fxp - o.x1, d1y = fyp - o.y1;
     const double d2x = fxp - o.x2, d2y = fyp - o.y2;
     const double r1s = d1x * d1x + d1y * d1y;
     const double r2s = d2x * d2x + d2y * d2y;
     double cs2f = 0.0;
-    if constexpr (NV == 3) {
-        const double phi = -o.m1 / sqrt(r1s + k.rs2) - o.m2 / sqrt(r2s + k.rs2);
-        cs2f = -phi * k.inv_mach2;
-    }
+    if constexpr (NV == 3) cs2f = (o.m1 * frsq(r1s + k.rs2) + o.m2 * frsq(r2s + k.rs2)) * k.inv_mach2;
 
     // viscous stress tau = (sigma_L + sigma_R) / 2 (src/physics.rs:322-340)
     double tau_x = 0.0, tau_y = 0.0;
-    const bool visc = k.viscosity_model != CDC_VISC_CONSTANT_NU || k.visc != 0.0;
-    if (visc) {
+    if (k.viscosity_model != CDC_VISC_CONSTANT_NU || k.visc != 0.0) {
         double nuL, nuR;
         if (k.viscosity_model == CDC_VISC_CONSTANT_NU) {
             nuL = nuR = k.visc;
         } else {
-            const double q1 = sqrt(r1s) + k.eps_dx;
-            const double q2 = sqrt(r2s) + k.eps_dx;
-            const double itwofreq = 1.0 / sqrt(o.m1 / (q1 * q1 * q1) + o.m2 / (q2 * q2 * q2));
+            // alpha c_s^2 / sqrt(gamma) / sqrt(m1/q1^3 + m2/q2^3), q = r + 1e-12 dx
+            const double q1 = fsqrt(r1s) + k.eps_dx;
+            const double q2 = fsqrt(r2s) + k.eps_dx;
+            const double c1 = q1 * q1 * q1, c2 = q2 * q2 * q2;
+            const double itwofreq = frsq((o.m1 * c2 + o.m2 * c1) * frcp(c1 * c2));
             if constexpr (NV == 3) {
                 nuL = nuR = k.visc * cs2f * itwofreq;
             } else {
-                nuL = k.visc * (k.gm * pL[3] / pL[0]) * itwofreq;
-                nuR = k.visc * (k.gm * pR[3] / pR[0]) * itwofreq;
+                nuL = k.visc * C[eL] * itwofreq;
+                nuR = k.visc * C[eR] * itwofreq;
             }
         }
         const double gxvxL = GX[1 * RXY + rL] * k.idx, gxvyL = GX[2 * RXY + rL] * k.idx;
@@ -187,38 +232,39 @@ __device__ __forceinline__ void face_flux(const StageArgs &A, int axis, const do
         const double gyvxR = GY[1 * RXY + rR] * k.idy, gyvyR = GY[2 * RXY + rR] * k.idy;
         const double offL = gxvyL + gyvxL, offR = gxvyR + gyvxR;
         const double divL = gxvxL + gyvyL, divR = gxvxR + gyvyR;
+        const double nsL = pL[0] * nuL, nsR = pR[0] * nuR;
         if (axis == 0) {
-            const double sxxL = pL[0] * (nuL * (4.0 / 3.0 * gxvxL - 2.0 / 3.0 * gyvyL) + k.lam * nuL * divL);
-            const double sxxR = pR[0] * (nuR * (4.0 / 3.0 * gxvxR - 2.0 / 3.0 * gyvyR) + k.lam * nuR * divR);
+            const double sxxL = nsL * ((4.0 / 3.0) * gxvxL - (2.0 / 3.0) * gyvyL + k.lam * divL);
+            const double sxxR = nsR * ((4.0 / 3.0) * gxvxR - (2.0 / 3.0) * gyvyR + k.lam * divR);
             tau_x = 0.5 * (sxxL + sxxR);
-            tau_y = 0.5 * (pL[0] * (nuL * offL) + pR[0] * (nuR * offR));
+            tau_y = 0.5 * (nsL * offL + nsR * offR);
         } else {
-            const double syyL = pL[0] * (nuL * (-2.0 / 3.0 * gxvxL + 4.0 / 3.0 * gyvyL) + k.lam * nuL * divL);
-            const double syyR = pR[0] * (nuR * (-2.0 / 3.0 * gxvxR + 4.0 / 3.0 * gyvyR) + k.lam * nuR * divR);
-            tau_x = 0.5 * (pL[0] * (nuL * offL) + pR[0] * (nuR * offR));
+            const double syyL = nsL * ((4.0 / 3.0) * gyvyL - (2.0 / 3.0) * gxvxL + k.lam * divL);
+            const double syyR = nsR * ((4.0 / 3.0) * gyvyR - (2.0 / 3.0) * gxvxR + k.lam * divR);
+            tau_x = 0.5 * (nsL * offL + nsR * offR);
             tau_y = 0.5 * (syyL + syyR);
         }
     }
 
     // HLLE (Toro ch. 10), outer wave speeds v_n -/+ c_s
     const double vnl = pl[1 + axis], vnr = pr[1 + axis];
-    double ul[NV], ur[NV], fl[NV], fr[NV], csl, csr;
+    double ul[NV], ur[NV], fl[NV], fr[NV], csl, csr, ppl, ppr;
     ul[0] = pl[0]; ul[1] = pl[0] * pl[1]; ul[2] = pl[0] * pl[2];
     ur[0] = pr[0]; ur[1] = pr[0] * pr[1]; ur[2] = pr[0] * pr[2];
-    double ppl, ppr;
     if constexpr (NV == 3) {
-        csl = csr = sqrt(cs2f);
+        csl = csr = fsqrt(cs2f);
         ppl = pl[0] * cs2f;
         ppr = pr[0] * cs2f;
     } else {
         ul[3] = 0.5 * pl[0] * (pl[1] * pl[1] + pl[2] * pl[2]) + pl[3] * k.inv_gm1;
         ur[3] = 0.5 * pr[0] * (pr[1] * pr[1] + pr[2] * pr[2]) + pr[3] * k.inv_gm1;
-        csl = sqrt(k.gm * pl[3] / pl[0]);
-        csr = sqrt(k.gm * pr[3] / pr[0]);
+        const double gpl = k.gm * pl[3], gpr = k.gm * pr[3];
+        csl = gpl * frsq(gpl * pl[0]);      // sqrt(gamma p / Sigma)
+        csr = gpr * frsq(gpr * pr[0]);
         ppl = pl[3];
         ppr = pr[3];
-        fl[3] = (ul[3] + pl[3]) * vnl;
-        fr[3] = (ur[3] + pr[3]) * vnr;
+        fl[3] = (ul[3] + ppl) * vnl;
+        fr[3] = (ur[3] + ppr) * vnr;
     }
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
@@ -229,7 +275,7 @@ __device__ __forceinline__ void face_flux(const StageArgs &A, int axis, const do
     fr[1 + axis] += ppr;
     const double ap = fmax(fmax(vnl + csl, vnr + csr), 0.0);
     const double am = fmin(fmin(vnl - csl, vnr - csr), 0.0);
-    const double inv = 1.0 / (ap - am);
+    const double inv = frcp(ap - am);
     const double apam = ap * am;
 #pragma unroll
     for (int v = 0; v < NV; ++v) F[v] = (fl[v] * ap - fr[v] * am - (ul[v] - ur[v]) * apam) * inv;
@@ -242,45 +288,31 @@ __device__ __forceinline__ void face_flux(const StageArgs &A, int axis, const do
     }
 }
 
-// Plummer acceleration of one body (kepler-two-body restated)
-__device__ __forceinline__ void plummer(double m, double bx, double by, double x, double y,
-                                        double rs2, double &ax, double &ay) {
-    const double dx = x - bx, dy = y - by;
-    const double t = dx * dx + dy * dy + rs2;
-    const double mag = -m / (t * sqrt(t));
-    ax = mag * dx;
-    ay = mag * dy;
-}
-
 // Isothermal / Euler source_terms (src/physics.rs:528-582, 683-774) with
-// Physics::source_terms (src/physics.rs:423-457).  it[item][v].
+// Physics::source_terms (src/physics.rs:423-457).  `sd, vx, vy, id, p0` is
+// the primitive recovery of u without floors; `brate` the static buffer
+// rate, `omega` the static beta-cooling frequency.  it[item][v].
 template <int NV>
-__device__ __forceinline__ void cell_sources(const StageArgs &A, const double *u, const double *u0,
-                                             double x, double y, double it[7][NV]) {
+__device__ __forceinline__ void cell_sources(const StageArgs &A, const double *u, const double *u0, double x,
+                                             double y, double vx, double vy, double id, double p0,
+                                             double brate, double omega, double it[7][NV]) {
     const StageConsts &k = A.k;
     const Orbit &o = A.orb;
     const double dt = A.dt;
     const double sd = u[0];
-    double ax1, ay1, ax2, ay2;
-    plummer(o.m1, o.x1, o.y1, x, y, k.rs2, ax1, ay1);
-    plummer(o.m2, o.x2, o.y2, x, y, k.rs2, ax2, ay2);
-    const double fx1 = sd * ax1, fy1 = sd * ay1, fx2 = sd * ax2, fy2 = sd * ay2;
 
+    // Plummer gravity of each body: a = -m d / (r^2 + r_s^2)^(3/2)
+    const double ex1 = x - o.x1, ey1 = y - o.y1, ex2 = x - o.x2, ey2 = y - o.y2;
+    const double q1 = ex1 * ex1 + ey1 * ey1, q2 = ex2 * ex2 + ey2 * ey2;
+    const double y1 = frsq(q1 + k.rs2), y2 = frsq(q2 + k.rs2);
+    const double g1 = -o.m1 * (y1 * y1 * y1) * sd, g2 = -o.m2 * (y2 * y2 * y2) * sd;
+    const double fx1 = g1 * ex1, fy1 = g1 * ey1, fx2 = g2 * ex2, fy2 = g2 * ey2;
+
+    // sink kernel rate (1 - r^2/s^2)^2 inside the sink radius
     double sr1 = 0.0, sr2 = 0.0;
-    {
-        const double dx1 = x - o.x1, dy1 = y - o.y1, dx2 = x - o.x2, dy2 = y - o.y2;
-        const double q1 = dx1 * dx1 + dy1 * dy1, q2 = dx2 * dx2 + dy2 * dy2;
-        if (q1 < k.sink_s2) { const double w = 1.0 - q1 * k.inv_sink_s2; sr1 = k.sink_rate * (w * w); }
-        if (q2 < k.sink_s2) { const double w = 1.0 - q2 * k.inv_sink_s2; sr2 = k.sink_rate * (w * w); }
-    }
-    const double r = sqrt(x * x + y * y);
-    const double brate = k.buffer_coef * (1.0 + tanh((r - k.domain_radius) * k.inv_buffer_scale));
+    if (q1 < k.sink_s2) { const double w = 1.0 - q1 * k.inv_sink_s2; sr1 = k.sink_rate * (w * w); }
+    if (q2 < k.sink_s2) { const double w = 1.0 - q2 * k.inv_sink_s2; sr2 = k.sink_rate * (w * w); }
 
-    // primitive velocities (no floors, as the reference's to_primitive)
-    const double id = 1.0 / sd;
-    const double vx = u[1] * id, vy = u[2] * id;
-
-    // sinks
     double s1[NV], s2[NV];
     if (k.sink_model == CDC_SINK_IMPARTIAL) {
 #pragma unroll
@@ -288,9 +320,7 @@ __device__ __forceinline__ void cell_sources(const StageArgs &A, const double *u
     } else {
         const double vx1 = k.one_body ? 0.0 : o.vx1, vy1 = k.one_body ? 0.0 : o.vy1;
         const double vx2 = k.one_body ? 0.0 : o.vx2, vy2 = k.one_body ? 0.0 : o.vy2;
-        const double ex1 = x - o.x1, ey1 = y - o.y1, ex2 = x - o.x2, ey2 = y - o.y2;
-        const double ir1 = 1.0 / sqrt(ex1 * ex1 + ey1 * ey1);
-        const double ir2 = 1.0 / sqrt(ex2 * ex2 + ey2 * ey2);
+        const double ir1 = frsq(q1), ir2 = frsq(q2);
         const double r1x = ex1 * ir1, r1y = ey1 * ir1, r2x = ex2 * ir2, r2y = ey2 * ir2;
         const double dv1 = (vx - vx1) * r1x + (vy - vy1) * r1y;
         const double dv2 = (vx - vx2) * r2x + (vy - vy2) * r2y;
@@ -299,16 +329,17 @@ __device__ __forceinline__ void cell_sources(const StageArgs &A, const double *u
         s1[0] = sd * -sr1; s1[1] = (sd * vx1c) * -sr1; s1[2] = (sd * vy1c) * -sr1;
         s2[0] = sd * -sr2; s2[1] = (sd * vx2c) * -sr2; s2[2] = (sd * vy2c) * -sr2;
         if constexpr (NV == 4) {
-            const double e0 = ((u[3] - 0.5 * sd * (vx * vx + vy * vy)) * k.gm1) * (id * k.inv_gm1);
+            const double e0 = p0 * (id * k.inv_gm1);
             s1[3] = (sd * e0 + 0.5 * sd * (vx1c * vx1c + vy1c * vy1c)) * -sr1;
             s2[3] = (sd * e0 + 0.5 * sd * (vx2c * vx2c + vy2c * vy2c)) * -sr2;
         }
     }
+    const double bfac = -dt * brate;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         it[CDC_ITEM_SINK1][v] = s1[v] * dt;
         it[CDC_ITEM_SINK2][v] = s2[v] * dt;
-        it[CDC_ITEM_BUFFER][v] = (u[v] - u0[v]) * (-dt * brate);
+        it[CDC_ITEM_BUFFER][v] = (u[v] - u0[v]) * bfac;
     }
     it[CDC_ITEM_GRAV1][0] = 0.0; it[CDC_ITEM_GRAV1][1] = fx1 * dt; it[CDC_ITEM_GRAV1][2] = fy1 * dt;
     it[CDC_ITEM_GRAV2][0] = 0.0; it[CDC_ITEM_GRAV2][1] = fx2 * dt; it[CDC_ITEM_GRAV2][2] = fy2 * dt;
@@ -326,15 +357,12 @@ __device__ __forceinline__ void cell_sources(const StageArgs &A, const double *u
     } else {
         it[CDC_ITEM_GRAV1][3] = (fx1 * vx + fy1 * vy) * dt;
         it[CDC_ITEM_GRAV2][3] = (fx2 * vx + fy2 * vy) * dt;
-        const double p0 = (u[3] - 0.5 * sd * (vx * vx + vy * vy)) * k.gm1;
         const double e0 = p0 * (id * k.inv_gm1);
         double ec;
         if (k.cooling_model == CDC_COOLING_BETA) {
-            const double r2 = x * x + y * y + k.rs2;
-            const double omega = sqrt(k.mtot / (r2 * sqrt(r2)));
-            const double b0 = u0[0], ib0 = 1.0 / b0;
+            const double ib0 = frcp(u0[0]);
             const double bvx = u0[1] * ib0, bvy = u0[2] * ib0;
-            const double bp = (u0[3] - 0.5 * b0 * (bvx * bvx + bvy * bvy)) * k.gm1;
+            const double bp = (u0[3] - 0.5 * u0[0] * (bvx * bvx + bvy * bvy)) * k.gm1;
             const double et = bp * (ib0 * k.inv_gm1);
             ec = et + (e0 - et) * exp(-dt * omega * k.cooling_inv_beta);
         } else {
@@ -354,6 +382,12 @@ __device__ __forceinline__ void cell_sources(const StageArgs &A, const double *u
     }
 }
 
+__device__ __forceinline__ double coherent_load(const double *p) {
+    const uint64_t b = __hip_atomic_load(reinterpret_cast<const uint64_t *>(p), __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    return __builtin_bit_cast(double, b);
+}
+
 }  // namespace dev
 
 // ---------------------------------------------------------------------------
@@ -367,13 +401,17 @@ __global__ void __launch_bounds__(NT) stage_kernel(const StageArgs A) {
     constexpr int CPT = TX * TY / NT;
     constexpr int NS = SumMap<NV>::n, NR = NS + 1;
     constexpr int CH = 16, CL = NT / CH;
-    constexpr int LDS_A = NV * EXY + 2 * NV * RXY;
+    constexpr int NC = NV == 4 ? EXY : 0;       // cell c_s^2 plane (Euler alpha viscosity)
+    constexpr int LDS_A = NV * EXY + NC + 2 * NV * RXY;
     constexpr int LDS_R = NR * (NT + CH);
     constexpr int LDS = LDS_A > LDS_R ? LDS_A : LDS_R;
     static_assert(NV * NF <= 2 * NV * RXY, "flux staging must fit the gradient region");
     static_assert((TX * TY) % NT == 0 && NT % CH == 0, "tile / thread mismatch");
+    static_assert(NR <= 64 && 7 * 4 <= 64, "block tail works in one wave");
     __shared__ double lds[LDS];
-    __shared__ int s_key;
+    __shared__ double s_ds[7 * NV];
+    __shared__ double s_de[7 * 4];
+    __shared__ int s_key, s_last, s_perr;
 
     const int ef = *(volatile const int32_t *)A.err_flag;
     if (ef != 0 && ef < A.stage_id) return;
@@ -390,9 +428,11 @@ __global__ void __launch_bounds__(NT) stage_kernel(const StageArgs A) {
     const size_t NN = (size_t)N * N;
     const dev::Axis axx = dev::block_axis(A.R, A.L, bi, bs);
     const dev::Axis axy = dev::block_axis(A.R, A.L, bj, bs);
+    const bool alpha = A.k.viscosity_model == CDC_VISC_ALPHA;
 
     double *P = lds;
-    double *GX = lds + NV * EXY;
+    double *C = lds + NV * EXY;
+    double *GX = C + NC;
     double *GY = GX + NV * RXY;
 
     // ---- phase 1: primitives on the tile + 2-zone halo --------------------
@@ -405,16 +445,19 @@ __global__ void __launch_bounds__(NT) stage_kernel(const StageArgs A) {
         gi = gi < 0 ? gi + N : (gi >= N ? gi - N : gi);
         gj = gj < 0 ? gj + N : (gj >= N ? gj - N : gj);
         const size_t g = (size_t)gi * N + gj;
-        double u[NV], p[NV], val = 0.0;
+        double u[NV], p[NV], val = 0.0, id;
 #pragma unroll
         for (int v = 0; v < NV; ++v) u[v] = A.uc[v * NN + g];
-        const int code = dev::try_to_primitive<NV>(A.k, u, p, val);
+        const int code = dev::try_to_primitive<NV>(A.k, u, p, val, id);
         if (code && ei >= 2 && ei < TX + 2 && ej >= 2 && ej < TY + 2) {
             const int c = (li0 + ei - 2) * bs + (lj0 + ej - 2);
             if (c < key) { key = c; kcode = code; kval = val; }
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v) P[v * EXY + e] = p[v];
+        if constexpr (NV == 4) {
+            if (alpha) C[e] = A.k.gm * p[3] * id;
+        }
     }
     __syncthreads();
 
@@ -446,14 +489,14 @@ __global__ void __launch_bounds__(NT) stage_kernel(const StageArgs A) {
             const int fi = f / TY, fj = f - (f / TY) * TY;
             const int eL = (fi + 1) * EY + fj + 2;
             const int rL = fi * RY + fj + 1;
-            dev::face_flux<NV, EXY, RXY>(A, 0, P, GX, GY, eL, eL + EY, rL, rL + RY,
+            dev::face_flux<NV, EXY, RXY>(A, 0, P, C, GX, GY, eL, eL + EY, rL, rL + RY,
                                          axx.v(li0 + fi), axy.c(lj0 + fj), fl[q]);
         } else if (f < NF) {
             const int g = f - NFX;
             const int fi = g / (TY + 1), fj = g - (g / (TY + 1)) * (TY + 1);
             const int eL = (fi + 2) * EY + fj + 1;
             const int rL = (fi + 1) * RY + fj;
-            dev::face_flux<NV, EXY, RXY>(A, 1, P, GX, GY, eL, eL + 1, rL, rL + 1,
+            dev::face_flux<NV, EXY, RXY>(A, 1, P, C, GX, GY, eL, eL + 1, rL, rL + 1,
                                          axx.c(li0 + fi), axy.v(lj0 + fj), fl[q]);
         }
     }
@@ -496,8 +539,23 @@ __global__ void __launch_bounds__(NT) stage_kernel(const StageArgs A) {
             const double *fy = FY + v * NFY + ci * (TY + 1) + cj;
             du[v] = ((fx[TY] - fx[0]) * A.k.idx + (fy[1] - fy[0]) * A.k.idy) * ndt;
         }
+        // primitive recovery of u without floors (the sources' to_primitive)
+        double vx, vy, id, p0 = 0.0;
+        if (A.reuse_prim) {
+            const int e = (ci + 2) * EY + (cj + 2);
+            vx = P[1 * EXY + e];
+            vy = P[2 * EXY + e];
+            id = dev::frcp(uc[0]);
+        } else {
+            id = dev::frcp(uc[0]);
+            vx = uc[1] * id;
+            vy = uc[2] * id;
+        }
+        if constexpr (NV == 4) p0 = (uc[3] - 0.5 * uc[0] * (vx * vx + vy * vy)) * A.k.gm1;
+        const double brate = A.aux[g];
+        const double omega = A.k.cooling_model == CDC_COOLING_BETA ? A.aux[NN + g] : 0.0;
         double it[7][NV];
-        dev::cell_sources<NV>(A, uc, u0, axx.c(li0 + ci), axy.c(lj0 + cj), it);
+        dev::cell_sources<NV>(A, uc, u0, axx.c(li0 + ci), axy.c(lj0 + cj), vx, vy, id, p0, brate, omega, it);
         double out[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -555,46 +613,84 @@ __global__ void __launch_bounds__(NT) stage_kernel(const StageArgs A) {
         A.tile_err[tile] = te;
         atomicCAS(A.err_flag, 0, A.stage_id);
     }
-}
 
-// One thread per block: fixed-order tile sum, dx*dy, perturbation, RK combine
-// of integrated_source_terms and orbital_elements_change
-// (src/scheme.rs:307-315, src/state.rs:195-197).
-template <int NV>
-__global__ void __launch_bounds__(64) finalize_kernel(const FinalizeArgs F) {
-    constexpr int NS = SumMap<NV>::n, NR = NS + 1;
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= F.nblk) return;
-    const int ef = *(volatile const int32_t *)F.err_flag;
-    if (ef != 0 && ef < F.stage_id) return;
-    double cs[NS];
-#pragma unroll
-    for (int s = 0; s < NS; ++s) cs[s] = 0.0;
-    for (int t = 0; t < F.tiles_per_block; ++t) {
-        const double *q = F.tile_sums + ((size_t)b * F.tiles_per_block + t) * NR;
-#pragma unroll
-        for (int s = 0; s < NS; ++s) cs[s] += q[s];
+    // ---- block tail: the last tile of the block to finish ------------------
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+        const int prev = atomicAdd(&A.blk_count[b], 1);
+        s_last = prev == tpb - 1;
     }
-    double ds[7 * NV];
-#pragma unroll
-    for (int k = 0; k < 7 * NV; ++k) ds[k] = 0.0;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) ds[SumMap<NV>::item(s) * NV + SumMap<NV>::comp(s)] = cs[s] * F.area;
-    double de[7 * 4];
-    if (!perturbation(F.orb, F.el, ds, NV, de)) {
-        F.perr[b] = CDC_ERR_UNBOUND_ORBIT;
-        atomicCAS(F.err_flag, 0, F.stage_id);
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    if (tid < NS) {
+        // fixed tile order, independent of arrival order
+        double t = 0.0;
+        for (int j = 0; j < tpb; ++j) t += dev::coherent_load(A.tile_sums + ((size_t)b * tpb + j) * NR + tid);
+        s_ds[SumMap<NV>::item(tid) * NV + SumMap<NV>::comp(tid)] = t * A.area;
+    } else if (tid < 7 * NV + NS) {
+        // structurally zero components
+        const int z = tid - NS;
+        bool nonzero = false;
+        for (int s = 0; s < NS; ++s) nonzero |= (SumMap<NV>::item(s) * NV + SumMap<NV>::comp(s)) == z;
+        if (!nonzero) s_ds[z] = 0.0;
+    }
+    if (tid == 0) {
+        s_perr = 0;
+        A.blk_count[b] = 0;
+    }
+    __syncthreads();
+    if (tid < 4) {
+        // ItemizedChange::perturbation (src/state.rs:258-282), one item per lane
+        const int item = tid == 0 ? CDC_ITEM_SINK1 : tid == 1 ? CDC_ITEM_SINK2 : tid == 2 ? CDC_ITEM_GRAV1 : CDC_ITEM_GRAV2;
+        const double *d = s_ds + item * NV;
+        double p[4];
+        const bool ok = (tid % 2 == 0) ? perturb(A.orb, -d[0], 0.0, -d[1], 0.0, -d[2], 0.0, p)
+                                       : perturb(A.orb, 0.0, -d[0], 0.0, -d[1], 0.0, -d[2], p);
+        for (int c = 0; c < 4; ++c) s_de[item * 4 + c] = ok ? p[c] - A.el[c] : 0.0;
+        if (!ok) atomicOr(&s_perr, 1);
+    } else if (tid < 4 + 3 * 4) {
+        const int z = tid - 4;   // buffer, cooling, fake_mass items: no orbital change
+        s_de[(CDC_ITEM_BUFFER + z / 4) * 4 + (z % 4)] = 0.0;
+    }
+    __syncthreads();
+    if (s_perr) {
+        if (tid == 0) {
+            A.perr[b] = CDC_ERR_UNBOUND_ORBIT;
+            atomicCAS(A.err_flag, 0, A.stage_id);
+        }
         return;
     }
-    F.perr[b] = 0;
+    if (tid == 0) A.perr[b] = 0;
     const size_t oi = (size_t)b * 7 * NV, oe = (size_t)b * 7 * 4;
-    for (int k = 0; k < 7 * NV; ++k) {
-        const double s1 = F.ist_c[oi + k] + ds[k];
-        F.ist_o[oi + k] = F.read_base ? s1 * F.c1 + F.ist_b[oi + k] * F.c0 : s1;
+    if (tid < 7 * 4) {
+        if (tid < 7 * NV) {
+            const double s1 = A.ist_c[oi + tid] + s_ds[tid];
+            A.ist_o[oi + tid] = A.read_base ? s1 * A.c1 + A.ist_b[oi + tid] * A.c0 : s1;
+        }
+        const double s1 = A.oec_c[oe + tid] + s_de[tid];
+        A.oec_o[oe + tid] = A.read_base ? s1 * A.c1 + A.oec_b[oe + tid] * A.c0 : s1;
     }
-    for (int k = 0; k < 7 * 4; ++k) {
-        const double s1 = F.oec_c[oe + k] + de[k];
-        F.oec_o[oe + k] = F.read_base ? s1 * F.c1 + F.oec_b[oe + k] * F.c0 : s1;
+}
+
+// Static per-cell planes (see AuxArgs), IEEE arithmetic in the reference's
+// operation order.
+__global__ void __launch_bounds__(256) aux_kernel(const AuxArgs a) {
+#pragma clang fp contract(off)
+    const int64_t N = a.N, NN = N * N;
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= NN) return;
+    const int gi = (int)(c / N), gj = (int)(c - (c / N) * N);
+    const dev::Axis ax = dev::block_axis(a.R, a.L, gi / a.bs, a.bs);
+    const dev::Axis ay = dev::block_axis(a.R, a.L, gj / a.bs, a.bs);
+    const double x = ax.c(gi % a.bs), y = ay.c(gj % a.bs);
+    const double r = sqrt(x * x + y * y);
+    const double yb = (r - a.R) / a.buffer_scale;
+    a.aux[c] = 0.5 * a.buffer_rate * (1.0 + tanh(yb)) * a.omega_outer;
+    if (a.naux > 1) {
+        const double r2 = x * x + y * y + a.rs2;
+        a.aux[NN + c] = sqrt(a.mtot / (r2 * sqrt(r2)));
     }
 }
 
@@ -686,10 +782,9 @@ hipError_t launch_stage(int nv, const TileShape &ts, const StageArgs &a, hipStre
     return launch_stage_t<4, 8, 8, 64>(a, s);
 }
 
-hipError_t launch_finalize(int nv, const FinalizeArgs &a, hipStream_t s) {
-    const int grid = (a.nblk + 63) / 64;
-    if (nv == 3) hipLaunchKernelGGL(finalize_kernel<3>, dim3(grid), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL(finalize_kernel<4>, dim3(grid), dim3(64), 0, s, a);
+hipError_t launch_aux(const AuxArgs &a, hipStream_t s) {
+    const int64_t n = (int64_t)a.N * a.N;
+    hipLaunchKernelGGL(aux_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

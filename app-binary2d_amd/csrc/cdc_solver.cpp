@@ -90,6 +90,9 @@ struct cdc_solver {
     cdc::TileErr *d_tile_err = nullptr;
     int32_t *d_perr = nullptr;
     int32_t *d_err_flag = nullptr;
+    int32_t *d_blk_count = nullptr;  // per-block tile arrivals within a stage
+    double *d_aux = nullptr;     // static per-cell planes (buffer rate, beta-cooling Omega)
+    int naux = 1;
     double *d_stage = nullptr;   // AoS staging
     double *d_amax = nullptr;    // max-speed partials
     int namax = 0;
@@ -158,11 +161,6 @@ cdc::StageConsts make_consts(const cdc_solver *s, double time, double dt) {
     k.eps_dx = 1e-12 * k.dx;
     double el[4];
     cdc::orbital_elements(p, el);
-    const double R = s->mesh.domain_radius;
-    k.mtot = el[1];
-    k.domain_radius = R;
-    k.inv_buffer_scale = 1.0 / p.buffer_scale;
-    k.buffer_coef = 0.5 * p.buffer_rate * sqrt(el[1] / (R * R * R));
     k.sink_s2 = p.sink_radius * p.sink_radius;
     k.inv_sink_s2 = 1.0 / k.sink_s2;
     k.sink_rate = p.sink_rate;
@@ -198,6 +196,8 @@ void free_all(cdc_solver *s) {
     (void)hipFree(s->d_tile_err);
     (void)hipFree(s->d_perr);
     (void)hipFree(s->d_err_flag);
+    (void)hipFree(s->d_blk_count);
+    (void)hipFree(s->d_aux);
     (void)hipFree(s->d_stage);
     (void)hipFree(s->d_amax);
     for (auto e : s->ev) (void)hipEventDestroy(e);
@@ -225,6 +225,30 @@ void cell_centre(const cdc_solver *s, int b, int cell, double *x, double *y) {
 int upload_conserved(cdc_solver *s, const double *aos, double *soa) {
     HIP_TRY(hipMemcpyAsync(s->d_stage, aos, s->nstate() * sizeof(double), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(cdc::launch_aos_to_soa(s->nv, s->nb, s->bs, s->d_stage, soa, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return CDC_OK;
+}
+
+// Static per-cell planes for the current physics (buffer rate, Omega)
+int compute_aux(cdc_solver *s) {
+    double el[4];
+    cdc::orbital_elements(s->phys, el);
+    const double R = s->mesh.domain_radius;
+    cdc::AuxArgs a;
+    memset(&a, 0, sizeof(a));
+    a.aux = s->d_aux;
+    a.N = s->N;
+    a.nb = s->nb;
+    a.bs = s->bs;
+    a.naux = s->naux;
+    a.R = R;
+    a.L = 2.0 * R / (double)s->nb;
+    a.buffer_rate = s->phys.buffer_rate;
+    a.buffer_scale = s->phys.buffer_scale;
+    a.omega_outer = sqrt(el[1] / (R * R * R));
+    a.rs2 = softening_length(s->phys) * softening_length(s->phys);
+    a.mtot = el[1];
+    HIP_TRY(cdc::launch_aux(a, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return CDC_OK;
 }
@@ -344,6 +368,9 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
     ALLOC(s->d_tile_err, (size_t)s->ntiles * sizeof(cdc::TileErr));
     ALLOC(s->d_perr, (size_t)s->nblk * sizeof(int32_t));
     ALLOC(s->d_err_flag, sizeof(int32_t));
+    ALLOC(s->d_blk_count, (size_t)s->nblk * sizeof(int32_t));
+    s->naux = (hydro->kind == CDC_EULER && hydro->cooling_model == CDC_COOLING_BETA) ? 2 : 1;
+    ALLOC(s->d_aux, (size_t)s->naux * s->ncell() * sizeof(double));
     s->namax = 1024;
     ALLOC(s->d_amax, (size_t)s->namax * sizeof(double));
 #undef ALLOC
@@ -353,8 +380,11 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
     if (hipMemcpyAsync(s->d_u[0], s->d_init, sb, hipMemcpyDeviceToDevice, s->stream) != hipSuccess ||
         hipMemsetAsync(s->d_ist[0], 0, (size_t)s->nblk * 7 * s->nv * sizeof(double), s->stream) != hipSuccess ||
         hipMemsetAsync(s->d_oec[0], 0, (size_t)s->nblk * 7 * 4 * sizeof(double), s->stream) != hipSuccess ||
+        hipMemsetAsync(s->d_blk_count, 0, (size_t)s->nblk * sizeof(int32_t), s->stream) != hipSuccess ||
         hipStreamSynchronize(s->stream) != hipSuccess)
         return fail(CDC_ERR_HIP, "initial state upload failed");
+    rc = compute_aux(s);
+    if (rc) return fail(rc, g_msg);
     s->cur = 0;
     s->time = 0.0;
     s->num = 0;
@@ -375,7 +405,7 @@ int32_t cdc_solver_set_physics(cdc_solver_t *s, const cdc_physics_t *physics) {
     int rc = validate(&s->mesh, &s->hydro, physics);
     if (rc) return rc;
     s->phys = *physics;
-    return CDC_OK;
+    return compute_aux(s);
 }
 
 int32_t cdc_solver_set_state(cdc_solver_t *s, double time, int64_t num, int64_t den, const double *conserved,
@@ -522,9 +552,19 @@ int32_t cdc_solver_advance(cdc_solver_t *s, int32_t fold, double *dt_out, cdc_er
         a.u0 = s->d_init;
         a.ub = s->d_u[base.idx];
         a.uo = s->d_u[out.idx];
+        a.aux = s->d_aux;
         a.tile_sums = s->d_tile_sums;
         a.tile_err = s->d_tile_err;
         a.err_flag = s->d_err_flag;
+        a.blk_count = s->d_blk_count;
+        a.ist_c = s->d_ist[in.idx];
+        a.ist_b = s->d_ist[base.idx];
+        a.oec_c = s->d_oec[in.idx];
+        a.oec_b = s->d_oec[base.idx];
+        a.ist_o = s->d_ist[out.idx];
+        a.oec_o = s->d_oec[out.idx];
+        a.perr = s->d_perr;
+        a.reuse_prim = s->hydro.has_density_floor ? 0 : 1;
         a.N = s->N;
         a.nb = s->nb;
         a.bs = s->bs;
@@ -540,32 +580,14 @@ int32_t cdc_solver_advance(cdc_solver_t *s, int32_t fold, double *dt_out, cdc_er
         a.c0 = bf;
         a.orb = orb;
         a.k = make_consts(s, in.time, dt);
+        a.area = a.k.dx * a.k.dy;
+        for (int i = 0; i < 4; ++i) a.el[i] = el[i];
         if (s->timing) HIP_TRY(hipEventRecord(s->ev[nev], s->stream));
         HIP_TRY(cdc::launch_stage(s->nv, s->ts, a, s->stream));
         if (s->timing) HIP_TRY(hipEventRecord(s->ev[nev + 1], s->stream));
         if (s->timing) nev += 2;
 
-        cdc::FinalizeArgs f;
-        memset(&f, 0, sizeof(f));
-        f.tile_sums = s->d_tile_sums;
-        f.ist_c = s->d_ist[in.idx];
-        f.ist_b = s->d_ist[base.idx];
-        f.oec_c = s->d_oec[in.idx];
-        f.oec_b = s->d_oec[base.idx];
-        f.ist_o = s->d_ist[out.idx];
-        f.oec_o = s->d_oec[out.idx];
-        f.perr = s->d_perr;
-        f.err_flag = s->d_err_flag;
-        f.nblk = s->nblk;
-        f.tiles_per_block = s->tpb;
-        f.stage_id = sid;
-        f.read_base = read_base ? 1 : 0;
-        f.c1 = c1;
-        f.c0 = bf;
-        f.area = a.k.dx * a.k.dy;
-        f.orb = orb;
-        for (int i = 0; i < 4; ++i) f.el[i] = el[i];
-        HIP_TRY(cdc::launch_finalize(s->nv, f, s->stream));
+
 
         // time and iteration of the stage result, then of the combine
         double t1 = in.time + dt;
