@@ -58,7 +58,6 @@ struct StageArgs {
     double *tile_sums;     // [ntiles][NSUM + 1] (last slot: max signal speed)
     TileErr *tile_err;     // [ntiles]
     int32_t *err_flag;     // stage id of the first failing stage, 0 if none
-    int32_t *blk_count;    // [nblk] tiles of the block done this stage (reset by the last)
     // per-block integrated_source_terms / orbital_elements_change
     const double *ist_c, *ist_b;   // [nblk][7][nv]
     const double *oec_c, *oec_b;   // [nblk][7][4]
@@ -82,6 +81,7 @@ TileShape tile_shape_for(int bs, int nv);
 int nsum_for(int nv);
 
 hipError_t launch_stage(int nv, const TileShape &ts, const StageArgs &a, hipStream_t s);
+hipError_t launch_finalize(int nv, const StageArgs &a, int tpb, hipStream_t s);
 hipError_t launch_aux(const AuxArgs &a, hipStream_t s);
 hipError_t launch_max_speed(int nv, const double *u, int64_t ncell, double gm, double *partial,
                             int nparts, hipStream_t s);

@@ -15,11 +15,10 @@
 //            cooling), the low-storage RK combine with the step base, and the
 //            per-tile sums of the itemized source terms
 //            (src/scheme.rs:273-316, src/state.rs:183-200)
-//   tail     the last tile of each block to finish sums the block's tiles in
-//            a fixed order, scales by dx*dy, applies the orbital-element
-//            perturbation and the RK combine of the block's
-//            integrated_source_terms / orbital_elements_change
-//            (src/scheme.rs:307-315, src/state.rs:195-197)
+// finalize_kernel then sums each block's tiles in a fixed order, scales by
+// dx*dy, applies the orbital-element perturbation and the RK combine of the
+// block's integrated_source_terms / orbital_elements_change
+// (src/scheme.rs:307-315, src/state.rs:195-197).
 //
 // FP64 reciprocals and square roots use the hardware estimates (v_rcp_f64,
 // v_rsq_f64) refined by Newton-Raphson steps to within an ulp or so of the
@@ -382,12 +381,6 @@ __device__ __forceinline__ void cell_sources(const StageArgs &A, const double *u
     }
 }
 
-__device__ __forceinline__ double coherent_load(const double *p) {
-    const uint64_t b = __hip_atomic_load(reinterpret_cast<const uint64_t *>(p), __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-    return __builtin_bit_cast(double, b);
-}
-
 }  // namespace dev
 
 // ---------------------------------------------------------------------------
@@ -407,11 +400,8 @@ __global__ void __launch_bounds__(NT) stage_kernel(const StageArgs A) {
     constexpr int LDS = LDS_A > LDS_R ? LDS_A : LDS_R;
     static_assert(NV * NF <= 2 * NV * RXY, "flux staging must fit the gradient region");
     static_assert((TX * TY) % NT == 0 && NT % CH == 0, "tile / thread mismatch");
-    static_assert(NR <= 64 && 7 * 4 <= 64, "block tail works in one wave");
     __shared__ double lds[LDS];
-    __shared__ double s_ds[7 * NV];
-    __shared__ double s_de[7 * 4];
-    __shared__ int s_key, s_last, s_perr;
+    __shared__ int s_key;
 
     const int ef = *(volatile const int32_t *)A.err_flag;
     if (ef != 0 && ef < A.stage_id) return;
@@ -613,36 +603,37 @@ __global__ void __launch_bounds__(NT) stage_kernel(const StageArgs A) {
         A.tile_err[tile] = te;
         atomicCAS(A.err_flag, 0, A.stage_id);
     }
+}
 
-    // ---- block tail: the last tile of the block to finish ------------------
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) {
-        const int prev = atomicAdd(&A.blk_count[b], 1);
-        s_last = prev == tpb - 1;
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
+// The per-block tail of a stage, one wave per block: the block's tile sums
+// in tile order, scaled by dx*dy, the four orbital-element perturbations
+// (ItemizedChange::perturbation, src/state.rs:258-282) one per lane, and the
+// RK combine of integrated_source_terms / orbital_elements_change
+// (src/scheme.rs:307-315, src/state.rs:195-197).
+template <int NV>
+__global__ void __launch_bounds__(64) finalize_kernel(const StageArgs A, int tpb) {
+    constexpr int NS = SumMap<NV>::n, NR = NS + 1;
+    static_assert(NS + 7 * NV <= 64, "one wave per block");
+    __shared__ double s_ds[7 * NV];
+    __shared__ double s_de[7 * 4];
+    __shared__ int s_perr;
+    const int ef = *(volatile const int32_t *)A.err_flag;
+    if (ef != 0 && ef < A.stage_id) return;
+    const int b = blockIdx.x, tid = threadIdx.x;
     if (tid < NS) {
-        // fixed tile order, independent of arrival order
         double t = 0.0;
-        for (int j = 0; j < tpb; ++j) t += dev::coherent_load(A.tile_sums + ((size_t)b * tpb + j) * NR + tid);
+        for (int j = 0; j < tpb; ++j) t += A.tile_sums[((size_t)b * tpb + j) * NR + tid];
         s_ds[SumMap<NV>::item(tid) * NV + SumMap<NV>::comp(tid)] = t * A.area;
-    } else if (tid < 7 * NV + NS) {
+    } else if (tid < NS + 7 * NV) {
         // structurally zero components
         const int z = tid - NS;
         bool nonzero = false;
         for (int s = 0; s < NS; ++s) nonzero |= (SumMap<NV>::item(s) * NV + SumMap<NV>::comp(s)) == z;
         if (!nonzero) s_ds[z] = 0.0;
     }
-    if (tid == 0) {
-        s_perr = 0;
-        A.blk_count[b] = 0;
-    }
+    if (tid == 0) s_perr = 0;
     __syncthreads();
     if (tid < 4) {
-        // ItemizedChange::perturbation (src/state.rs:258-282), one item per lane
         const int item = tid == 0 ? CDC_ITEM_SINK1 : tid == 1 ? CDC_ITEM_SINK2 : tid == 2 ? CDC_ITEM_GRAV1 : CDC_ITEM_GRAV2;
         const double *d = s_ds + item * NV;
         double p[4];
@@ -780,6 +771,13 @@ hipError_t launch_stage(int nv, const TileShape &ts, const StageArgs &a, hipStre
     if (ts.tx == 16 && ts.ty == 32) return launch_stage_t<4, 16, 32, 256>(a, s);
     if (ts.tx == 16 && ts.ty == 16) return launch_stage_t<4, 16, 16, 256>(a, s);
     return launch_stage_t<4, 8, 8, 64>(a, s);
+}
+
+hipError_t launch_finalize(int nv, const StageArgs &a, int tpb, hipStream_t s) {
+    const int nblk = a.nb * a.nb;
+    if (nv == 3) hipLaunchKernelGGL(finalize_kernel<3>, dim3(nblk), dim3(64), 0, s, a, tpb);
+    else hipLaunchKernelGGL(finalize_kernel<4>, dim3(nblk), dim3(64), 0, s, a, tpb);
+    return hipGetLastError();
 }
 
 hipError_t launch_aux(const AuxArgs &a, hipStream_t s) {

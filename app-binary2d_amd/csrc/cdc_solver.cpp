@@ -90,7 +90,6 @@ struct cdc_solver {
     cdc::TileErr *d_tile_err = nullptr;
     int32_t *d_perr = nullptr;
     int32_t *d_err_flag = nullptr;
-    int32_t *d_blk_count = nullptr;  // per-block tile arrivals within a stage
     double *d_aux = nullptr;     // static per-cell planes (buffer rate, beta-cooling Omega)
     int naux = 1;
     double *d_stage = nullptr;   // AoS staging
@@ -196,7 +195,6 @@ void free_all(cdc_solver *s) {
     (void)hipFree(s->d_tile_err);
     (void)hipFree(s->d_perr);
     (void)hipFree(s->d_err_flag);
-    (void)hipFree(s->d_blk_count);
     (void)hipFree(s->d_aux);
     (void)hipFree(s->d_stage);
     (void)hipFree(s->d_amax);
@@ -368,7 +366,6 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
     ALLOC(s->d_tile_err, (size_t)s->ntiles * sizeof(cdc::TileErr));
     ALLOC(s->d_perr, (size_t)s->nblk * sizeof(int32_t));
     ALLOC(s->d_err_flag, sizeof(int32_t));
-    ALLOC(s->d_blk_count, (size_t)s->nblk * sizeof(int32_t));
     s->naux = (hydro->kind == CDC_EULER && hydro->cooling_model == CDC_COOLING_BETA) ? 2 : 1;
     ALLOC(s->d_aux, (size_t)s->naux * s->ncell() * sizeof(double));
     s->namax = 1024;
@@ -380,7 +377,6 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
     if (hipMemcpyAsync(s->d_u[0], s->d_init, sb, hipMemcpyDeviceToDevice, s->stream) != hipSuccess ||
         hipMemsetAsync(s->d_ist[0], 0, (size_t)s->nblk * 7 * s->nv * sizeof(double), s->stream) != hipSuccess ||
         hipMemsetAsync(s->d_oec[0], 0, (size_t)s->nblk * 7 * 4 * sizeof(double), s->stream) != hipSuccess ||
-        hipMemsetAsync(s->d_blk_count, 0, (size_t)s->nblk * sizeof(int32_t), s->stream) != hipSuccess ||
         hipStreamSynchronize(s->stream) != hipSuccess)
         return fail(CDC_ERR_HIP, "initial state upload failed");
     rc = compute_aux(s);
@@ -556,7 +552,6 @@ int32_t cdc_solver_advance(cdc_solver_t *s, int32_t fold, double *dt_out, cdc_er
         a.tile_sums = s->d_tile_sums;
         a.tile_err = s->d_tile_err;
         a.err_flag = s->d_err_flag;
-        a.blk_count = s->d_blk_count;
         a.ist_c = s->d_ist[in.idx];
         a.ist_b = s->d_ist[base.idx];
         a.oec_c = s->d_oec[in.idx];
@@ -586,6 +581,7 @@ int32_t cdc_solver_advance(cdc_solver_t *s, int32_t fold, double *dt_out, cdc_er
         HIP_TRY(cdc::launch_stage(s->nv, s->ts, a, s->stream));
         if (s->timing) HIP_TRY(hipEventRecord(s->ev[nev + 1], s->stream));
         if (s->timing) nev += 2;
+        HIP_TRY(cdc::launch_finalize(s->nv, a, s->tpb, s->stream));
 
 
 
