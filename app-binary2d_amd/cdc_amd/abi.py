@@ -137,6 +137,9 @@ SIGNATURES = {
     "cdc_solver_max_signal_speed": (c_int32, [c_void_p, POINTER(c_double)]),
     "cdc_solver_set_timing": (c_int32, [c_void_p, c_int32]),
     "cdc_solver_get_timing": (c_int32, [c_void_p, POINTER(c_int64), POINTER(c_double), POINTER(c_double)]),
+    "cdc_solver_set_tile": (c_int32, [c_void_p, c_int32, c_int32]),
+    "cdc_solver_get_tile": (c_int32, [c_void_p, POINTER(c_int32), POINTER(c_int32)]),
+    "cdc_solver_autotune": (c_int32, [c_void_p, c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_double)]),
     "cdc_orbital_state_from_time": (c_int32, [POINTER(CPhysics), c_double, POINTER(CPointMass)]),
     "cdc_model_seeded_cavity": (c_int32, [POINTER(CMesh), POINTER(CHydro), POINTER(CCavityModel), c_void_p]),
 }

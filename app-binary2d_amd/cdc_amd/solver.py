@@ -139,6 +139,21 @@ class Solver:
         _check(self.lib, self.lib.cdc_solver_max_signal_speed(self.handle, byref(a)))
         return a.value
 
+    def set_tile(self, tx=0, ty=0):
+        _check(self.lib, self.lib.cdc_solver_set_tile(self.handle, int(tx), int(ty)))
+
+    def get_tile(self):
+        tx, ty = ctypes.c_int32(), ctypes.c_int32()
+        _check(self.lib, self.lib.cdc_solver_get_tile(self.handle, byref(tx), byref(ty)))
+        return tx.value, ty.value
+
+    def autotune(self, reps=3):
+        """Time every supported tile shape on the current state; keep the fastest.
+        Returns (tx, ty, ms per stage launch)."""
+        tx, ty, ms = ctypes.c_int32(), ctypes.c_int32(), c_double()
+        _check(self.lib, self.lib.cdc_solver_autotune(self.handle, int(reps), byref(tx), byref(ty), byref(ms)))
+        return tx.value, ty.value, ms.value
+
     def set_timing(self, enable=True):
         _check(self.lib, self.lib.cdc_solver_set_timing(self.handle, int(enable)))
 

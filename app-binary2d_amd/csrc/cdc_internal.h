@@ -77,6 +77,7 @@ struct TileShape {
     int tx, ty, nt;
 };
 TileShape tile_shape_for(int bs, int nv);
+int tile_shapes(const TileShape **out);
 
 int nsum_for(int nv);
 

@@ -385,8 +385,11 @@ __device__ __forceinline__ void cell_sources(const StageArgs &A, const double *u
 
 // ---------------------------------------------------------------------------
 
+// One-cell-per-thread tiles fit four 256-thread workgroups per CU in LDS;
+// ask the register allocator for the matching four waves per SIMD.
 template <int NV, int TX, int TY, int NT>
-__global__ void __launch_bounds__(NT) stage_kernel(const StageArgs A) {
+__global__ void __launch_bounds__(NT)
+__attribute__((amdgpu_waves_per_eu((TX * TY == NT) ? 4 : 2))) stage_kernel(const StageArgs A) {
     constexpr int EX = TX + 4, EY = TY + 4, EXY = EX * EY;
     constexpr int RX = TX + 2, RY = TY + 2, RXY = RX * RY;
     constexpr int NFX = (TX + 1) * TY, NFY = TX * (TY + 1), NF = NFX + NFY;
@@ -396,7 +399,10 @@ __global__ void __launch_bounds__(NT) stage_kernel(const StageArgs A) {
     constexpr int CH = 16, CL = NT / CH;
     constexpr int NC = NV == 4 ? EXY : 0;       // cell c_s^2 plane (Euler alpha viscosity)
     constexpr int LDS_A = NV * EXY + NC + 2 * NV * RXY;
-    constexpr int LDS_R = NR * (NT + CH);
+    constexpr int CHUNK_FIT = (LDS_A - NR * CH) / NT;
+    constexpr int CHUNK = CHUNK_FIT >= NR ? NR : (CHUNK_FIT < 1 ? 1 : CHUNK_FIT);
+    constexpr int NPASS = (NR + CHUNK - 1) / CHUNK;
+    constexpr int LDS_R = NR * CH + CHUNK * NT;
     constexpr int LDS = LDS_A > LDS_R ? LDS_A : LDS_R;
     static_assert(NV * NF <= 2 * NV * RXY, "flux staging must fit the gradient region");
     static_assert((TX * TY) % NT == 0 && NT % CH == 0, "tile / thread mismatch");
@@ -563,25 +569,35 @@ __global__ void __launch_bounds__(NT) stage_kernel(const StageArgs A) {
     }
 
     // ---- deterministic tile reductions --------------------------------------
+    // Fixed-order two-level tree through LDS, in passes of CHUNK values so
+    // that the reduction fits in the tile's own LDS.  Level 1: partial c of
+    // value s sums entries c, c + CH, c + 2 CH, ... (consecutive lanes read
+    // consecutive words: no bank conflicts).
     if (key != INT_MAX) atomicMin(&s_key, key);
     __syncthreads();
-    double *red = lds;
-    double *red2 = lds + NR * NT;
+    double *red2 = lds;
+    double *red = lds + NR * CH;
 #pragma unroll
-    for (int s = 0; s < NR; ++s) red[s * NT + tid] = acc[s];
-    __syncthreads();
-    for (int w = tid; w < NR * CH; w += NT) {
-        const int s = w / CH, c = w - (w / CH) * CH;
-        const double *q = red + s * NT + c * CL;
-        double t = q[0];
-        if (s == NS) {
-            for (int j = 1; j < CL; ++j) t = fmax(t, q[j]);
-        } else {
-            for (int j = 1; j < CL; ++j) t += q[j];
+    for (int pass = 0; pass < NPASS; ++pass) {
+        const int s0 = pass * CHUNK;
+#pragma unroll
+        for (int s = 0; s < NR; ++s)
+            if (s >= s0 && s < s0 + CHUNK) red[(s - s0) * NT + tid] = acc[s];
+        __syncthreads();
+        const int nv_pass = (NR - s0) < CHUNK ? (NR - s0) : CHUNK;
+        for (int w = tid; w < nv_pass * CH; w += NT) {
+            const int sl = w / CH, c = w - (w / CH) * CH;
+            const double *q = red + sl * NT + c;
+            double t = q[0];
+            if (s0 + sl == NS) {
+                for (int j = 1; j < CL; ++j) t = fmax(t, q[j * CH]);
+            } else {
+                for (int j = 1; j < CL; ++j) t += q[j * CH];
+            }
+            red2[(s0 + sl) * CH + c] = t;
         }
-        red2[s * CH + c] = t;
+        __syncthreads();
     }
-    __syncthreads();
     for (int s = tid; s < NR; s += NT) {
         const double *q = red2 + s * CH;
         double t = q[0];
@@ -749,10 +765,19 @@ __global__ void __launch_bounds__(256) column_max_kernel(const double *x, int n,
 // ---------------------------------------------------------------------------
 // Host launchers
 
+// Supported tile shapes (TX x TY cells, NT threads), largest first
+static const TileShape kShapes[] = {{16, 32, 256}, {32, 16, 256}, {16, 16, 256},
+                                    {8, 32, 256},  {32, 8, 256},  {8, 8, 64}};
+
+int tile_shapes(const TileShape **out) {
+    *out = kShapes;
+    return (int)(sizeof(kShapes) / sizeof(kShapes[0]));
+}
+
 TileShape tile_shape_for(int bs, int nv) {
     (void)nv;
-    if (bs % 32 == 0) return {16, 32, 256};
-    if (bs % 16 == 0) return {16, 16, 256};
+    for (const TileShape &t : kShapes)
+        if (bs % t.tx == 0 && bs % t.ty == 0) return t;
     return {8, 8, 64};
 }
 
@@ -762,15 +787,19 @@ static hipError_t launch_stage_t(const StageArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <int NV>
+static hipError_t launch_stage_nv(const TileShape &ts, const StageArgs &a, hipStream_t s) {
+    if (ts.tx == 16 && ts.ty == 32) return launch_stage_t<NV, 16, 32, 256>(a, s);
+    if (ts.tx == 32 && ts.ty == 16) return launch_stage_t<NV, 32, 16, 256>(a, s);
+    if (ts.tx == 16 && ts.ty == 16) return launch_stage_t<NV, 16, 16, 256>(a, s);
+    if (ts.tx == 8 && ts.ty == 32) return launch_stage_t<NV, 8, 32, 256>(a, s);
+    if (ts.tx == 32 && ts.ty == 8) return launch_stage_t<NV, 32, 8, 256>(a, s);
+    if (ts.tx == 8 && ts.ty == 8) return launch_stage_t<NV, 8, 8, 64>(a, s);
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_stage(int nv, const TileShape &ts, const StageArgs &a, hipStream_t s) {
-    if (nv == 3) {
-        if (ts.tx == 16 && ts.ty == 32) return launch_stage_t<3, 16, 32, 256>(a, s);
-        if (ts.tx == 16 && ts.ty == 16) return launch_stage_t<3, 16, 16, 256>(a, s);
-        return launch_stage_t<3, 8, 8, 64>(a, s);
-    }
-    if (ts.tx == 16 && ts.ty == 32) return launch_stage_t<4, 16, 32, 256>(a, s);
-    if (ts.tx == 16 && ts.ty == 16) return launch_stage_t<4, 16, 16, 256>(a, s);
-    return launch_stage_t<4, 8, 8, 64>(a, s);
+    return nv == 3 ? launch_stage_nv<3>(ts, a, s) : launch_stage_nv<4>(ts, a, s);
 }
 
 hipError_t launch_finalize(int nv, const StageArgs &a, int tpb, hipStream_t s) {

@@ -77,7 +77,7 @@ struct cdc_solver {
     cdc_mesh_t mesh;
     cdc_hydro_t hydro;
     cdc_physics_t phys;
-    int nv, nb, bs, N, nblk, ntiles, tpb, nsum;
+    int nv, nb, bs, N, nblk, ntiles, tpb, nsum, max_tiles;
     cdc::TileShape ts;
     hipStream_t stream = nullptr;
     int device = 0;
@@ -336,6 +336,14 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
     s->ts = cdc::tile_shape_for(s->bs, s->nv);
     s->tpb = (s->bs / s->ts.tx) * (s->bs / s->ts.ty);
     s->ntiles = s->nblk * s->tpb;
+    s->max_tiles = s->ntiles;
+    {
+        const cdc::TileShape *shapes;
+        const int n = cdc::tile_shapes(&shapes);
+        for (int i = 0; i < n; ++i)
+            if (s->bs % shapes[i].tx == 0 && s->bs % shapes[i].ty == 0)
+                s->max_tiles = std::max(s->max_tiles, s->nblk * (s->bs / shapes[i].tx) * (s->bs / shapes[i].ty));
+    }
     s->nsum = cdc::nsum_for(s->nv);
     (void)hipGetDevice(&s->device);
 
@@ -362,8 +370,8 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
         ALLOC(s->d_ist[i], (size_t)s->nblk * 7 * s->nv * sizeof(double));
         ALLOC(s->d_oec[i], (size_t)s->nblk * 7 * 4 * sizeof(double));
     }
-    ALLOC(s->d_tile_sums, (size_t)s->ntiles * (s->nsum + 1) * sizeof(double));
-    ALLOC(s->d_tile_err, (size_t)s->ntiles * sizeof(cdc::TileErr));
+    ALLOC(s->d_tile_sums, (size_t)s->max_tiles * (s->nsum + 1) * sizeof(double));
+    ALLOC(s->d_tile_err, (size_t)s->max_tiles * sizeof(cdc::TileErr));
     ALLOC(s->d_perr, (size_t)s->nblk * sizeof(int32_t));
     ALLOC(s->d_err_flag, sizeof(int32_t));
     s->naux = (hydro->kind == CDC_EULER && hydro->cooling_model == CDC_COOLING_BETA) ? 2 : 1;
@@ -466,6 +474,98 @@ static int compute_amax(cdc_solver *s, double *out) {
     double a = 0.0;
     for (double v : h) a = std::max(v, a);
     *out = a;
+    return CDC_OK;
+}
+
+int32_t cdc_solver_set_tile(cdc_solver_t *s, int32_t tx, int32_t ty) {
+    if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
+    cdc::TileShape t = cdc::tile_shape_for(s->bs, s->nv);
+    if (tx != 0 || ty != 0) {
+        const cdc::TileShape *shapes;
+        const int n = cdc::tile_shapes(&shapes);
+        bool found = false;
+        for (int i = 0; i < n; ++i)
+            if (shapes[i].tx == tx && shapes[i].ty == ty) { t = shapes[i]; found = true; }
+        if (!found) return set_msg(CDC_ERR_INVALID_ARGUMENT, "unsupported tile shape");
+        if (s->bs % tx != 0 || s->bs % ty != 0)
+            return set_msg(CDC_ERR_INVALID_ARGUMENT, "tile shape must divide block_size");
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->ts = t;
+    s->tpb = (s->bs / t.tx) * (s->bs / t.ty);
+    s->ntiles = s->nblk * s->tpb;
+    s->amax_valid = false;   // the per-tile max-speed column was written with the old tiling
+    return CDC_OK;
+}
+
+int32_t cdc_solver_get_tile(cdc_solver_t *s, int32_t *tx, int32_t *ty) {
+    if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
+    if (tx) *tx = s->ts.tx;
+    if (ty) *ty = s->ts.ty;
+    return CDC_OK;
+}
+
+int32_t cdc_solver_autotune(cdc_solver_t *s, int32_t reps, int32_t *tx, int32_t *ty, double *best_ms) {
+    if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
+    if (reps < 1) reps = 1;
+    const cdc::TileShape *shapes;
+    const int n = cdc::tile_shapes(&shapes);
+    const double L = 2.0 * s->mesh.domain_radius / (double)s->nb;
+    double el[4];
+    cdc::orbital_elements(s->phys, el);
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const int scratch = (s->cur + 1) % NBUF;
+    double best = 1e300;
+    cdc::TileShape pick = s->ts;
+    for (int i = 0; i < n; ++i) {
+        const cdc::TileShape t = shapes[i];
+        if (s->bs % t.tx != 0 || s->bs % t.ty != 0) continue;
+        cdc::StageArgs a;
+        memset(&a, 0, sizeof(a));
+        a.uc = s->d_u[s->cur];
+        a.u0 = s->d_init;
+        a.ub = s->d_u[s->cur];
+        a.aux = s->d_aux;
+        a.uo = s->d_u[scratch];
+        a.tile_sums = s->d_tile_sums;
+        a.tile_err = s->d_tile_err;
+        a.err_flag = s->d_err_flag;
+        a.N = s->N;
+        a.nb = s->nb;
+        a.bs = s->bs;
+        a.ntiles = s->nblk * (s->bs / t.tx) * (s->bs / t.ty);
+        a.stage_id = 1;
+        a.read_base = 1;        // time the costlier later-stage form
+        a.reuse_prim = s->hydro.has_density_floor ? 0 : 1;
+        a.R = s->mesh.domain_radius;
+        a.L = L;
+        a.time = s->time;
+        a.dt = 1e-9;
+        a.c1 = 0.5;
+        a.c0 = 0.5;
+        a.orb = cdc::orbital_state(el, s->time);
+        a.k = make_consts(s, s->time, a.dt);
+        HIP_TRY(cdc::launch_set_int(s->d_err_flag, 0, s->stream));
+        HIP_TRY(cdc::launch_stage(s->nv, t, a, s->stream));   // warm-up
+        HIP_TRY(hipEventRecord(e0, s->stream));
+        for (int r = 0; r < reps; ++r) HIP_TRY(cdc::launch_stage(s->nv, t, a, s->stream));
+        HIP_TRY(hipEventRecord(e1, s->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        const double per = ms / reps;
+        if (per < best) { best = per; pick = t; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_TRY(cdc::launch_set_int(s->d_err_flag, 0, s->stream));
+    int rc = cdc_solver_set_tile(s, pick.tx, pick.ty);
+    if (rc) return rc;
+    if (tx) *tx = pick.tx;
+    if (ty) *ty = pick.ty;
+    if (best_ms) *best_ms = best;
     return CDC_OK;
 }
 

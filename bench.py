@@ -61,6 +61,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -80,6 +81,9 @@ def main():
         g.advance(min(fold, args.warmup - done))
         done += min(fold, args.warmup - done)
 
+    tile = (0, 0, 0.0)
+    if not args.no_autotune:
+        tile = g.autotune(reps=3)
     g.set_timing(True)
     t0 = time.perf_counter()
     for _ in range(steps // fold):
@@ -105,7 +109,8 @@ def main():
         "data": "synthetic (seeded cavity disk, seed %d)" % w["seed"],
         "config": {"workload": args.workload, "grid": f"{mesh.num_blocks * mesh.block_size}^2",
                    "blocks": f"{mesh.num_blocks}x{mesh.num_blocks} of {mesh.block_size}^2",
-                   "hydro": type(hydro).__name__.lower(), "rk": ph.rk_order, "fold": fold},
+                   "hydro": type(hydro).__name__.lower(), "rk": ph.rk_order, "fold": fold,
+                   "tile": "%dx%d" % g.get_tile()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "stage_kernel", "launches": launches,

@@ -210,6 +210,19 @@ int32_t cdc_solver_advance(cdc_solver_t *solver, int32_t fold, double *dt_out,
 /* State::max_signal_speed (src/state.rs:121-133) of the current state. */
 int32_t cdc_solver_max_signal_speed(cdc_solver_t *solver, double *out);
 
+/* Tile shape (tx rows along x by ty columns along y, inside one block) of
+ * the fused stage kernel.  Supported: 16x32, 32x16, 16x16, 8x32, 32x8, 8x8,
+ * each dividing block_size; (0, 0) restores the default.  No reference
+ * counterpart: the reference has no tiles. */
+int32_t cdc_solver_set_tile(cdc_solver_t *solver, int32_t tx, int32_t ty);
+int32_t cdc_solver_get_tile(cdc_solver_t *solver, int32_t *tx, int32_t *ty);
+
+/* Autotuner (no reference counterpart): times `reps` stage-kernel launches
+ * of every supported tile shape on the current state, writing only scratch
+ * buffers (the state is unchanged), and keeps the fastest shape. */
+int32_t cdc_solver_autotune(cdc_solver_t *solver, int32_t reps, int32_t *tx,
+                            int32_t *ty, double *best_ms);
+
 /* Per-launch timing of the fused stage kernel with HIP events on the
  * handle's stream.  enable=1 clears and starts accumulating. */
 int32_t cdc_solver_set_timing(cdc_solver_t *solver, int32_t enable);

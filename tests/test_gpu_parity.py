@@ -268,3 +268,39 @@ def test_timing_counts_stage_launches():
     g.advance(2)
     n, ms, cells = g.get_timing()
     assert n == 6 and ms > 0 and cells == 64 * 64
+
+
+SHAPES = [(16, 32), (32, 16), (16, 16), (8, 32), (32, 8), (8, 8)]
+
+
+@pytest.mark.parametrize("kind", ["iso", "euler"])
+@pytest.mark.parametrize("shape", SHAPES, ids=[f"{a}x{b}" for a, b in SHAPES])
+def test_every_tile_shape_matches_oracle(kind, shape):
+    kw = dict(rk_order="RK3", viscosity={"alpha": 0.1}, binary_eccentricity=0.3, binary_mass_ratio=0.7)
+    mesh, hydro, ph = H.small(kind, nb=2, bs=32, R=4.0, **kw)
+    u0 = H.initial(mesh, hydro, ph)
+    g, o = pair(mesh, hydro, ph, u0)
+    g.set_tile(*shape)
+    assert g.get_tile() == shape
+    for _ in range(3):
+        assert g.advance(1) == pytest.approx(o.advance(1), rel=1e-12)
+        check_states(g, o)
+
+
+def test_autotune_keeps_state_and_picks_a_valid_shape():
+    mesh, hydro, ph = H.small("euler", nb=2, bs=64, R=4.0)
+    u0 = H.initial(mesh, hydro, ph)
+    g, o = pair(mesh, hydro, ph, u0)
+    g.advance(1); o.advance(1)
+    before = g.get_state()
+    tx, ty, ms = g.autotune(reps=2)
+    assert (tx, ty) in SHAPES and ms > 0 and g.get_tile() == (tx, ty)
+    after = g.get_state()
+    np.testing.assert_array_equal(before["conserved"], after["conserved"])
+    np.testing.assert_array_equal(before["ist"], after["ist"])
+    assert before["time"] == after["time"]
+    for _ in range(2):
+        assert g.advance(1) == pytest.approx(o.advance(1), rel=1e-12)
+        check_states(g, o)
+    with pytest.raises(Exception):
+        g.set_tile(64, 64)
