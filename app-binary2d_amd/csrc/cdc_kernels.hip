@@ -435,24 +435,53 @@ __attribute__((amdgpu_waves_per_eu((TX * TY == NT) ? 4 : 2))) stage_kernel(const
     int key = INT_MAX, kcode = 0;
     double kval = 0.0;
     if (tid == 0) s_key = INT_MAX;
-    for (int e = tid; e < EXY; e += NT) {
-        const int ei = e / EY, ej = e - (e / EY) * EY;
-        int gi = gi0 - 2 + ei, gj = gj0 - 2 + ej;
-        gi = gi < 0 ? gi + N : (gi >= N ? gi - N : gi);
-        gj = gj < 0 ? gj + N : (gj >= N ? gj - N : gj);
-        const size_t g = (size_t)gi * N + gj;
-        double u[NV], p[NV], val = 0.0, id;
+    // all of this thread's halo-tile loads first, then the update phase's
+    // operands, so that their latencies overlap each other and phases 1-3
+    constexpr int KE = (EXY + NT - 1) / NT;
+    double ue[KE][NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) u[v] = A.uc[v * NN + g];
-        const int code = dev::try_to_primitive<NV>(A.k, u, p, val, id);
-        if (code && ei >= 2 && ei < TX + 2 && ej >= 2 && ej < TY + 2) {
-            const int c = (li0 + ei - 2) * bs + (lj0 + ej - 2);
-            if (c < key) { key = c; kcode = code; kval = val; }
+    for (int q = 0; q < KE; ++q) {
+        const int e = tid + q * NT;
+        if (e < EXY) {
+            const int ei = e / EY, ej = e - (e / EY) * EY;
+            int gi = gi0 - 2 + ei, gj = gj0 - 2 + ej;
+            gi = gi < 0 ? gi + N : (gi >= N ? gi - N : gi);
+            gj = gj < 0 ? gj + N : (gj >= N ? gj - N : gj);
+            const size_t g = (size_t)gi * N + gj;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) ue[q][v] = A.uc[v * NN + g];
         }
+    }
+    double pu0[CPT][NV], pub[CPT][NV], pbr[CPT], pom[CPT];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) P[v * EXY + e] = p[v];
-        if constexpr (NV == 4) {
-            if (alpha) C[e] = A.k.gm * p[3] * id;
+    for (int q = 0; q < CPT; ++q) {
+        const int c = tid + q * NT;
+        const int ci = c / TY, cj = c - (c / TY) * TY;
+        const size_t g = (size_t)(gi0 + ci) * N + (gj0 + cj);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            pu0[q][v] = A.u0[v * NN + g];
+            pub[q][v] = A.read_base ? A.ub[v * NN + g] : 0.0;
+        }
+        pbr[q] = A.aux[g];
+        pom[q] = A.k.cooling_model == CDC_COOLING_BETA ? A.aux[NN + g] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < KE; ++q) {
+        const int e = tid + q * NT;
+        if (e < EXY) {
+            const int ei = e / EY, ej = e - (e / EY) * EY;
+            double p[NV], val = 0.0, id;
+            const int code = dev::try_to_primitive<NV>(A.k, ue[q], p, val, id);
+            if (code && ei >= 2 && ei < TX + 2 && ej >= 2 && ej < TY + 2) {
+                const int c = (li0 + ei - 2) * bs + (lj0 + ej - 2);
+                if (c < key) { key = c; kcode = code; kval = val; }
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) P[v * EXY + e] = p[v];
+            if constexpr (NV == 4) {
+                if (alpha) C[e] = A.k.gm * p[3] * id;
+            }
         }
     }
     __syncthreads();
@@ -522,13 +551,10 @@ __attribute__((amdgpu_waves_per_eu((TX * TY == NT) ? 4 : 2))) stage_kernel(const
         const int c = tid + q * NT;
         const int ci = c / TY, cj = c - (c / TY) * TY;
         const size_t g = (size_t)(gi0 + ci) * N + (gj0 + cj);
-        double uc[NV], u0[NV], ub[NV], du[NV];
+        double uc[NV], du[NV];
+        const double *u0 = pu0[q], *ub = pub[q];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            uc[v] = A.uc[v * NN + g];
-            u0[v] = A.u0[v * NN + g];
-            ub[v] = A.read_base ? A.ub[v * NN + g] : 0.0;
-        }
+        for (int v = 0; v < NV; ++v) uc[v] = A.uc[v * NN + g];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const double *fx = FX + v * NFX + ci * TY + cj;
@@ -548,8 +574,8 @@ __attribute__((amdgpu_waves_per_eu((TX * TY == NT) ? 4 : 2))) stage_kernel(const
             vy = uc[2] * id;
         }
         if constexpr (NV == 4) p0 = (uc[3] - 0.5 * uc[0] * (vx * vx + vy * vy)) * A.k.gm1;
-        const double brate = A.aux[g];
-        const double omega = A.k.cooling_model == CDC_COOLING_BETA ? A.aux[NN + g] : 0.0;
+        const double brate = pbr[q];
+        const double omega = pom[q];
         double it[7][NV];
         dev::cell_sources<NV>(A, uc, u0, axx.c(li0 + ci), axy.c(lj0 + cj), vx, vy, id, p0, brate, omega, it);
         double out[NV];
