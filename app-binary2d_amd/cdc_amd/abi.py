@@ -140,6 +140,12 @@ SIGNATURES = {
     "cdc_solver_set_tile": (c_int32, [c_void_p, c_int32, c_int32]),
     "cdc_solver_get_tile": (c_int32, [c_void_p, POINTER(c_int32), POINTER(c_int32)]),
     "cdc_solver_autotune": (c_int32, [c_void_p, c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_double)]),
+    "cdc_ensemble_create": (c_int32, [POINTER(c_void_p), c_int32, POINTER(c_void_p)]),
+    "cdc_ensemble_destroy": (None, [c_void_p]),
+    "cdc_ensemble_advance": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "cdc_ensemble_set_graph": (c_int32, [c_void_p, c_int32]),
+    "cdc_ensemble_get_tile": (c_int32, [c_void_p, POINTER(c_int32), POINTER(c_int32)]),
+    "cdc_ensemble_get_stats": (c_int32, [c_void_p, POINTER(c_int64), POINTER(c_double)]),
     "cdc_orbital_state_from_time": (c_int32, [POINTER(CPhysics), c_double, POINTER(CPointMass)]),
     "cdc_model_seeded_cavity": (c_int32, [POINTER(CMesh), POINTER(CHydro), POINTER(CCavityModel), c_void_p]),
 }

@@ -124,6 +124,12 @@ class Solver:
                                                       _ptr(u), _ptr(ist), _ptr(oec)))
         return {"time": t.value, "iteration": (n.value, d.value), "conserved": u, "ist": ist, "oec": oec}
 
+    def time(self):
+        """Committed simulation time (no state copy)."""
+        t = c_double()
+        _check(self.lib, self.lib.cdc_solver_get_state(self.handle, byref(t), None, None, None, None, None))
+        return t.value
+
     def advance(self, fold=1):
         """scheme::advance: returns dt; raises HydroError (state unchanged)."""
         dt = c_double()
@@ -161,3 +167,72 @@ class Solver:
         n, ms, cells = c_int64(), c_double(), c_double()
         _check(self.lib, self.lib.cdc_solver_get_timing(self.handle, byref(n), byref(ms), byref(cells)))
         return n.value, ms.value, cells.value
+
+
+class Ensemble:
+    """One RK step of many independent runs per call (cdc_ensemble_*).
+
+    `members` are Solver objects of one hydro kind and rk_order; each active
+    member advances one step at its own dt and ends bitwise where
+    `member.advance(1)` would have left it.  The members stay owned by the
+    caller and must outlive the ensemble.
+    """
+
+    def __init__(self, members, graph=True):
+        self.lib = abi.load()
+        self.members = list(members)
+        n = len(self.members)
+        arr = (c_void_p * n)(*[m.handle for m in self.members])
+        h = c_void_p()
+        _check(self.lib, self.lib.cdc_ensemble_create(arr, n, byref(h)))
+        self.handle = h
+        self.n = n
+        if not graph:
+            self.set_graph(False)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.cdc_ensemble_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_graph(self, enable=True):
+        _check(self.lib, self.lib.cdc_ensemble_set_graph(self.handle, int(enable)))
+
+    def get_tile(self):
+        tx, ty = ctypes.c_int32(), ctypes.c_int32()
+        _check(self.lib, self.lib.cdc_ensemble_get_tile(self.handle, byref(tx), byref(ty)))
+        return tx.value, ty.value
+
+    def stats(self):
+        n, z = c_int64(), c_double()
+        _check(self.lib, self.lib.cdc_ensemble_get_stats(self.handle, byref(n), byref(z)))
+        return n.value, z.value
+
+    def advance(self, active=None):
+        """One step of every active member.  Returns (dt, codes, errors): per
+        member dt (nan when inactive), error code (0 = committed) and the
+        HydroError of each failed member (None otherwise)."""
+        n = self.n
+        act = None if active is None else np.ascontiguousarray(active, dtype=np.int32).reshape(n)
+        dt = np.full(n, np.nan)
+        codes = np.zeros(n, dtype=np.int32)
+        errs = (abi.CError * n)()
+        rc = self.lib.cdc_ensemble_advance(self.handle, None if act is None else _ptr(act), _ptr(dt), _ptr(codes),
+                                           ctypes.cast(errs, c_void_p))
+        hydro = (abi.CDC_ERR_NEGATIVE_DENSITY, abi.CDC_ERR_NEGATIVE_PRESSURE, abi.CDC_ERR_UNBOUND_ORBIT)
+        if rc != abi.CDC_OK and rc not in hydro:
+            _check(self.lib, rc)
+        out = [HydroError(errs[i]) if codes[i] in hydro else None for i in range(n)]
+        return dt, codes, out

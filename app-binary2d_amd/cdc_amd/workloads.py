@@ -5,6 +5,8 @@
   euler2048 gamma-law, 32x32 blocks of 64^2 (2048^2), RK3, alpha viscosity,
             eccentric (e = 0.5) unequal (q = 0.5) binary, beta cooling, pressure
             floor 1e-10 (as the reference's cooling-cbd setup floors pressure), 100 steps
+ensemble64 (ensemble_members) 64 isothermal RK2 runs of 256^2 to 1024^2 with
+            drawn binary and disk parameters, half an orbit each
 Domain [-8a, 8a]^2, a = M = G = 1, softening = sink radius = 0.05, sink rate
 10, Mach 10, PLM theta 1.5, CFL 0.4, buffer (rate 1000, scale 0.3) relaxing to
 the initial state near the domain edge; initial state: the seeded cavity
@@ -45,6 +47,28 @@ WORKLOADS = {
                                            viscosity={"alpha": 0.1}),
                       seed=3, steps=100, orbits=None),
 }
+
+
+def ensemble_members(seed=4, counts=((24, 4), (24, 8), (16, 16)), block_size=64):
+    """The 64-run parameter study: 24 runs at 256^2, 24 at 512^2, 16 at 1024^2
+    (blocks of 64^2), isothermal RK2, each with binary eccentricity
+    e ~ U[0, 0.8], mass ratio q ~ log-U[0.1, 1], Mach number ~ U[10, 40] and
+    kinematic viscosity nu ~ log-U[1e-4, 1e-2] drawn from one seeded
+    generator; every run has its own initial-state seed."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for count, nb in counts:
+        for _ in range(count):
+            e = float(rng.uniform(0.0, 0.8))
+            q = float(10.0 ** rng.uniform(-1.0, 0.0))
+            mach = float(rng.uniform(10.0, 40.0))
+            nu = float(10.0 ** rng.uniform(-4.0, -2.0))
+            out.append(dict(mesh=Mesh(nb, block_size, 8.0), hydro=Isothermal(mach_number=mach),
+                            physics=base_physics(binary_eccentricity=e, binary_mass_ratio=q,
+                                                 viscosity={"constant_nu": nu}),
+                            seed=1000 * seed + len(out), steps=None, orbits=0.5))
+    return out
 
 
 def get(name, **over):

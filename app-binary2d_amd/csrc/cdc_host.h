@@ -1,0 +1,116 @@
+// cdc_host.h -- host-side solver object and the pieces of one advance that the
+// single-run path (cdc_solver.cpp) and the batched path (cdc_batch.cpp)
+// share.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/cdc_solver.h"
+#include "cdc_internal.h"
+
+namespace cdc {
+namespace host {
+
+// Records the message returned by cdc_last_error_message and returns code.
+int set_msg(int code, const std::string &m);
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return ::cdc::host::set_msg(CDC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int NBUF = 4;
+
+// The state one RK stage reads or writes: buffer index plus the time and
+// iteration carried with it.
+struct Slot {
+    int idx;
+    double time;
+    int64_t num, den;
+};
+
+// The stages of `fold` RK steps at one dt: launch arguments in launch order,
+// the input time of every stage (the orbital state an error reports), and
+// the slot that holds the result.
+struct StepPlan {
+    std::vector<StageArgs> args;
+    std::vector<double> stage_times;
+    Slot out;
+};
+
+}  // namespace host
+}  // namespace cdc
+
+struct cdc_solver {
+    cdc_mesh_t mesh;
+    cdc_hydro_t hydro;
+    cdc_physics_t phys;
+    int nv, nb, bs, N, nblk, ntiles, tpb, nsum, max_tiles;
+    cdc::TileShape ts;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    double *d_init = nullptr;
+    double *d_u[cdc::host::NBUF] = {};
+    double *d_ist[cdc::host::NBUF] = {};
+    double *d_oec[cdc::host::NBUF] = {};
+    double *d_tile_sums = nullptr;
+    cdc::TileErr *d_tile_err = nullptr;
+    int32_t *d_perr = nullptr;
+    int32_t *d_err_flag = nullptr;
+    double *d_aux = nullptr;     // static per-cell planes (buffer rate, beta-cooling Omega)
+    int naux = 1;
+    double *d_stage = nullptr;   // AoS staging
+    double *d_amax = nullptr;    // max-speed partials
+    int namax = 0;
+
+    int cur = 0;                 // committed buffer
+    double time = 0.0;
+    int64_t num = 0, den = 1;
+    bool amax_valid = false;     // tile_sums[:, nsum] hold the committed state's max speed
+    uint64_t version = 0;        // bumped whenever the committed state changes
+
+    // launch arguments of the stages of one advance: pinned staging and its
+    // device copy, and the batch-of-one offsets {0, ntiles, 0, nblk}
+    cdc::StageArgs *h_args = nullptr, *d_args = nullptr;
+    size_t cap_args = 0;
+    int32_t *d_begin = nullptr;
+
+    bool timing = false;
+    int64_t launches = 0;
+    double total_ms = 0.0;
+    std::vector<hipEvent_t> ev;
+
+    size_t ncell() const { return (size_t)N * N; }
+    size_t nstate() const { return ncell() * nv; }
+};
+
+namespace cdc {
+namespace host {
+
+// Stage (and, when finalize, per-block finalize) launches of args[0..n) for
+// this one run (a batch of one), after one upload of the arguments; events
+// ev[2i], ev[2i+1] bracket stage launch i when ev is given.
+int launch_stages(cdc_solver *s, const StageArgs *args, size_t n, const cdc::TileShape &ts, hipEvent_t *ev,
+                  bool finalize);
+// Clears the run's error flag.
+int launch_set_int_checked(cdc_solver *s);
+// Global max signal speed of the committed state (State::max_signal_speed).
+int compute_amax(cdc_solver *s, double *out);
+// dt = cfl * dx / amax, optionally raised to dt_min (src/scheme.rs:336-340).
+double step_dt(const cdc_solver *s, double amax);
+// Launch arguments of `fold` RK steps at dt from the committed state.
+void plan_steps(const cdc_solver *s, int fold, double dt, StepPlan &plan);
+// Makes the plan's result the committed state.
+void commit(cdc_solver *s, const Slot &out);
+// After a failed advance (err_flag = flag): the error join_all would report.
+int resolve_error(cdc_solver *s, int flag, const StepPlan &plan, cdc_error_t *err);
+
+}  // namespace host
+}  // namespace cdc

@@ -63,7 +63,7 @@ struct StageArgs {
     const double *oec_c, *oec_b;   // [nblk][7][4]
     double *ist_o, *oec_o;
     int32_t *perr;                 // [nblk] perturbation error of the block
-    int32_t N, nb, bs, ntiles;
+    int32_t N, nb, bs, ntiles, tpb;  // tpb: tiles per block
     int32_t stage_id, read_base, want_amax, reuse_prim;
     double R, L;           // domain radius, block length
     double time, dt, c1, c0, area;
@@ -81,9 +81,18 @@ int tile_shapes(const TileShape **out);
 
 int nsum_for(int nv);
 
-hipError_t launch_stage(int nv, const TileShape &ts, const StageArgs &a, hipStream_t s);
-hipError_t launch_finalize(int nv, const StageArgs &a, int tpb, hipStream_t s);
 hipError_t launch_aux(const AuxArgs &a, hipStream_t s);
+
+// One RK stage (fused stage kernel, then the per-block finalize) over a batch
+// of runs: device arrays of StageArgs, one per run; tile_begin / block_begin
+// hold nruns + 1 prefix offsets.  A single run is a batch of one.
+hipError_t launch_stage_batch(int nv, const TileShape &ts, const StageArgs *args, const int32_t *tile_begin,
+                              int nruns, int ntiles, hipStream_t s);
+hipError_t launch_finalize_batch(int nv, const StageArgs *args, const int32_t *block_begin, int nruns,
+                                 int nblocks, hipStream_t s);
+hipError_t launch_batch_reset(const StageArgs *args, int nruns, hipStream_t s);
+hipError_t launch_batch_tail(int nv, const StageArgs *args, int nruns, double *amax, int32_t *flags,
+                             hipStream_t s);
 hipError_t launch_max_speed(int nv, const double *u, int64_t ncell, double gm, double *partial,
                             int nparts, hipStream_t s);
 hipError_t launch_aos_to_soa(int nv, int nb, int bs, const double *aos, double *soa, hipStream_t s);

@@ -1,7 +1,7 @@
 // cdc_kernels.hip -- gfx950 kernels of the per-block Runge-Kutta update.
 //
-// One fused FP64 kernel per RK stage (stage_kernel) does, per tile of
-// TX x TY cells inside one reference block:
+// One fused FP64 kernel per RK stage (stage_kernel_batch, over one run or a
+// batch of runs) does, per tile of TX x TY cells inside one reference block:
 //   phase 1  conserved -> primitive on the tile plus its 2-zone halo, halo
 //            cells read straight from the neighbouring blocks' HBM with
 //            periodic wrap (guard extension, src/scheme.rs:126-127), into LDS;
@@ -15,7 +15,7 @@
 //            cooling), the low-storage RK combine with the step base, and the
 //            per-tile sums of the itemized source terms
 //            (src/scheme.rs:273-316, src/state.rs:183-200)
-// finalize_kernel then sums each block's tiles in a fixed order, scales by
+// finalize_kernel_batch then sums each block's tiles in a fixed order, scales by
 // dx*dy, applies the orbital-element perturbation and the RK combine of the
 // block's integrated_source_terms / orbital_elements_change
 // (src/scheme.rs:307-315, src/state.rs:195-197).
@@ -125,7 +125,7 @@ __device__ __forceinline__ Axis block_axis(double R, double L, int bi, int bs) {
 __device__ __forceinline__ double minmod3(double a, double b, double c) {
     const double mn = fmin(a, fmin(b, c));
     const double mx = fmax(a, fmax(b, c));
-    return mn > 0.0 ? mn : (mx < 0.0 ? mx : 0.0);
+    return fmax(mn, 0.0) + fmin(mx, 0.0);
 }
 
 // godunov-core plm_gradient: minmod(theta (b - a), (c - a) / 2, theta (c - b))
@@ -385,11 +385,17 @@ __device__ __forceinline__ void cell_sources(const StageArgs &A, const double *u
 
 // ---------------------------------------------------------------------------
 
-// One-cell-per-thread tiles fit four 256-thread workgroups per CU in LDS;
-// ask the register allocator for the matching four waves per SIMD.
+// Tile index of workgroup wg: consecutive tiles go to one XCD (workgroups
+// are dealt round-robin over the 8 XCDs), so halo re-reads of neighbouring
+// tiles hit the same L2.
+__device__ __forceinline__ int xcd_tile(int wg, int ntiles) {
+    return (ntiles % 8 == 0) ? (wg % 8) * (ntiles / 8) + wg / 8 : wg;
+}
+
+// One RK stage on one TX x TY tile of one block (the P, F and U phases of
+// try_advance_rk, src/scheme.rs:88-173, restricted to the tile).
 template <int NV, int TX, int TY, int NT>
-__global__ void __launch_bounds__(NT)
-__attribute__((amdgpu_waves_per_eu((TX * TY == NT) ? 4 : 2))) stage_kernel(const StageArgs A) {
+__device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
     constexpr int EX = TX + 4, EY = TY + 4, EXY = EX * EY;
     constexpr int RX = TX + 2, RY = TY + 2, RXY = RX * RY;
     constexpr int NFX = (TX + 1) * TY, NFY = TX * (TY + 1), NF = NFX + NFY;
@@ -409,12 +415,7 @@ __attribute__((amdgpu_waves_per_eu((TX * TY == NT) ? 4 : 2))) stage_kernel(const
     __shared__ double lds[LDS];
     __shared__ int s_key;
 
-    const int ef = *(volatile const int32_t *)A.err_flag;
-    if (ef != 0 && ef < A.stage_id) return;
-
     const int tid = threadIdx.x;
-    const int wg = blockIdx.x;
-    const int tile = (A.ntiles % 8 == 0) ? (wg % 8) * (A.ntiles / 8) + wg / 8 : wg;
     const int bs = A.bs, N = A.N;
     const int tpby = bs / TY, tpb = (bs / TX) * tpby;
     const int b = tile / tpb, lt = tile - b * tpb;
@@ -647,21 +648,53 @@ __attribute__((amdgpu_waves_per_eu((TX * TY == NT) ? 4 : 2))) stage_kernel(const
     }
 }
 
+// Run index of global tile (or block) t: the last r with begin[r] <= t.
+__device__ __forceinline__ int run_of(const int32_t *__restrict__ begin, int nruns, int t) {
+    int lo = 0, hi = nruns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begin[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// A stage that already failed (err_flag holds an earlier stage id) skips the
+// remaining stages of the call.
+__device__ __forceinline__ bool stage_skipped(const StageArgs &A) {
+    const int ef = *(volatile const int32_t *)A.err_flag;
+    return ef != 0 && ef < A.stage_id;
+}
+
+// One RK stage of a batch of runs in one launch (a single run is a batch of
+// one, so every run executes the same machine code): tiles of run r are
+// global tiles begin[r] .. begin[r + 1] - 1, and each run reads its own
+// StageArgs.  One-cell-per-thread tiles fit four 256-thread workgroups per
+// CU in LDS; ask the register allocator for the matching four waves per SIMD.
+template <int NV, int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT)
+__attribute__((amdgpu_waves_per_eu((TX * TY == NT) ? 4 : 2)))
+stage_kernel_batch(const StageArgs *__restrict__ args, const int32_t *__restrict__ begin, int nruns) {
+    const int t = xcd_tile(blockIdx.x, begin[nruns]);
+    const int r = run_of(begin, nruns, t);
+    const StageArgs &A = args[r];
+    if (stage_skipped(A)) return;
+    stage_tile<NV, TX, TY, NT>(A, t - begin[r]);
+}
+
 // The per-block tail of a stage, one wave per block: the block's tile sums
 // in tile order, scaled by dx*dy, the four orbital-element perturbations
 // (ItemizedChange::perturbation, src/state.rs:258-282) one per lane, and the
 // RK combine of integrated_source_terms / orbital_elements_change
 // (src/scheme.rs:307-315, src/state.rs:195-197).
 template <int NV>
-__global__ void __launch_bounds__(64) finalize_kernel(const StageArgs A, int tpb) {
+__device__ __forceinline__ void finalize_block(const StageArgs &A, const int b, const int tpb) {
     constexpr int NS = SumMap<NV>::n, NR = NS + 1;
     static_assert(NS + 7 * NV <= 64, "one wave per block");
     __shared__ double s_ds[7 * NV];
     __shared__ double s_de[7 * 4];
     __shared__ int s_perr;
-    const int ef = *(volatile const int32_t *)A.err_flag;
-    if (ef != 0 && ef < A.stage_id) return;
-    const int b = blockIdx.x, tid = threadIdx.x;
+    const int tid = threadIdx.x;
     if (tid < NS) {
         double t = 0.0;
         for (int j = 0; j < tpb; ++j) t += A.tile_sums[((size_t)b * tpb + j) * NR + tid];
@@ -704,6 +737,42 @@ __global__ void __launch_bounds__(64) finalize_kernel(const StageArgs A, int tpb
         }
         const double s1 = A.oec_c[oe + tid] + s_de[tid];
         A.oec_o[oe + tid] = A.read_base ? s1 * A.c1 + A.oec_b[oe + tid] * A.c0 : s1;
+    }
+}
+
+// Blocks of run r are global blocks begin[r] .. begin[r+1]-1.
+template <int NV>
+__global__ void __launch_bounds__(64)
+finalize_kernel_batch(const StageArgs *__restrict__ args, const int32_t *__restrict__ begin, int nruns) {
+    const int r = run_of(begin, nruns, blockIdx.x);
+    const StageArgs &A = args[r];
+    if (stage_skipped(A)) return;
+    finalize_block<NV>(A, blockIdx.x - begin[r], A.tpb);
+}
+
+// Batch prologue: clear every run's error flag (a kernel, not a memset, so
+// that the sequence can be captured in a graph and replayed).
+__global__ void __launch_bounds__(64) batch_reset_kernel(const StageArgs *__restrict__ args, int nruns) {
+    for (int r = threadIdx.x; r < nruns; r += 64) *args[r].err_flag = 0;
+}
+
+// Batch epilogue, one workgroup per run: the run's max signal speed from the
+// last stage's per-tile column, and its error flag.
+__global__ void __launch_bounds__(256) batch_tail_kernel(const StageArgs *__restrict__ args, int nr,
+                                                         double *amax, int32_t *flags) {
+    __shared__ double red[256];
+    const StageArgs &A = args[blockIdx.x];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < A.ntiles; i += 256) m = fmax(m, A.tile_sums[(size_t)i * nr + nr - 1]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        amax[blockIdx.x] = red[0];
+        flags[blockIdx.x] = *(volatile const int32_t *)A.err_flag;
     }
 }
 
@@ -808,30 +877,47 @@ TileShape tile_shape_for(int bs, int nv) {
 }
 
 template <int NV, int TX, int TY, int NT>
-static hipError_t launch_stage_t(const StageArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL((stage_kernel<NV, TX, TY, NT>), dim3(a.ntiles), dim3(NT), 0, s, a);
+static hipError_t launch_batch_t(const StageArgs *args, const int32_t *begin, int nruns, int ntiles,
+                                 hipStream_t s) {
+    hipLaunchKernelGGL((stage_kernel_batch<NV, TX, TY, NT>), dim3(ntiles), dim3(NT), 0, s, args, begin, nruns);
     return hipGetLastError();
 }
 
 template <int NV>
-static hipError_t launch_stage_nv(const TileShape &ts, const StageArgs &a, hipStream_t s) {
-    if (ts.tx == 16 && ts.ty == 32) return launch_stage_t<NV, 16, 32, 256>(a, s);
-    if (ts.tx == 32 && ts.ty == 16) return launch_stage_t<NV, 32, 16, 256>(a, s);
-    if (ts.tx == 16 && ts.ty == 16) return launch_stage_t<NV, 16, 16, 256>(a, s);
-    if (ts.tx == 8 && ts.ty == 32) return launch_stage_t<NV, 8, 32, 256>(a, s);
-    if (ts.tx == 32 && ts.ty == 8) return launch_stage_t<NV, 32, 8, 256>(a, s);
-    if (ts.tx == 8 && ts.ty == 8) return launch_stage_t<NV, 8, 8, 64>(a, s);
+static hipError_t launch_batch_nv(const TileShape &ts, const StageArgs *args, const int32_t *begin, int nruns,
+                                  int ntiles, hipStream_t s) {
+    if (ts.tx == 16 && ts.ty == 32) return launch_batch_t<NV, 16, 32, 256>(args, begin, nruns, ntiles, s);
+    if (ts.tx == 32 && ts.ty == 16) return launch_batch_t<NV, 32, 16, 256>(args, begin, nruns, ntiles, s);
+    if (ts.tx == 16 && ts.ty == 16) return launch_batch_t<NV, 16, 16, 256>(args, begin, nruns, ntiles, s);
+    if (ts.tx == 8 && ts.ty == 32) return launch_batch_t<NV, 8, 32, 256>(args, begin, nruns, ntiles, s);
+    if (ts.tx == 32 && ts.ty == 8) return launch_batch_t<NV, 32, 8, 256>(args, begin, nruns, ntiles, s);
+    if (ts.tx == 8 && ts.ty == 8) return launch_batch_t<NV, 8, 8, 64>(args, begin, nruns, ntiles, s);
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_stage(int nv, const TileShape &ts, const StageArgs &a, hipStream_t s) {
-    return nv == 3 ? launch_stage_nv<3>(ts, a, s) : launch_stage_nv<4>(ts, a, s);
+hipError_t launch_stage_batch(int nv, const TileShape &ts, const StageArgs *args, const int32_t *tile_begin,
+                              int nruns, int ntiles, hipStream_t s) {
+    return nv == 3 ? launch_batch_nv<3>(ts, args, tile_begin, nruns, ntiles, s)
+                   : launch_batch_nv<4>(ts, args, tile_begin, nruns, ntiles, s);
 }
 
-hipError_t launch_finalize(int nv, const StageArgs &a, int tpb, hipStream_t s) {
-    const int nblk = a.nb * a.nb;
-    if (nv == 3) hipLaunchKernelGGL(finalize_kernel<3>, dim3(nblk), dim3(64), 0, s, a, tpb);
-    else hipLaunchKernelGGL(finalize_kernel<4>, dim3(nblk), dim3(64), 0, s, a, tpb);
+hipError_t launch_finalize_batch(int nv, const StageArgs *args, const int32_t *block_begin, int nruns,
+                                 int nblocks, hipStream_t s) {
+    if (nv == 3)
+        hipLaunchKernelGGL(finalize_kernel_batch<3>, dim3(nblocks), dim3(64), 0, s, args, block_begin, nruns);
+    else
+        hipLaunchKernelGGL(finalize_kernel_batch<4>, dim3(nblocks), dim3(64), 0, s, args, block_begin, nruns);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_reset(const StageArgs *args, int nruns, hipStream_t s) {
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, args, nruns);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_tail(int nv, const StageArgs *args, int nruns, double *amax, int32_t *flags,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(batch_tail_kernel, dim3(nruns), dim3(256), 0, s, args, nsum_for(nv) + 1, amax, flags);
     return hipGetLastError();
 }
 

@@ -21,25 +21,18 @@
 #include <vector>
 
 #include "../../include/cdc_solver.h"
+#include "cdc_host.h"
 #include "cdc_internal.h"
+
+using cdc::host::NBUF;
+using cdc::host::set_msg;
+using cdc::host::Slot;
+using cdc::host::StepPlan;
 
 namespace {
 
 thread_local std::string g_msg;
 
-int set_msg(int code, const std::string &m) {
-    g_msg = m;
-    return code;
-}
-
-#define HIP_TRY(expr)                                                                       \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return set_msg(CDC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-constexpr int NBUF = 4;
 constexpr double ORBITAL_PERIOD = 2.0 * 3.14159265358979323846;
 
 int64_t gcd64(int64_t a, int64_t b) {
@@ -73,42 +66,10 @@ double softening_length(const cdc_physics_t &p) {
 
 }  // namespace
 
-struct cdc_solver {
-    cdc_mesh_t mesh;
-    cdc_hydro_t hydro;
-    cdc_physics_t phys;
-    int nv, nb, bs, N, nblk, ntiles, tpb, nsum, max_tiles;
-    cdc::TileShape ts;
-    hipStream_t stream = nullptr;
-    int device = 0;
-
-    double *d_init = nullptr;
-    double *d_u[NBUF] = {};
-    double *d_ist[NBUF] = {};
-    double *d_oec[NBUF] = {};
-    double *d_tile_sums = nullptr;
-    cdc::TileErr *d_tile_err = nullptr;
-    int32_t *d_perr = nullptr;
-    int32_t *d_err_flag = nullptr;
-    double *d_aux = nullptr;     // static per-cell planes (buffer rate, beta-cooling Omega)
-    int naux = 1;
-    double *d_stage = nullptr;   // AoS staging
-    double *d_amax = nullptr;    // max-speed partials
-    int namax = 0;
-
-    int cur = 0;                 // committed buffer
-    double time = 0.0;
-    int64_t num = 0, den = 1;
-    bool amax_valid = false;     // tile_sums[:, nsum] hold the committed state's max speed
-
-    bool timing = false;
-    int64_t launches = 0;
-    double total_ms = 0.0;
-    std::vector<hipEvent_t> ev;
-
-    size_t ncell() const { return (size_t)N * N; }
-    size_t nstate() const { return ncell() * nv; }
-};
+int cdc::host::set_msg(int code, const std::string &m) {
+    g_msg = m;
+    return code;
+}
 
 namespace {
 
@@ -198,6 +159,9 @@ void free_all(cdc_solver *s) {
     (void)hipFree(s->d_aux);
     (void)hipFree(s->d_stage);
     (void)hipFree(s->d_amax);
+    (void)hipFree(s->d_args);
+    (void)hipFree(s->d_begin);
+    if (s->h_args) (void)hipHostFree(s->h_args);
     for (auto e : s->ev) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
@@ -251,19 +215,239 @@ int compute_aux(cdc_solver *s) {
     return CDC_OK;
 }
 
-// The state that one RK stage reads or writes: buffer index plus the time
-// and iteration carried with it.
-struct Slot {
-    int idx;
-    double time;
-    int64_t num, den;
-};
-
-struct StageRec {
-    double time;      // stage input time (orbital state of the failing stage)
-};
-
 }  // namespace
+
+namespace cdc {
+namespace host {
+
+int launch_set_int_checked(cdc_solver *s) {
+    HIP_TRY(cdc::launch_set_int(s->d_err_flag, 0, s->stream));
+    return CDC_OK;
+}
+
+int launch_stages(cdc_solver *s, const StageArgs *args, size_t n, const cdc::TileShape &ts, hipEvent_t *ev,
+                  bool finalize) {
+    if (n > s->cap_args) {
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        if (s->h_args) (void)hipHostFree(s->h_args);
+        if (s->d_args) (void)hipFree(s->d_args);
+        s->h_args = nullptr;
+        s->d_args = nullptr;
+        s->cap_args = 0;
+        const size_t cap = std::max<size_t>(n, 32);
+        HIP_TRY(hipHostMalloc((void **)&s->h_args, cap * sizeof(StageArgs), hipHostMallocDefault));
+        HIP_TRY(hipMalloc((void **)&s->d_args, cap * sizeof(StageArgs)));
+        s->cap_args = cap;
+    }
+    // the staging buffer is free: every earlier call synchronised its stream
+    memcpy(s->h_args, args, n * sizeof(StageArgs));
+    HIP_TRY(hipMemcpyAsync(s->d_args, s->h_args, n * sizeof(StageArgs), hipMemcpyHostToDevice, s->stream));
+    const int32_t begin[4] = {0, args[0].ntiles, 0, s->nblk};
+    HIP_TRY(hipMemcpyAsync(s->d_begin, begin, sizeof(begin), hipMemcpyHostToDevice, s->stream));
+    for (size_t i = 0; i < n; ++i) {
+        if (ev) HIP_TRY(hipEventRecord(ev[2 * i], s->stream));
+        HIP_TRY(cdc::launch_stage_batch(s->nv, ts, s->d_args + i, s->d_begin, 1, args[i].ntiles, s->stream));
+        if (ev) HIP_TRY(hipEventRecord(ev[2 * i + 1], s->stream));
+        if (finalize)
+            HIP_TRY(cdc::launch_finalize_batch(s->nv, s->d_args + i, s->d_begin + 2, 1, s->nblk, s->stream));
+    }
+    return CDC_OK;
+}
+
+int compute_amax(cdc_solver *s, double *out) {
+    std::vector<double> h;
+    if (s->amax_valid) {
+        // max signal speed of the committed state, left by the last stage
+        const int nr = s->nsum + 1;
+        double a = 0.0;
+        HIP_TRY(cdc::launch_column_max(s->d_tile_sums + s->nsum, s->ntiles, nr, s->d_amax, s->stream));
+        HIP_TRY(hipMemcpyAsync(&a, s->d_amax, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        *out = a;
+        return CDC_OK;
+    }
+    HIP_TRY(cdc::launch_max_speed(s->nv, s->d_u[s->cur], (int64_t)s->ncell(),
+                                  s->hydro.kind == CDC_EULER ? s->hydro.gamma_law_index : 1.0, s->d_amax, s->namax,
+                                  s->stream));
+    h.resize(s->namax);
+    HIP_TRY(hipMemcpyAsync(h.data(), s->d_amax, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    double a = 0.0;
+    for (double v : h) a = std::max(v, a);
+    *out = a;
+    return CDC_OK;
+}
+
+double step_dt(const cdc_solver *s, double amax) {
+    const cdc_physics_t &ph = s->phys;
+    const double L = 2.0 * s->mesh.domain_radius / (double)s->nb;
+    const double dx = L / (double)s->bs;
+    double dt = ph.cfl * dx / amax;
+    if (ph.has_dt_min) dt = std::max(dt, ph.dt_min);
+    return dt;
+}
+
+void plan_steps(const cdc_solver *s, int fold, double dt, StepPlan &plan) {
+    const cdc_physics_t &ph = s->phys;
+    const double L = 2.0 * s->mesh.domain_radius / (double)s->nb;
+    double el[4];
+    cdc::orbital_elements(ph, el);
+    const int rk = ph.rk_order;
+    plan.args.clear();
+    plan.stage_times.clear();
+    plan.args.reserve((size_t)fold * rk);
+    plan.stage_times.reserve((size_t)fold * rk);
+
+    auto pick = [&](int a, int b, int c) {
+        for (int i = 0; i < NBUF; ++i)
+            if (i != a && i != b && i != c) return i;
+        return -1;
+    };
+
+    // one stage: out = L(in) combined with base when read_base
+    auto stage = [&](const Slot &in, const Slot &base, bool read_base, int64_t bn, int64_t bd, bool last,
+                     Slot &out) {
+        const int sid = (int)plan.args.size() + 1;
+        plan.stage_times.push_back(in.time);
+        out.idx = pick(in.idx, base.idx, s->cur);
+        const double bf = read_base ? (double)bn / (double)bd : 0.0;
+        const double c1 = -bf + 1.0;
+
+        cdc::StageArgs a;
+        memset(&a, 0, sizeof(a));
+        a.uc = s->d_u[in.idx];
+        a.u0 = s->d_init;
+        a.ub = s->d_u[base.idx];
+        a.uo = s->d_u[out.idx];
+        a.aux = s->d_aux;
+        a.tile_sums = s->d_tile_sums;
+        a.tile_err = s->d_tile_err;
+        a.err_flag = s->d_err_flag;
+        a.ist_c = s->d_ist[in.idx];
+        a.ist_b = s->d_ist[base.idx];
+        a.oec_c = s->d_oec[in.idx];
+        a.oec_b = s->d_oec[base.idx];
+        a.ist_o = s->d_ist[out.idx];
+        a.oec_o = s->d_oec[out.idx];
+        a.perr = s->d_perr;
+        a.reuse_prim = s->hydro.has_density_floor ? 0 : 1;
+        a.N = s->N;
+        a.nb = s->nb;
+        a.bs = s->bs;
+        a.ntiles = s->ntiles;
+        a.tpb = s->tpb;
+        a.stage_id = sid;
+        a.read_base = read_base ? 1 : 0;
+        a.want_amax = last ? 1 : 0;
+        a.R = s->mesh.domain_radius;
+        a.L = L;
+        a.time = in.time;
+        a.dt = dt;
+        a.c1 = c1;
+        a.c0 = bf;
+        a.orb = cdc::orbital_state(el, in.time);
+        a.k = make_consts(s, in.time, dt);
+        a.area = a.k.dx * a.k.dy;
+        for (int i = 0; i < 4; ++i) a.el[i] = el[i];
+        plan.args.push_back(a);
+
+        // time and iteration of the stage result, then of the combine
+        double t1 = in.time + dt;
+        int64_t n1 = in.num + in.den, d1 = in.den;
+        rat_norm(n1, d1);
+        if (read_base) {
+            out.time = t1 * c1 + base.time * bf;
+            rat_wavg(n1, d1, base.num, base.den, bn, bd, out.num, out.den);
+        } else {
+            out.time = t1;
+            out.num = n1;
+            out.den = d1;
+        }
+    };
+
+    Slot cur{s->cur, s->time, s->num, s->den};
+    for (int f = 0; f < fold; ++f) {
+        const bool lastf = f == fold - 1;
+        Slot s1, s2, s3;
+        if (rk == 1) {
+            stage(cur, cur, false, 0, 1, lastf, s1);
+            cur = s1;
+        } else if (rk == 2) {
+            stage(cur, cur, false, 0, 1, false, s1);
+            stage(s1, cur, true, 1, 2, lastf, s2);
+            cur = s2;
+        } else {
+            stage(cur, cur, false, 0, 1, false, s1);
+            stage(s1, cur, true, 3, 4, false, s2);
+            stage(s2, cur, true, 1, 3, lastf, s3);
+            cur = s3;
+        }
+    }
+    plan.out = cur;
+}
+
+void commit(cdc_solver *s, const Slot &out) {
+    s->cur = out.idx;
+    s->time = out.time;
+    s->num = out.num;
+    s->den = out.den;
+    s->amax_valid = true;
+    s->version += 1;
+}
+
+int resolve_error(cdc_solver *s, int flag, const StepPlan &plan, cdc_error_t *err) {
+    // Resolve the error as join_all over blocks would (src/scheme.rs:161-165):
+    // block b fails with the first failing 3x3 neighbour's P-phase error
+    // (try_join_3by3 order), else with its own perturbation error.
+    s->amax_valid = false;
+    if (flag < 1 || (size_t)flag > plan.stage_times.size())
+        return set_msg(CDC_ERR_HIP, "error flag names no stage of the call");
+    std::vector<cdc::TileErr> te((size_t)s->ntiles);
+    std::vector<int32_t> pe((size_t)s->nblk);
+    HIP_TRY(hipMemcpy(te.data(), s->d_tile_err, te.size() * sizeof(cdc::TileErr), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(pe.data(), s->d_perr, pe.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<cdc::TileErr> berr((size_t)s->nblk, cdc::TileErr{0, 0, 0.0});
+    for (int b = 0; b < s->nblk; ++b)
+        for (int t = 0; t < s->tpb; ++t) {
+            const cdc::TileErr &e = te[(size_t)b * s->tpb + t];
+            if (e.code && (!berr[b].code || e.cell < berr[b].cell)) berr[b] = e;
+        }
+    const double stage_time = plan.stage_times[(size_t)flag - 1];
+    double el[4];
+    cdc::orbital_elements(s->phys, el);
+    cdc::Orbit orb = cdc::orbital_state(el, stage_time);
+    for (int b = 0; b < s->nblk; ++b) {
+        const int bi = b / s->nb, bj = b % s->nb;
+        int code = 0, nbk = -1;
+        for (int r = 0; r < 3 && !code; ++r)
+            for (int c = 0; c < 3 && !code; ++c) {
+                const int n = ((bi + s->nb + r - 1) % s->nb) * s->nb + (bj + s->nb + c - 1) % s->nb;
+                if (berr[n].code) { code = berr[n].code; nbk = n; }
+            }
+        if (!code && pe[b]) code = pe[b];
+        if (!code) continue;
+        if (err) {
+            err->code = code;
+            err->time = stage_time;
+            err->has_binary = 1;
+            err->binary[0] = {orb.m1, orb.x1, orb.y1, orb.vx1, orb.vy1};
+            err->binary[1] = {orb.m2, orb.x2, orb.y2, orb.vx2, orb.vy2};
+            if (nbk >= 0) {
+                err->value = berr[nbk].value;
+                cell_centre(s, nbk, berr[nbk].cell, &err->x, &err->y);
+            }
+            snprintf(err->message, sizeof(err->message), "%s %.4e at position (%.4f %.4f)", cdc_error_string(code),
+                     err->value, err->x, err->y);
+        }
+        return set_msg(code, cdc_error_string(code));
+    }
+    return set_msg(CDC_ERR_HIP, "error flag set without a failing block");
+}
+
+}  // namespace host
+}  // namespace cdc
+
+using cdc::host::compute_amax;
 
 extern "C" {
 
@@ -378,6 +562,7 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
     ALLOC(s->d_aux, (size_t)s->naux * s->ncell() * sizeof(double));
     s->namax = 1024;
     ALLOC(s->d_amax, (size_t)s->namax * sizeof(double));
+    ALLOC(s->d_begin, 4 * sizeof(int32_t));
 #undef ALLOC
 
     rc = upload_conserved(s, initial_conserved, s->d_init);
@@ -430,6 +615,7 @@ int32_t cdc_solver_set_state(cdc_solver_t *s, double time, int64_t num, int64_t 
     s->den = den;
     rat_norm(s->num, s->den);
     s->amax_valid = false;
+    s->version += 1;
     return CDC_OK;
 }
 
@@ -450,30 +636,6 @@ int32_t cdc_solver_get_state(cdc_solver_t *s, double *time, int64_t *num, int64_
         HIP_TRY(hipMemcpyAsync(oec, s->d_oec[s->cur], (size_t)s->nblk * 7 * 4 * sizeof(double),
                                hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    return CDC_OK;
-}
-
-static int compute_amax(cdc_solver *s, double *out) {
-    std::vector<double> h;
-    if (s->amax_valid) {
-        // max signal speed of the committed state, left by the last stage
-        const int nr = s->nsum + 1;
-        double a = 0.0;
-        HIP_TRY(cdc::launch_column_max(s->d_tile_sums + s->nsum, s->ntiles, nr, s->d_amax, s->stream));
-        HIP_TRY(hipMemcpyAsync(&a, s->d_amax, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-        HIP_TRY(hipStreamSynchronize(s->stream));
-        *out = a;
-        return CDC_OK;
-    }
-    HIP_TRY(cdc::launch_max_speed(s->nv, s->d_u[s->cur], (int64_t)s->ncell(),
-                                  s->hydro.kind == CDC_EULER ? s->hydro.gamma_law_index : 1.0, s->d_amax, s->namax,
-                                  s->stream));
-    h.resize(s->namax);
-    HIP_TRY(hipMemcpyAsync(h.data(), s->d_amax, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    double a = 0.0;
-    for (double v : h) a = std::max(v, a);
-    *out = a;
     return CDC_OK;
 }
 
@@ -513,15 +675,17 @@ int32_t cdc_solver_autotune(cdc_solver_t *s, int32_t reps, int32_t *tx, int32_t 
     const double L = 2.0 * s->mesh.domain_radius / (double)s->nb;
     double el[4];
     cdc::orbital_elements(s->phys, el);
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
     const int scratch = (s->cur + 1) % NBUF;
     double best = 1e300;
     cdc::TileShape pick = s->ts;
-    for (int i = 0; i < n; ++i) {
+    std::vector<hipEvent_t> marks(2 * ((size_t)reps + 1), nullptr);
+    for (auto &m : marks) HIP_TRY(hipEventCreate(&m));
+    int rc = CDC_OK;
+    for (int i = 0; i < n && rc == CDC_OK; ++i) {
         const cdc::TileShape t = shapes[i];
         if (s->bs % t.tx != 0 || s->bs % t.ty != 0) continue;
+        // stage launches only (no finalize) from the committed state into a
+        // scratch buffer, in the costlier later-stage form
         cdc::StageArgs a;
         memset(&a, 0, sizeof(a));
         a.uc = s->d_u[s->cur];
@@ -535,9 +699,10 @@ int32_t cdc_solver_autotune(cdc_solver_t *s, int32_t reps, int32_t *tx, int32_t 
         a.N = s->N;
         a.nb = s->nb;
         a.bs = s->bs;
-        a.ntiles = s->nblk * (s->bs / t.tx) * (s->bs / t.ty);
+        a.tpb = (s->bs / t.tx) * (s->bs / t.ty);
+        a.ntiles = s->nblk * a.tpb;
         a.stage_id = 1;
-        a.read_base = 1;        // time the costlier later-stage form
+        a.read_base = 1;
         a.reuse_prim = s->hydro.has_density_floor ? 0 : 1;
         a.R = s->mesh.domain_radius;
         a.L = L;
@@ -547,21 +712,24 @@ int32_t cdc_solver_autotune(cdc_solver_t *s, int32_t reps, int32_t *tx, int32_t 
         a.c0 = 0.5;
         a.orb = cdc::orbital_state(el, s->time);
         a.k = make_consts(s, s->time, a.dt);
-        HIP_TRY(cdc::launch_set_int(s->d_err_flag, 0, s->stream));
-        HIP_TRY(cdc::launch_stage(s->nv, t, a, s->stream));   // warm-up
-        HIP_TRY(hipEventRecord(e0, s->stream));
-        for (int r = 0; r < reps; ++r) HIP_TRY(cdc::launch_stage(s->nv, t, a, s->stream));
-        HIP_TRY(hipEventRecord(e1, s->stream));
-        HIP_TRY(hipEventSynchronize(e1));
-        float ms = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        const double per = ms / reps;
+        const std::vector<cdc::StageArgs> run((size_t)reps + 1, a);   // first launch warms up
+        if ((rc = cdc::host::launch_set_int_checked(s))) break;
+        if ((rc = cdc::host::launch_stages(s, run.data(), run.size(), t, marks.data(), false))) break;
+        const hipError_t es = hipEventSynchronize(marks.back());
+        if (es != hipSuccess) { rc = set_msg(CDC_ERR_HIP, hipGetErrorString(es)); break; }
+        double sum = 0.0;
+        for (int r = 1; r <= reps; ++r) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, marks[2 * r], marks[2 * r + 1]);
+            sum += ms;
+        }
+        const double per = sum / reps;
         if (per < best) { best = per; pick = t; }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    for (auto &m : marks) (void)hipEventDestroy(m);
+    if (rc) return rc;
     HIP_TRY(cdc::launch_set_int(s->d_err_flag, 0, s->stream));
-    int rc = cdc_solver_set_tile(s, pick.tx, pick.ty);
+    rc = cdc_solver_set_tile(s, pick.tx, pick.ty);
     if (rc) return rc;
     if (tx) *tx = pick.tx;
     if (ty) *ty = pick.ty;
@@ -595,193 +763,45 @@ int32_t cdc_solver_advance(cdc_solver_t *s, int32_t fold, double *dt_out, cdc_er
     if (err) memset(err, 0, sizeof(*err));
     if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
     if (fold < 0) return set_msg(CDC_ERR_INVALID_ARGUMENT, "fold must be >= 0");
-    const cdc_physics_t &ph = s->phys;
 
     // dt = cfl * dx / max signal speed (src/scheme.rs:336-340)
     double amax = 0.0;
     int rc = compute_amax(s, &amax);
     if (rc) return rc;
-    const double L = 2.0 * s->mesh.domain_radius / (double)s->nb;
-    const double dx = L / (double)s->bs;
-    double dt = ph.cfl * dx / amax;
-    if (ph.has_dt_min) dt = std::max(dt, ph.dt_min);
+    const double dt = cdc::host::step_dt(s, amax);
     if (dt_out) *dt_out = dt;
     if (fold == 0) return CDC_OK;
 
-    HIP_TRY(cdc::launch_set_int(s->d_err_flag, 0, s->stream));
-
-    double el[4];
-    cdc::orbital_elements(ph, el);
-    const int rk = ph.rk_order;
-    std::vector<StageRec> stages;
-    stages.reserve((size_t)fold * rk);
-
-    if (s->timing && s->ev.size() < (size_t)2 * fold * rk) {
-        const size_t need = (size_t)2 * fold * rk;
-        while (s->ev.size() < need) {
+    StepPlan plan;
+    cdc::host::plan_steps(s, fold, dt, plan);
+    const size_t nst = plan.args.size();
+    if (s->timing) {
+        while (s->ev.size() < 2 * nst) {
             hipEvent_t e;
             HIP_TRY(hipEventCreate(&e));
             s->ev.push_back(e);
         }
     }
-    int nev = 0;
-
-    auto pick = [&](int a, int b, int c) {
-        for (int i = 0; i < NBUF; ++i)
-            if (i != a && i != b && i != c) return i;
-        return -1;
-    };
-
-    // one stage: out = L(in) combined with base when read_base
-    auto stage = [&](const Slot &in, const Slot &base, bool read_base, int64_t bn, int64_t bd, bool last,
-                     Slot &out) -> int {
-        const int sid = (int)stages.size() + 1;
-        stages.push_back({in.time});
-        out.idx = pick(in.idx, base.idx, s->cur);
-        const double bf = read_base ? (double)bn / (double)bd : 0.0;
-        const double c1 = -bf + 1.0;
-        cdc::Orbit orb = cdc::orbital_state(el, in.time);
-
-        cdc::StageArgs a;
-        memset(&a, 0, sizeof(a));
-        a.uc = s->d_u[in.idx];
-        a.u0 = s->d_init;
-        a.ub = s->d_u[base.idx];
-        a.uo = s->d_u[out.idx];
-        a.aux = s->d_aux;
-        a.tile_sums = s->d_tile_sums;
-        a.tile_err = s->d_tile_err;
-        a.err_flag = s->d_err_flag;
-        a.ist_c = s->d_ist[in.idx];
-        a.ist_b = s->d_ist[base.idx];
-        a.oec_c = s->d_oec[in.idx];
-        a.oec_b = s->d_oec[base.idx];
-        a.ist_o = s->d_ist[out.idx];
-        a.oec_o = s->d_oec[out.idx];
-        a.perr = s->d_perr;
-        a.reuse_prim = s->hydro.has_density_floor ? 0 : 1;
-        a.N = s->N;
-        a.nb = s->nb;
-        a.bs = s->bs;
-        a.ntiles = s->ntiles;
-        a.stage_id = sid;
-        a.read_base = read_base ? 1 : 0;
-        a.want_amax = last ? 1 : 0;
-        a.R = s->mesh.domain_radius;
-        a.L = L;
-        a.time = in.time;
-        a.dt = dt;
-        a.c1 = c1;
-        a.c0 = bf;
-        a.orb = orb;
-        a.k = make_consts(s, in.time, dt);
-        a.area = a.k.dx * a.k.dy;
-        for (int i = 0; i < 4; ++i) a.el[i] = el[i];
-        if (s->timing) HIP_TRY(hipEventRecord(s->ev[nev], s->stream));
-        HIP_TRY(cdc::launch_stage(s->nv, s->ts, a, s->stream));
-        if (s->timing) HIP_TRY(hipEventRecord(s->ev[nev + 1], s->stream));
-        if (s->timing) nev += 2;
-        HIP_TRY(cdc::launch_finalize(s->nv, a, s->tpb, s->stream));
-
-
-
-        // time and iteration of the stage result, then of the combine
-        double t1 = in.time + dt;
-        int64_t n1 = in.num + in.den, d1 = in.den;
-        rat_norm(n1, d1);
-        if (read_base) {
-            out.time = t1 * c1 + base.time * bf;
-            rat_wavg(n1, d1, base.num, base.den, bn, bd, out.num, out.den);
-        } else {
-            out.time = t1;
-            out.num = n1;
-            out.den = d1;
-        }
-        return CDC_OK;
-    };
-
-    Slot cur{s->cur, s->time, s->num, s->den};
-    for (int f = 0; f < fold; ++f) {
-        const bool lastf = f == fold - 1;
-        Slot s1, s2, s3;
-        if (rk == 1) {
-            if ((rc = stage(cur, cur, false, 0, 1, lastf, s1))) return rc;
-            cur = s1;
-        } else if (rk == 2) {
-            if ((rc = stage(cur, cur, false, 0, 1, false, s1))) return rc;
-            if ((rc = stage(s1, cur, true, 1, 2, lastf, s2))) return rc;
-            cur = s2;
-        } else {
-            if ((rc = stage(cur, cur, false, 0, 1, false, s1))) return rc;
-            if ((rc = stage(s1, cur, true, 3, 4, false, s2))) return rc;
-            if ((rc = stage(s2, cur, true, 1, 3, lastf, s3))) return rc;
-            cur = s3;
-        }
-    }
+    HIP_TRY(cdc::launch_set_int(s->d_err_flag, 0, s->stream));
+    rc = cdc::host::launch_stages(s, plan.args.data(), nst, s->ts, s->timing ? s->ev.data() : nullptr, true);
+    if (rc) return rc;
 
     int32_t flag = 0;
     HIP_TRY(hipMemcpyAsync(&flag, s->d_err_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     if (s->timing) {
-        for (int i = 0; i < nev; i += 2) {
+        for (size_t i = 0; i < nst; ++i) {
             float ms = 0.0f;
-            HIP_TRY(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]));
+            HIP_TRY(hipEventElapsedTime(&ms, s->ev[2 * i], s->ev[2 * i + 1]));
             s->total_ms += ms;
             s->launches += 1;
         }
     }
-
     if (flag == 0) {
-        s->cur = cur.idx;
-        s->time = cur.time;
-        s->num = cur.num;
-        s->den = cur.den;
-        s->amax_valid = true;
+        cdc::host::commit(s, plan.out);
         return CDC_OK;
     }
-
-    // Resolve the error as join_all over blocks would (src/scheme.rs:161-165):
-    // block b fails with the first failing 3x3 neighbour's P-phase error
-    // (try_join_3by3 order), else with its own perturbation error.
-    s->amax_valid = false;
-    std::vector<cdc::TileErr> te((size_t)s->ntiles);
-    std::vector<int32_t> pe((size_t)s->nblk);
-    HIP_TRY(hipMemcpy(te.data(), s->d_tile_err, te.size() * sizeof(cdc::TileErr), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(pe.data(), s->d_perr, pe.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-    std::vector<cdc::TileErr> berr((size_t)s->nblk, cdc::TileErr{0, 0, 0.0});
-    for (int b = 0; b < s->nblk; ++b)
-        for (int t = 0; t < s->tpb; ++t) {
-            const cdc::TileErr &e = te[(size_t)b * s->tpb + t];
-            if (e.code && (!berr[b].code || e.cell < berr[b].cell)) berr[b] = e;
-        }
-    const double stage_time = stages[(size_t)flag - 1].time;
-    cdc::Orbit orb = cdc::orbital_state(el, stage_time);
-    for (int b = 0; b < s->nblk; ++b) {
-        const int bi = b / s->nb, bj = b % s->nb;
-        int code = 0, nbk = -1;
-        for (int r = 0; r < 3 && !code; ++r)
-            for (int c = 0; c < 3 && !code; ++c) {
-                const int n = ((bi + s->nb + r - 1) % s->nb) * s->nb + (bj + s->nb + c - 1) % s->nb;
-                if (berr[n].code) { code = berr[n].code; nbk = n; }
-            }
-        if (!code && pe[b]) code = pe[b];
-        if (!code) continue;
-        if (err) {
-            err->code = code;
-            err->time = stage_time;
-            err->has_binary = 1;
-            err->binary[0] = {orb.m1, orb.x1, orb.y1, orb.vx1, orb.vy1};
-            err->binary[1] = {orb.m2, orb.x2, orb.y2, orb.vx2, orb.vy2};
-            if (nbk >= 0) {
-                err->value = berr[nbk].value;
-                cell_centre(s, nbk, berr[nbk].cell, &err->x, &err->y);
-            }
-            snprintf(err->message, sizeof(err->message), "%s %.4e at position (%.4f %.4f)", cdc_error_string(code),
-                     err->value, err->x, err->y);
-        }
-        return set_msg(code, cdc_error_string(code));
-    }
-    return set_msg(CDC_ERR_HIP, "error flag set without a failing block");
+    return cdc::host::resolve_error(s, flag, plan, err);
 }
 
 }  // extern "C"

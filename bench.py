@@ -51,6 +51,64 @@ def cpu_baseline(w, u0, seconds, threads):
             "sample": f"{n} RK steps of the same workload ({el:.1f} s), C oracle, OpenMP over blocks"}
 
 
+def run_ensemble(args):
+    """The 64-run parameter study, every run from t = 0 to half an orbit,
+    one ensemble call per step of all unfinished runs."""
+    import math
+    import numpy as np
+    from cdc_amd import Ensemble, Solver, workloads
+    members = workloads.ensemble_members()
+    inits = [workloads.initial_state(m) for m in members]
+    solvers = [Solver(m["mesh"], m["hydro"], m["physics"], u) for m, u in zip(members, inits)]
+    ens = Ensemble(solvers)
+    for _ in range(args.warmup):
+        ens.advance()
+    for g, u in zip(solvers, inits):
+        g.set_state(u)
+    target = 0.5 * 2.0 * math.pi
+    n = len(solvers)
+    times = np.zeros(n)
+    calls = 0
+    z0 = ens.stats()[1]
+    t0 = time.perf_counter()
+    while True:
+        active = (times < target).astype(np.int32)
+        if not active.any():
+            break
+        dt, codes, errs = ens.advance(active)
+        if np.any(codes != 0):
+            bad = int(np.nonzero(codes)[0][0])
+            raise RuntimeError(f"ensemble member {bad} failed: {errs[bad]}")
+        times = np.where(active == 1, times + dt, times)
+        calls += 1
+    wall = time.perf_counter() - t0
+    zones = ens.stats()[1] - z0
+    exact = np.array([g.time() for g in solvers])
+    assert np.all(exact >= target * (1 - 1e-12)), "a run stopped short of half an orbit"
+    sizes = sorted({m["mesh"].num_blocks * m["mesh"].block_size for m in members})
+    line = {
+        "metric": "f64 zone-updates/s, 64-run ensemble to half an orbit",
+        "value": zones / wall,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": calls,
+        "warmup": args.warmup,
+        "ms_per_step": wall / calls * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded cavity disks, parameters drawn with seed 4)",
+        "config": {"workload": "ensemble64", "runs": n, "grids": [f"{s}^2" for s in sizes],
+                   "hydro": "isothermal", "rk": "RK2", "tile": "%dx%d" % ens.get_tile(),
+                   "zone_updates": zones, "wall_s": wall},
+    }
+    ens.close()
+    for g in solvers:
+        g.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +126,9 @@ def main():
               f"(see DESIGN.md for how the path would shard)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "ensemble64":
+        run_ensemble(args)
+        return
     from cdc_amd import Solver, workloads
     w = workloads.get(args.workload)
     u0 = workloads.initial_state(w)

@@ -229,6 +229,40 @@ int32_t cdc_solver_set_timing(cdc_solver_t *solver, int32_t enable);
 int32_t cdc_solver_get_timing(cdc_solver_t *solver, int64_t *launches,
                               double *total_ms, double *cells_per_launch);
 
+/* ---- ensemble: one RK step of many runs per call ------------------------- */
+/*
+ * No counterpart in the reference (scheme::advance, src/scheme.rs:323-349,
+ * steps one run; a parameter study there is one process per run).  An
+ * ensemble groups existing solver handles of one hydro kind and rk_order on
+ * one device.  cdc_ensemble_advance advances every active member by one RK
+ * step at the member's own dt (fold = 1), launching each RK stage once for
+ * all members, and leaves each member bitwise where
+ * cdc_solver_advance(member, 1, ...) would have.  A member that fails keeps
+ * its pre-call state and reports the error cdc_solver_advance would have
+ * (codes_out[i], errs_out[i]); the others commit.  Creating the ensemble sets
+ * every member's tile shape to one shared shape (results of a member depend
+ * on its tile shape only through the summation order of the integrated
+ * source terms).  Members stay usable on their own; the ensemble does not
+ * own them and must be destroyed before them.
+ *   active:    n flags (NULL = all), dt_out: n (written for active members),
+ *   codes_out: n, errs_out: n (either may be NULL).
+ * Returns CDC_OK, or the code of the first failing member in member order.
+ */
+typedef struct cdc_ensemble cdc_ensemble_t;
+
+int32_t cdc_ensemble_create(cdc_solver_t *const *members, int32_t n,
+                            cdc_ensemble_t **out);
+void    cdc_ensemble_destroy(cdc_ensemble_t *ensemble);
+int32_t cdc_ensemble_advance(cdc_ensemble_t *ensemble, const int32_t *active,
+                             double *dt_out, int32_t *codes_out,
+                             cdc_error_t *errs_out);
+/* Replay each step from a captured HIP graph (default 1) or launch it directly. */
+int32_t cdc_ensemble_set_graph(cdc_ensemble_t *ensemble, int32_t enable);
+int32_t cdc_ensemble_get_tile(cdc_ensemble_t *ensemble, int32_t *tx, int32_t *ty);
+/* Member-steps committed and zone-updates done since creation. */
+int32_t cdc_ensemble_get_stats(cdc_ensemble_t *ensemble, int64_t *member_steps,
+                               double *zone_updates);
+
 /* ---- host utilities (no GPU) -------------------------------------------- */
 
 /* Physics::orbital_state_from_time (src/physics.rs:419-421). */

@@ -1,0 +1,139 @@
+"""The ensemble path (cdc_ensemble_*): one launch per RK stage over many
+runs.  Every member must end bitwise where advancing it alone
+(cdc_solver_advance(member, 1)) leaves it, with the graph replay and with
+direct launches, under an active mask, and when a member fails."""
+import numpy as np
+import pytest
+
+from cdc_amd import Ensemble, HydroError, Solver, abi
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    lib = abi.load()
+    assert lib.cdc_device_count() >= 1, "no HIP device: the HIP path must run on a GPU"
+
+
+def members_iso():
+    out = []
+    for i, (nb, bs, kw) in enumerate([
+            (2, 32, dict(binary_eccentricity=0.3, binary_mass_ratio=0.7)),
+            (4, 16, dict(viscosity={"alpha": 0.1})),
+            (1, 64, dict(sink_model="torque_free", binary_mass_ratio=0.4)),
+            (3, 16, dict(binary_eccentricity=0.6, viscosity={"constant_nu": 3e-3}))]):
+        mesh, hydro, ph = H.small("iso", nb=nb, bs=bs, R=4.0, rk_order="RK2", **kw)
+        out.append((mesh, hydro, ph, H.initial(mesh, hydro, ph, seed=11 + i)))
+    return out
+
+
+def members_euler():
+    out = []
+    for i, (kind, nb, bs, kw) in enumerate([
+            ("euler", 2, 32, dict(viscosity={"alpha": 0.1})),
+            ("euler_beta", 2, 16, dict(binary_eccentricity=0.4))]):
+        mesh, hydro, ph = H.small(kind, nb=nb, bs=bs, R=4.0, rk_order="RK3", **kw)
+        out.append((mesh, hydro, ph, H.initial(mesh, hydro, ph, seed=21 + i)))
+    return out
+
+
+def build(specs, graph=True):
+    solo = [Solver(*s) for s in specs]
+    batch = [Solver(*s) for s in specs]
+    ens = Ensemble(batch, graph=graph)
+    tile = ens.get_tile()
+    for g in solo:
+        g.set_tile(*tile)
+    return solo, batch, ens
+
+
+def same_state(a, b):
+    sa, sb = a.get_state(), b.get_state()
+    assert sa["time"] == sb["time"] and sa["iteration"] == sb["iteration"]
+    np.testing.assert_array_equal(sa["conserved"], sb["conserved"])
+    np.testing.assert_array_equal(sa["ist"], sb["ist"])
+    np.testing.assert_array_equal(sa["oec"], sb["oec"])
+
+
+@pytest.mark.parametrize("graph", [True, False], ids=["graph", "direct"])
+@pytest.mark.parametrize("which", ["iso", "euler"])
+def test_members_match_solo_bitwise(which, graph):
+    specs = members_iso() if which == "iso" else members_euler()
+    solo, batch, ens = build(specs, graph)
+    for step in range(4):
+        dt, codes, errs = ens.advance()
+        assert np.all(codes == 0) and all(e is None for e in errs)
+        for i, g in enumerate(solo):
+            assert g.advance(1) == dt[i], f"member {i} step {step}: dt differs"
+            same_state(g, batch[i])
+    n, z = ens.stats()
+    assert n == 4 * len(specs)
+    assert z == 4 * sum(m.mesh.total_zones() for m in batch)
+
+
+def test_active_mask_and_solo_steps_between_batches():
+    specs = members_iso()
+    solo, batch, ens = build(specs)
+    ens.advance()
+    for g in solo:
+        g.advance(1)
+    active = np.array([1, 0, 1, 1], dtype=np.int32)
+    before = batch[1].get_state()
+    dt, codes, _ = ens.advance(active)
+    assert np.isnan(dt[1]) and np.all(codes == 0)
+    after = batch[1].get_state()
+    np.testing.assert_array_equal(before["conserved"], after["conserved"])
+    for i in (0, 2, 3):
+        solo[i].advance(1)
+    # a member advanced on its own between batch steps
+    batch[1].advance(1)
+    solo[1].advance(1)
+    for _ in range(2):
+        ens.advance()
+        for g in solo:
+            g.advance(1)
+    for g, b in zip(solo, batch):
+        same_state(g, b)
+
+
+def test_failing_member_rolls_back_alone():
+    specs = members_iso()
+    solo, batch, ens = build(specs)
+    ens.advance()
+    for g in solo:
+        g.advance(1)
+    bad = batch[2].get_state()
+    u = bad["conserved"].copy()
+    u[0, 5, 7, 0] = -2e-9
+    for g in (solo[2], batch[2]):
+        g.set_state(u, time=bad["time"], iteration=bad["iteration"], ist=bad["ist"], oec=bad["oec"])
+    dt, codes, errs = ens.advance()
+    assert codes[2] == abi.CDC_ERR_NEGATIVE_DENSITY and isinstance(errs[2], HydroError)
+    assert [codes[i] for i in (0, 1, 3)] == [0, 0, 0]
+    with pytest.raises(HydroError) as es:
+        solo[2].advance(1)
+    e, s = errs[2], es.value
+    assert (e.kind, e.value, e.position, e.time) == (s.kind, s.value, s.position, s.time)
+    np.testing.assert_array_equal(batch[2].get_state()["conserved"], u)
+    for i in (0, 1, 3):
+        solo[i].advance(1)
+    for g, b in zip(solo, batch):
+        same_state(g, b)
+    # the failed member is repaired and rejoins
+    u[0, 5, 7, 0] = 2e-9
+    for g in (solo[2], batch[2]):
+        g.set_state(u, time=bad["time"], iteration=bad["iteration"], ist=bad["ist"], oec=bad["oec"])
+    ens.advance()
+    for g in solo:
+        g.advance(1)
+    for g, b in zip(solo, batch):
+        same_state(g, b)
+
+
+def test_ensemble_rejects_mixed_members():
+    a = members_iso()[0]
+    b = members_euler()[0]
+    with pytest.raises(Exception):
+        Ensemble([Solver(*a), Solver(*b)])
