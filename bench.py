@@ -31,6 +31,21 @@ def algorithmic_bytes_per_step(nv, rk):
     return 8 * nv * (3 + 4 * (rk - 1))
 
 
+def measured_traffic(workload, tile):
+    """HBM bytes per stage launch from the committed PMC summary
+    (FETCH_SIZE and WRITE_SIZE passes calibrated on a plain copy kernel),
+    when it covers this workload and tile; None otherwise."""
+    path = os.path.join(ROOT, "profiles", f"stage_kernel_{workload}_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload or t.get("tile") != tile:
+        return None
+    return t.get("traffic_bytes_per_launch")
+
+
 def cpu_baseline(w, u0, seconds, threads):
     from oracle import oracle as O
     O.set_threads(threads)
@@ -120,6 +135,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tile", default=None, help="TXxTY stage-kernel tile (skips the autotuner)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -142,9 +158,10 @@ def main():
         g.advance(min(fold, args.warmup - done))
         done += min(fold, args.warmup - done)
 
-    tile = (0, 0, 0.0)
-    if not args.no_autotune:
-        tile = g.autotune(reps=3)
+    if args.tile:
+        g.set_tile(*(int(v) for v in args.tile.lower().split("x")))
+    elif not args.no_autotune:
+        g.autotune(reps=3)
     g.set_timing(True)
     t0 = time.perf_counter()
     for _ in range(steps // fold):
@@ -173,8 +190,9 @@ def main():
                    "hydro": type(hydro).__name__.lower(), "rk": ph.rk_order, "fold": fold,
                    "tile": "%dx%d" % g.get_tile()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "stage_kernel", "launches": launches,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload, "%dx%d" % g.get_tile()),
+                     "kernel": "stage_kernel_batch", "launches": launches,
                      "avg_launch_ms": kernel_ms / max(launches, 1),
                      "bytes_per_launch": bytes_total / max(launches, 1)},
     }
