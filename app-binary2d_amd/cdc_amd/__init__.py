@@ -5,8 +5,9 @@ package is its Python binding and the reference-shaped configuration types.
 """
 from . import abi
 from .config import Configuration, Euler, Isothermal, Mesh, Physics
-from .solver import CdcError, Ensemble, HydroError, Solver, orbital_state_from_time, seeded_cavity
+from .solver import (CdcError, Ensemble, HydroError, Solver, orbital_state_from_time, resample_state,
+                     seeded_cavity)
 from . import workloads
 
 __all__ = ["abi", "Configuration", "Euler", "Isothermal", "Mesh", "Physics", "CdcError", "Ensemble", "HydroError",
-           "Solver", "orbital_state_from_time", "seeded_cavity", "workloads"]
+           "Solver", "orbital_state_from_time", "resample_state", "seeded_cavity", "workloads"]

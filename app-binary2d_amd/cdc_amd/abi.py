@@ -119,6 +119,19 @@ class CCavityModel(ctypes.Structure):
     ]
 
 
+class CClosureConfig(ctypes.Structure):
+    _fields_ = [
+        ("grid", c_int32),
+        ("max_samples", c_int32),
+        ("seed", c_uint64),
+        ("domain_radius", c_double),
+        ("feature_scale", c_double * 3),
+        ("output_scale", c_double * 3),
+        ("loss_weight", c_double * 3),
+        ("init_output_gain", c_double),
+    ]
+
+
 # name -> (restype, argtypes); every symbol the header declares
 SIGNATURES = {
     "cdc_abi_version": (c_int32, []),
@@ -134,6 +147,10 @@ SIGNATURES = {
     "cdc_solver_get_state": (c_int32, [c_void_p, POINTER(c_double), POINTER(c_int64), POINTER(c_int64),
                                         c_void_p, c_void_p, c_void_p]),
     "cdc_solver_advance": (c_int32, [c_void_p, c_int32, POINTER(c_double), POINTER(CError)]),
+    "cdc_solver_advance_dt": (c_int32, [c_void_p, c_int32, c_double, POINTER(CError)]),
+    "cdc_solver_resample_from": (c_int32, [c_void_p, c_void_p]),
+    "cdc_resample_state": (c_int32, [POINTER(CMesh), POINTER(CMesh), c_int32, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_void_p]),
     "cdc_solver_max_signal_speed": (c_int32, [c_void_p, POINTER(c_double)]),
     "cdc_solver_set_timing": (c_int32, [c_void_p, c_int32]),
     "cdc_solver_get_timing": (c_int32, [c_void_p, POINTER(c_int64), POINTER(c_double), POINTER(c_double)]),
@@ -146,6 +163,21 @@ SIGNATURES = {
     "cdc_ensemble_set_graph": (c_int32, [c_void_p, c_int32]),
     "cdc_ensemble_get_tile": (c_int32, [c_void_p, POINTER(c_int32), POINTER(c_int32)]),
     "cdc_ensemble_get_stats": (c_int32, [c_void_p, POINTER(c_int64), POINTER(c_double)]),
+    "cdc_closure_create": (c_int32, [POINTER(CClosureConfig), POINTER(c_void_p)]),
+    "cdc_closure_destroy": (None, [c_void_p]),
+    "cdc_closure_num_params": (c_int64, []),
+    "cdc_closure_get_params": (c_int32, [c_void_p, c_void_p]),
+    "cdc_closure_set_params": (c_int32, [c_void_p, c_void_p]),
+    "cdc_closure_get_config": (c_int32, [c_void_p, POINTER(CClosureConfig)]),
+    "cdc_closure_set_scales": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "cdc_closure_features": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "cdc_closure_forward": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "cdc_closure_add_sample": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_double]),
+    "cdc_closure_num_samples": (c_int32, [c_void_p, POINTER(c_int32), c_void_p]),
+    "cdc_closure_clear_samples": (c_int32, [c_void_p]),
+    "cdc_closure_sample_grad": (c_int32, [c_void_p, c_int32, POINTER(c_double), c_void_p]),
+    "cdc_closure_train": (c_int32, [c_void_p, c_int32, c_double, c_uint64, c_void_p]),
+    "cdc_closure_advance": (c_int32, [c_void_p, c_void_p, POINTER(c_double), POINTER(CError)]),
     "cdc_orbital_state_from_time": (c_int32, [POINTER(CPhysics), c_double, POINTER(CPointMass)]),
     "cdc_model_seeded_cavity": (c_int32, [POINTER(CMesh), POINTER(CHydro), POINTER(CCavityModel), c_void_p]),
 }
@@ -174,7 +206,7 @@ def load(path=None):
 
 def check_layout(lib=None):
     lib = lib or load()
-    sizes = {0: CMesh, 1: CHydro, 2: CPhysics, 3: CError, 4: CCavityModel}
+    sizes = {0: CMesh, 1: CHydro, 2: CPhysics, 3: CError, 4: CCavityModel, 5: CClosureConfig}
     for k, cls in sizes.items():
         got = lib.cdc_struct_size(k)
         if got != ctypes.sizeof(cls):

@@ -51,6 +51,24 @@ def orbital_state_from_time(physics, t):
     return np.array([[p.mass, p.position_x, p.position_y, p.velocity_x, p.velocity_y] for p in out])
 
 
+def resample_state(src_mesh, dst_mesh, conserved, ist=None, oec=None):
+    """Re-grid a boundary-layout state between meshes of one domain
+    (cdc_resample_state).  Returns (conserved, ist, oec) on dst_mesh."""
+    lib = abi.load()
+    nv = conserved.shape[-1]
+    u = np.ascontiguousarray(conserved, dtype=np.float64)
+    nbd, bsd = dst_mesh.num_blocks, dst_mesh.block_size
+    out = np.empty((nbd * nbd, bsd, bsd, nv))
+    oi = np.empty((nbd * nbd, 7, nv))
+    oe = np.empty((nbd * nbd, 7, 4))
+    i = None if ist is None else np.ascontiguousarray(ist, dtype=np.float64)
+    e = None if oec is None else np.ascontiguousarray(oec, dtype=np.float64)
+    ms, md = src_mesh.to_c(), dst_mesh.to_c()
+    _check(lib, lib.cdc_resample_state(byref(ms), byref(md), nv, _ptr(u), None if i is None else _ptr(i),
+                                       None if e is None else _ptr(e), _ptr(out), _ptr(oi), _ptr(oe)))
+    return out, oi, oe
+
+
 def seeded_cavity(mesh, hydro, cavity_radius=2.5, cavity_exponent=12.0, density_floor=1e-4,
                   noise_amplitude=1e-3, softening_length=0.05, mach_number=10.0, seed=1,
                   taper_radius=0.0, taper_width=0.25):
@@ -139,6 +157,18 @@ class Solver:
             raise HydroError(err)
         _check(self.lib, rc)
         return dt.value
+
+    def advance_dt(self, dt, fold=1):
+        """`fold` RK steps at the given dt (no CFL); raises HydroError."""
+        err = abi.CError()
+        rc = self.lib.cdc_solver_advance_dt(self.handle, int(fold), float(dt), byref(err))
+        if rc in (abi.CDC_ERR_NEGATIVE_DENSITY, abi.CDC_ERR_NEGATIVE_PRESSURE, abi.CDC_ERR_UNBOUND_ORBIT):
+            raise HydroError(err)
+        _check(self.lib, rc)
+
+    def resample_from(self, src):
+        """This solver's committed state := src's, re-gridded on the device."""
+        _check(self.lib, self.lib.cdc_solver_resample_from(self.handle, src.handle))
 
     def max_signal_speed(self):
         a = c_double()

@@ -1,0 +1,835 @@
+// cdc_closure.hip -- learned flux closure for coarse runs: a small periodic
+// 3x3 convolutional net (4 -> 32 -> 32 -> 32 -> 6 channels, ReLU) evaluated
+// and trained on gfx950 matrix cores (bf16 operands, fp32 accumulation),
+// whose 6 output channels are corrections to the x- and y-face fluxes of
+// (Sigma, Sx, Sy).  No counterpart in the reference.
+//
+// A corrected coarse step: features of the state u_n at the step's start ->
+// net -> Y; the plain RK step u_n -> u* (cdc_solver_advance); then
+//   u_v = u*_v + dt r_v,  r_v = -s_v ((Y_v[i,j] - Y_v[i-1,j]) / dx
+//                                   + (Y_{3+v}[i,j] - Y_{3+v}[i,j-1]) / dy)
+// for v = Sigma, Sx, Sy: a flux divergence, so mass and momentum are
+// conserved.  Training fits r to (u_fine - u_plain) / dt on pairs made from a
+// down-sampled fine run.
+//
+// Layouts: activations [pixel][32] bf16 (channel fastest, 64 B per pixel),
+// pixel = frame * n^2 + i * n + j; the net's 6 outputs [pixel][8] fp32.
+// Every layer is stored padded to 32 x 32 channels (zeros for the first
+// layer's inputs 4..31 and the last layer's outputs 6..31, which stay zero
+// under training because their gradients vanish).  Master weights, gradients
+// and Adam moments are fp32 [layer][co][ci][tap] + [layer][co] (tap = 3 ky + kx);
+// the matrix-core operands are bf16 copies [layer][n][k], k = tap * 32 + c:
+// forward n = co, c = ci; input gradient n = ci, c = co with the tap flipped.
+//
+// MFMA 32x32x8 bf16 (v_mfma_f32_32x32x8_bf16): lane l supplies row l % 32 of
+// A and column l % 32 of B over k = 4 (l / 32) .. + 3, and holds D rows
+// 8 (r / 4) + 4 (l / 32) + r % 4, r = 0..15, of column l % 32.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/cdc_solver.h"
+#include "cdc_host.h"
+#include "cdc_internal.h"
+
+using cdc::host::set_msg;
+
+namespace cdc {
+namespace nn {
+
+constexpr int C = 32;          // padded channels
+constexpr int K = 9 * C;       // taps x channels
+constexpr int NL = 4;          // conv layers
+constexpr int LW = C * C * 9;  // padded weights per layer
+constexpr int LP = LW + C;     // + bias
+constexpr int NPAR = NL * LP;
+constexpr int YS = 8;          // output stride (6 used)
+constexpr int CIN[NL] = {4, 32, 32, 32};
+constexpr int COUT[NL] = {32, 32, 32, 6};
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__host__ __device__ inline uint16_t f2bf(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__host__ __device__ inline float bf2f(uint16_t h) {
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+__device__ inline int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+struct ConvArgs {
+    const uint16_t *x;      // [P][32]
+    const uint16_t *w;      // [32][K] bf16
+    const float *bias;      // [32] or null
+    const uint16_t *mask;   // [P][32]: output zeroed where mask <= 0 (or null)
+    uint16_t *out_bf;       // [P][32] (or null)
+    float *out_f;           // [P][YS] first out_ch channels (when out_bf is null)
+    int n, P, relu, out_ch;
+};
+
+// One wave: 32 pixels x 32 output channels; four waves per workgroup.
+__global__ void __launch_bounds__(256) conv3x3_kernel(const ConvArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int p0 = (blockIdx.x * 4 + wave) * 32;
+    if (p0 >= a.P) return;
+    const int m = lane & 31, half = lane >> 5;
+    const int nn = a.n * a.n;
+    const int p = p0 + m;
+    const int f = p / nn, rr = p - f * nn;
+    const int i = rr / a.n, j = rr - (rr / a.n) * a.n;
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    const uint16_t *wrow = a.w + (size_t)m * K + half * 4;
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+        const int ii = wrap(i + tap / 3 - 1, a.n), jj = wrap(j + tap % 3 - 1, a.n);
+        const uint16_t *xr = a.x + (size_t)(f * nn + ii * a.n + jj) * C + half * 4;
+#pragma unroll
+        for (int c = 0; c < C / 8; ++c) {
+            const bf16x4 av = *(const bf16x4 *)(xr + c * 8);
+            const bf16x4 bv = *(const bf16x4 *)(wrow + tap * C + c * 8);
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+        }
+    }
+    const int ch = lane & 31;
+    const float b = a.bias ? a.bias[ch] : 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int pr = p0 + 8 * (r >> 2) + 4 * half + (r & 3);
+        float v = acc[r] + b;
+        if (a.relu) v = fmaxf(v, 0.0f);
+        if (a.mask && !(bf2f(a.mask[(size_t)pr * C + ch]) > 0.0f)) v = 0.0f;
+        if (a.out_bf) a.out_bf[(size_t)pr * C + ch] = f2bf(v);
+        else if (ch < a.out_ch) a.out_f[(size_t)pr * YS + ch] = v;
+    }
+}
+
+struct WgradArgs {
+    const uint16_t *g;   // [P][32] output gradient
+    const uint16_t *x;   // [P][32] layer input
+    float *part_w;       // [nwg][9][32][32]
+    float *part_b;       // [nwg][32]
+    int n, P, chunk;     // chunk: pixels per workgroup, a multiple of 8
+};
+
+// dW[co][ci][tap] = sum_p g[p][co] x[p + off(tap)][ci] over one chunk of
+// pixels, one wave per tap (A = g^T, B = shifted x, K = pixels); wave 0 also
+// sums g for the bias.
+__global__ void __launch_bounds__(576) wgrad_kernel(const WgradArgs a) {
+    const int lane = threadIdx.x & 63, tap = threadIdx.x >> 6;
+    const int half = lane >> 5, col = lane & 31;
+    const int nn = a.n * a.n;
+    const int start = blockIdx.x * a.chunk;
+    const int end = min(a.P, start + a.chunk);
+    const int di = tap / 3 - 1, dj = tap % 3 - 1;
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    float bsum = 0.0f;
+    for (int p8 = start; p8 < end; p8 += 8) {
+        bf16x4 av, bv;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int p = p8 + half * 4 + q;
+            const int f = p / nn, rr = p - f * nn;
+            const int i = rr / a.n, j = rr - (rr / a.n) * a.n;
+            const int src = f * nn + wrap(i + di, a.n) * a.n + wrap(j + dj, a.n);
+            const uint16_t gv = a.g[(size_t)p * C + col];
+            av[q] = (short)gv;
+            bv[q] = (short)a.x[(size_t)src * C + col];
+            if (tap == 0) bsum += bf2f(gv);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+    float *pw = a.part_w + ((size_t)blockIdx.x * 9 + tap) * C * C;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int co = 8 * (r >> 2) + 4 * half + (r & 3);
+        pw[co * C + col] = acc[r];
+    }
+    if (tap == 0) {
+        const float other = __shfl_down(bsum, 32, 64);
+        if (half == 0) a.part_b[(size_t)blockIdx.x * C + col] = bsum + other;
+    }
+}
+
+// grad[layer] = sum over workgroups of the partials, in workgroup order.
+__global__ void __launch_bounds__(256) wgrad_reduce_kernel(const float *part_w, const float *part_b, int nwg,
+                                                           float *grad_layer) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < LW) {
+        // canonical index t = (co * 32 + ci) * 9 + tap
+        const int tap = t % 9, cc = t / 9, ci = cc % C, co = cc / C;
+        float s = 0.0f;
+        for (int w = 0; w < nwg; ++w) s += part_w[(((size_t)w * 9 + tap) * C + co) * C + ci];
+        grad_layer[t] = s;
+    } else if (t < LP) {
+        const int co = t - LW;
+        float s = 0.0f;
+        for (int w = 0; w < nwg; ++w) s += part_b[(size_t)w * C + co];
+        grad_layer[t] = s;
+    }
+}
+
+struct LossArgs {
+    const float *y;        // [P][YS]
+    const float *t;        // [P][4] target
+    uint16_t *gy;          // [P][32] dL/dY
+    float *part;           // [nblocks] per-block loss sums
+    int n, P;
+    float idx, idy, inv_p;
+    float s[3], w[3];
+};
+
+__device__ inline float corr_r(const LossArgs &a, int f, int i, int j, int v) {
+    const int nn = a.n * a.n;
+    const size_t c = (size_t)(f * nn + i * a.n + j);
+    const size_t xm = (size_t)(f * nn + wrap(i - 1, a.n) * a.n + j);
+    const size_t ym = (size_t)(f * nn + i * a.n + wrap(j - 1, a.n));
+    return -a.s[v] * ((a.y[c * YS + v] - a.y[xm * YS + v]) * a.idx + (a.y[c * YS + 3 + v] - a.y[ym * YS + 3 + v]) * a.idy);
+}
+
+__global__ void __launch_bounds__(256) loss_kernel(const LossArgs a) {
+    __shared__ float red[256];
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    float l = 0.0f;
+    if (p < a.P) {
+        const int nn = a.n * a.n;
+        const int f = p / nn, rr = p - f * nn;
+        const int i = rr / a.n, j = rr - (rr / a.n) * a.n;
+        const int ip = wrap(i + 1, a.n), jp = wrap(j + 1, a.n);
+        uint16_t *g = a.gy + (size_t)p * C;
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            const float e0 = corr_r(a, f, i, j, v) - a.t[(size_t)p * 4 + v];
+            const float ex = corr_r(a, f, ip, j, v) - a.t[(size_t)(f * nn + ip * a.n + j) * 4 + v];
+            const float ey = corr_r(a, f, i, jp, v) - a.t[(size_t)(f * nn + i * a.n + jp) * 4 + v];
+            l += a.w[v] * e0 * e0;
+            const float k = 2.0f * a.inv_p * a.w[v] * a.s[v];
+            g[v] = f2bf(-k * a.idx * (e0 - ex));
+            g[3 + v] = f2bf(-k * a.idy * (e0 - ey));
+        }
+        for (int c = 6; c < C; ++c) g[c] = 0;
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.part[blockIdx.x] = red[0];
+}
+
+// loss of step `step` = (sum of block partials in order) / P
+__global__ void loss_sum_kernel(const float *part, int nb, float inv_p, float *hist, int step) {
+    if (threadIdx.x != 0) return;
+    float s = 0.0f;
+    for (int b = 0; b < nb; ++b) s += part[b];
+    hist[step] = s * inv_p;
+}
+
+struct AdamArgs {
+    float *p, *m, *v;
+    const float *g;
+    uint16_t *wf, *wb;   // bf16 operand copies [NL][32][K]
+    float lr, b1, b2, eps, c1, c2;   // c1 = 1 / (1 - b1^t), c2 = 1 / (1 - b2^t)
+    int update;          // 0: only refresh the bf16 copies
+};
+
+__global__ void __launch_bounds__(256) adam_kernel(const AdamArgs a) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= NPAR) return;
+    float p = a.p[t];
+    if (a.update) {
+        const float g = a.g[t];
+        const float m = a.b1 * a.m[t] + (1.0f - a.b1) * g;
+        const float v = a.b2 * a.v[t] + (1.0f - a.b2) * g * g;
+        a.m[t] = m;
+        a.v[t] = v;
+        p -= a.lr * (m * a.c1) / (sqrtf(v * a.c2) + a.eps);
+        a.p[t] = p;
+    }
+    const int l = t / LP, o = t - l * LP;
+    if (o < LW) {
+        const int tap = o % 9, cc = o / 9, ci = cc % C, co = cc / C;
+        const uint16_t h = f2bf(p);
+        a.wf[((size_t)l * C + co) * K + tap * C + ci] = h;
+        a.wb[((size_t)l * C + ci) * K + (8 - tap) * C + co] = h;
+    }
+}
+
+struct FeatArgs {
+    const double *u;     // SoA [nv][N][N]
+    uint16_t *x;         // [N*N][32]
+    int N, nv;
+    double gm1, a0, a1, a2;
+};
+
+__global__ void __launch_bounds__(256) features_kernel(const FeatArgs a) {
+    const int64_t NN = (int64_t)a.N * a.N;
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= NN) return;
+    const double sig = a.u[p], sx = a.u[NN + p], sy = a.u[2 * NN + p];
+    const double vx = sx / sig, vy = sy / sig;
+    float f[4];
+    f[0] = (float)(a.a0 * log(fmax(sig, 1e-30)));
+    f[1] = (float)(a.a1 * vx);
+    f[2] = (float)(a.a1 * vy);
+    f[3] = 0.0f;
+    if (a.nv == 4) {
+        const double pr = (a.u[3 * NN + p] - 0.5 * sig * (vx * vx + vy * vy)) * a.gm1;
+        f[3] = (float)(a.a2 * log(fmax(pr / sig, 1e-30)));
+    }
+    uint16_t *o = a.x + p * C;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) o[c] = f2bf(f[c]);
+    for (int c = 4; c < C; ++c) o[c] = 0;
+}
+
+struct TargetArgs {
+    const double *plain, *truth;   // SoA [nv][N][N]
+    float *t;                      // [N*N][4]
+    float *part;                   // [nblocks][3] sums of t_v^2
+    int N;
+    double inv_dt;
+};
+
+__global__ void __launch_bounds__(256) target_kernel(const TargetArgs a) {
+    __shared__ float red[3][256];
+    const int64_t NN = (int64_t)a.N * a.N;
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    float sq[3] = {0.0f, 0.0f, 0.0f};
+    if (p < NN) {
+        for (int v = 0; v < 3; ++v) {
+            const float d = (float)((a.truth[v * NN + p] - a.plain[v * NN + p]) * a.inv_dt);
+            a.t[p * 4 + v] = d;
+            sq[v] = d * d;
+        }
+        a.t[p * 4 + 3] = 0.0f;
+    }
+    for (int v = 0; v < 3; ++v) red[v][threadIdx.x] = sq[v];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int v = 0; v < 3; ++v) red[v][threadIdx.x] += red[v][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) a.part[blockIdx.x * 3 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+struct ApplyArgs {
+    double *u;            // SoA [nv][N][N], corrected in place
+    const float *y;       // [N*N][YS]
+    int N;
+    double dt, idx, idy;
+    double s[3];
+};
+
+__global__ void __launch_bounds__(256) apply_kernel(const ApplyArgs a) {
+    const int64_t NN = (int64_t)a.N * a.N;
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= NN) return;
+    const int i = (int)(p / a.N), j = (int)(p - (p / a.N) * a.N);
+    const int64_t xm = (int64_t)wrap(i - 1, a.N) * a.N + j, ym = (int64_t)i * a.N + wrap(j - 1, a.N);
+    for (int v = 0; v < 3; ++v) {
+        const double fx = (double)a.y[p * YS + v] - (double)a.y[xm * YS + v];
+        const double fy = (double)a.y[p * YS + 3 + v] - (double)a.y[ym * YS + 3 + v];
+        const double r = -a.s[v] * (fx * a.idx + fy * a.idy);
+        a.u[v * NN + p] += a.dt * r;
+    }
+}
+
+}  // namespace nn
+}  // namespace cdc
+
+using namespace cdc::nn;
+
+struct cdc_closure {
+    cdc_closure_config_t cfg;
+    int n = 0, P = 0;
+    hipStream_t stream = nullptr;
+    float *d_par = nullptr, *d_grad = nullptr, *d_m = nullptr, *d_v = nullptr;
+    uint16_t *d_wf = nullptr, *d_wb = nullptr;
+    uint16_t *d_act[4] = {};      // X0, A1, A2, A3
+    uint16_t *d_g[2] = {};        // gradient ping-pong [P][32]
+    float *d_y = nullptr;         // [P][YS]
+    float *d_part_w = nullptr, *d_part_b = nullptr, *d_part_l = nullptr;
+    int nwg = 0, chunk = 0, nlb = 0;
+    // dataset
+    int cap = 0, count = 0;
+    uint16_t *d_feat = nullptr;   // [cap][P][32]
+    float *d_tgt = nullptr;       // [cap][P][4]
+    double tsq[3] = {0, 0, 0};    // sum over samples and cells of target^2
+    float *d_hist = nullptr;
+    int hist_cap = 0;
+    int64_t adam_step = 0;
+};
+
+namespace {
+
+void free_closure(cdc_closure *c) {
+    float *fl[] = {c->d_par, c->d_grad, c->d_m, c->d_v, c->d_y, c->d_part_w, c->d_part_b, c->d_part_l, c->d_tgt,
+                   c->d_hist};
+    for (float *p : fl) (void)hipFree(p);
+    uint16_t *hl[] = {c->d_wf, c->d_wb, c->d_act[0], c->d_act[1], c->d_act[2], c->d_act[3], c->d_g[0], c->d_g[1],
+                      c->d_feat};
+    for (uint16_t *p : hl) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+int refresh_weights(cdc_closure *c, hipStream_t s) {
+    AdamArgs a{};
+    a.p = c->d_par;
+    a.m = c->d_m;
+    a.v = c->d_v;
+    a.g = c->d_grad;
+    a.wf = c->d_wf;
+    a.wb = c->d_wb;
+    a.update = 0;
+    hipLaunchKernelGGL(adam_kernel, dim3((NPAR + 255) / 256), dim3(256), 0, s, a);
+    HIP_TRY(hipGetLastError());
+    return CDC_OK;
+}
+
+int conv(cdc_closure *c, const uint16_t *x, int layer, bool backward, const uint16_t *mask, uint16_t *out_bf,
+         float *out_f, hipStream_t s) {
+    ConvArgs a{};
+    a.x = x;
+    a.w = (backward ? c->d_wb : c->d_wf) + (size_t)layer * C * K;
+    a.bias = backward ? nullptr : c->d_par + (size_t)layer * LP + LW;
+    a.mask = mask;
+    a.out_bf = out_bf;
+    a.out_f = out_f;
+    a.n = c->n;
+    a.P = c->P;
+    a.relu = (!backward && layer < NL - 1) ? 1 : 0;
+    a.out_ch = 6;
+    hipLaunchKernelGGL(conv3x3_kernel, dim3((c->P + 127) / 128), dim3(256), 0, s, a);
+    HIP_TRY(hipGetLastError());
+    return CDC_OK;
+}
+
+int forward(cdc_closure *c, hipStream_t s) {
+    int rc;
+    for (int l = 0; l < NL - 1; ++l)
+        if ((rc = conv(c, c->d_act[l], l, false, nullptr, c->d_act[l + 1], nullptr, s))) return rc;
+    return conv(c, c->d_act[NL - 1], NL - 1, false, nullptr, nullptr, c->d_y, s);
+}
+
+int wgrad(cdc_closure *c, const uint16_t *g, const uint16_t *x, int layer, hipStream_t s) {
+    WgradArgs a{};
+    a.g = g;
+    a.x = x;
+    a.part_w = c->d_part_w;
+    a.part_b = c->d_part_b;
+    a.n = c->n;
+    a.P = c->P;
+    a.chunk = c->chunk;
+    hipLaunchKernelGGL(wgrad_kernel, dim3(c->nwg), dim3(576), 0, s, a);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((LP + 255) / 256), dim3(256), 0, s, (const float *)c->d_part_w,
+                       (const float *)c->d_part_b, c->nwg, c->d_grad + (size_t)layer * LP);
+    HIP_TRY(hipGetLastError());
+    return CDC_OK;
+}
+
+// loss of the current forward pass against target t; dY into d_g[0]; the
+// step's loss into d_hist[step]
+int loss(cdc_closure *c, const float *t, int step, hipStream_t s) {
+    LossArgs a{};
+    a.y = c->d_y;
+    a.t = t;
+    a.gy = c->d_g[0];
+    a.part = c->d_part_l;
+    a.n = c->n;
+    a.P = c->P;
+    const double dx = 2.0 * c->cfg.domain_radius / c->n;
+    a.idx = (float)(1.0 / dx);
+    a.idy = (float)(1.0 / dx);
+    a.inv_p = 1.0f / (float)c->P;
+    for (int v = 0; v < 3; ++v) {
+        a.s[v] = (float)c->cfg.output_scale[v];
+        a.w[v] = (float)c->cfg.loss_weight[v];
+    }
+    hipLaunchKernelGGL(loss_kernel, dim3(c->nlb), dim3(256), 0, s, a);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(loss_sum_kernel, dim3(1), dim3(64), 0, s, (const float *)c->d_part_l, c->nlb, a.inv_p,
+                       c->d_hist, step);
+    HIP_TRY(hipGetLastError());
+    return CDC_OK;
+}
+
+// gradient of the loss of sample k into d_grad (forward included)
+int backprop(cdc_closure *c, int k, int step, hipStream_t s) {
+    HIP_TRY(hipMemcpyAsync(c->d_act[0], c->d_feat + (size_t)k * c->P * C, (size_t)c->P * C * sizeof(uint16_t),
+                           hipMemcpyDeviceToDevice, s));
+    int rc;
+    if ((rc = forward(c, s))) return rc;
+    if ((rc = loss(c, c->d_tgt + (size_t)k * c->P * 4, step, s))) return rc;
+    int cur = 0;
+    for (int l = NL - 1; l >= 0; --l) {
+        if ((rc = wgrad(c, c->d_g[cur], c->d_act[l], l, s))) return rc;
+        if (l > 0) {
+            // dA_l = conv^T(g, W_l) masked by A_l > 0
+            if ((rc = conv(c, c->d_g[cur], l, true, c->d_act[l], c->d_g[1 - cur], nullptr, s))) return rc;
+            cur = 1 - cur;
+        }
+    }
+    return CDC_OK;
+}
+
+int adam_update(cdc_closure *c, double lr, hipStream_t s) {
+    c->adam_step += 1;
+    AdamArgs a{};
+    a.p = c->d_par;
+    a.m = c->d_m;
+    a.v = c->d_v;
+    a.g = c->d_grad;
+    a.wf = c->d_wf;
+    a.wb = c->d_wb;
+    a.lr = (float)lr;
+    a.b1 = 0.9f;
+    a.b2 = 0.999f;
+    a.eps = 1e-8f;
+    a.c1 = (float)(1.0 / (1.0 - pow(0.9, (double)c->adam_step)));
+    a.c2 = (float)(1.0 / (1.0 - pow(0.999, (double)c->adam_step)));
+    a.update = 1;
+    hipLaunchKernelGGL(adam_kernel, dim3((NPAR + 255) / 256), dim3(256), 0, s, a);
+    HIP_TRY(hipGetLastError());
+    return CDC_OK;
+}
+
+int ensure_hist(cdc_closure *c, int n) {
+    if (n <= c->hist_cap) return CDC_OK;
+    (void)hipFree(c->d_hist);
+    c->d_hist = nullptr;
+    c->hist_cap = 0;
+    HIP_TRY(hipMalloc((void **)&c->d_hist, (size_t)n * sizeof(float)));
+    c->hist_cap = n;
+    return CDC_OK;
+}
+
+// flat canonical (logical) index <-> padded device index
+template <typename F>
+void for_each_param(F fn) {
+    int64_t k = 0;
+    for (int l = 0; l < NL; ++l) {
+        for (int co = 0; co < COUT[l]; ++co)
+            for (int ci = 0; ci < CIN[l]; ++ci)
+                for (int tap = 0; tap < 9; ++tap) fn(k++, (int64_t)l * LP + ((int64_t)co * C + ci) * 9 + tap);
+        for (int co = 0; co < COUT[l]; ++co) fn(k++, (int64_t)l * LP + LW + co);
+    }
+}
+
+int64_t logical_params() {
+    int64_t n = 0;
+    for_each_param([&](int64_t, int64_t) { ++n; });
+    return n;
+}
+
+int features(cdc_closure *c, const cdc_solver *s, uint16_t *x, hipStream_t st) {
+    FeatArgs a{};
+    a.u = s->d_u[s->cur];
+    a.x = x;
+    a.N = s->N;
+    a.nv = s->nv;
+    a.gm1 = s->hydro.kind == CDC_EULER ? s->hydro.gamma_law_index - 1.0 : 0.0;
+    a.a0 = c->cfg.feature_scale[0];
+    a.a1 = c->cfg.feature_scale[1];
+    a.a2 = c->cfg.feature_scale[2];
+    hipLaunchKernelGGL(features_kernel, dim3((unsigned)(((int64_t)s->N * s->N + 255) / 256)), dim3(256), 0, st, a);
+    HIP_TRY(hipGetLastError());
+    return CDC_OK;
+}
+
+int check_solver(const cdc_closure *c, const cdc_solver *s) {
+    if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
+    if (s->N != c->n) return set_msg(CDC_ERR_INVALID_ARGUMENT, "solver grid size differs from the closure's");
+    if (s->mesh.domain_radius != c->cfg.domain_radius)
+        return set_msg(CDC_ERR_INVALID_ARGUMENT, "solver domain differs from the closure's");
+    return CDC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t cdc_closure_create(const cdc_closure_config_t *cfg, cdc_closure_t **out) {
+    if (!out) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    if (!cfg) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null config");
+    if (cfg->grid < 8 || cfg->grid % 8 != 0 || cfg->grid > 4096)
+        return set_msg(CDC_ERR_INVALID_ARGUMENT, "grid must be a multiple of 8 in [8, 4096]");
+    if (cfg->max_samples < 0) return set_msg(CDC_ERR_INVALID_ARGUMENT, "max_samples must be >= 0");
+    if (!(cfg->domain_radius > 0.0)) return set_msg(CDC_ERR_INVALID_ARGUMENT, "domain_radius must be > 0");
+    if (cdc_device_count() < 1) return set_msg(CDC_ERR_NO_DEVICE, "no HIP device visible");
+    cdc_closure *c = new cdc_closure();
+    c->cfg = *cfg;
+    c->n = cfg->grid;
+    c->P = cfg->grid * cfg->grid;
+    c->chunk = std::max(8, ((c->P / 256) + 7) / 8 * 8);
+    c->nwg = (c->P + c->chunk - 1) / c->chunk;
+    c->nlb = (c->P + 255) / 256;
+    c->cap = cfg->max_samples;
+    auto fail = [&](int code, const char *m) {
+        free_closure(c);
+        delete c;
+        return set_msg(code, m);
+    };
+    bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+    const size_t act = (size_t)c->P * C * sizeof(uint16_t);
+    ok = ok && hipMalloc((void **)&c->d_par, NPAR * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->d_grad, NPAR * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->d_m, NPAR * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->d_v, NPAR * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->d_wf, (size_t)NL * C * K * sizeof(uint16_t)) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->d_wb, (size_t)NL * C * K * sizeof(uint16_t)) == hipSuccess;
+    for (int i = 0; i < 4 && ok; ++i) ok = hipMalloc((void **)&c->d_act[i], act) == hipSuccess;
+    for (int i = 0; i < 2 && ok; ++i) ok = hipMalloc((void **)&c->d_g[i], act) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->d_y, (size_t)c->P * YS * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->d_part_w, (size_t)c->nwg * 9 * C * C * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->d_part_b, (size_t)c->nwg * C * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->d_part_l, (size_t)c->nlb * 3 * sizeof(float)) == hipSuccess;
+    if (ok && c->cap > 0) {
+        ok = hipMalloc((void **)&c->d_feat, (size_t)c->cap * act) == hipSuccess &&
+             hipMalloc((void **)&c->d_tgt, (size_t)c->cap * c->P * 4 * sizeof(float)) == hipSuccess;
+    }
+    if (!ok) {
+        (void)hipGetLastError();
+        return fail(CDC_ERR_OUT_OF_MEMORY, "closure buffers");
+    }
+    // He-normal initial weights from the seed (zero biases, zero padding)
+    std::vector<float> h(NPAR, 0.0f);
+    std::mt19937_64 rng(cfg->seed);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    for_each_param([&](int64_t, int64_t d) {
+        const int l = (int)(d / LP), o = (int)(d - (int64_t)l * LP);
+        if (o < LW) h[(size_t)d] = (float)(nd(rng) * sqrt(2.0 / (9.0 * CIN[l])) * (l == NL - 1 ? cfg->init_output_gain : 1.0));
+    });
+    std::vector<float> z(NPAR, 0.0f);
+    ok = hipMemcpy(c->d_par, h.data(), NPAR * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(c->d_m, z.data(), NPAR * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(c->d_v, z.data(), NPAR * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(c->d_grad, z.data(), NPAR * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) return fail(CDC_ERR_HIP, "closure parameter upload");
+    if (refresh_weights(c, c->stream) || hipStreamSynchronize(c->stream) != hipSuccess)
+        return fail(CDC_ERR_HIP, "closure weight conversion");
+    *out = c;
+    return CDC_OK;
+}
+
+void cdc_closure_destroy(cdc_closure_t *c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_closure(c);
+    delete c;
+}
+
+int64_t cdc_closure_num_params(void) { return logical_params(); }
+
+int32_t cdc_closure_get_params(cdc_closure_t *c, float *out) {
+    if (!c || !out) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null pointer");
+    std::vector<float> h(NPAR);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(h.data(), c->d_par, NPAR * sizeof(float), hipMemcpyDeviceToHost));
+    for_each_param([&](int64_t k, int64_t d) { out[k] = h[(size_t)d]; });
+    return CDC_OK;
+}
+
+int32_t cdc_closure_set_params(cdc_closure_t *c, const float *in) {
+    if (!c || !in) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null pointer");
+    std::vector<float> h(NPAR, 0.0f);
+    for_each_param([&](int64_t k, int64_t d) { h[(size_t)d] = in[k]; });
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(c->d_par, h.data(), NPAR * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<float> z(NPAR, 0.0f);
+    HIP_TRY(hipMemcpy(c->d_m, z.data(), NPAR * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_v, z.data(), NPAR * sizeof(float), hipMemcpyHostToDevice));
+    c->adam_step = 0;
+    int rc = refresh_weights(c, c->stream);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CDC_OK;
+}
+
+int32_t cdc_closure_get_config(cdc_closure_t *c, cdc_closure_config_t *out) {
+    if (!c || !out) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null pointer");
+    *out = c->cfg;
+    return CDC_OK;
+}
+
+int32_t cdc_closure_set_scales(cdc_closure_t *c, const double *output_scale, const double *loss_weight) {
+    if (!c) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null closure");
+    for (int v = 0; v < 3; ++v) {
+        if (output_scale) c->cfg.output_scale[v] = output_scale[v];
+        if (loss_weight) c->cfg.loss_weight[v] = loss_weight[v];
+    }
+    return CDC_OK;
+}
+
+int32_t cdc_closure_forward(cdc_closure_t *c, const cdc_solver_t *s, float *y_out) {
+    if (!c) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null closure");
+    int rc = check_solver(c, s);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if ((rc = features(c, s, c->d_act[0], c->stream))) return rc;
+    if ((rc = forward(c, c->stream))) return rc;
+    if (y_out) {
+        std::vector<float> h((size_t)c->P * YS);
+        HIP_TRY(hipMemcpyAsync(h.data(), c->d_y, h.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (int p = 0; p < c->P; ++p)
+            for (int k = 0; k < 6; ++k) y_out[(size_t)p * 6 + k] = h[(size_t)p * YS + k];
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CDC_OK;
+}
+
+int32_t cdc_closure_features(cdc_closure_t *c, const cdc_solver_t *s, float *x_out) {
+    if (!c || !x_out) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null pointer");
+    int rc = check_solver(c, s);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if ((rc = features(c, s, c->d_act[0], c->stream))) return rc;
+    std::vector<uint16_t> h((size_t)c->P * C);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->d_act[0], h.size() * sizeof(uint16_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int p = 0; p < c->P; ++p)
+        for (int k = 0; k < 4; ++k) x_out[(size_t)p * 4 + k] = bf2f(h[(size_t)p * C + k]);
+    return CDC_OK;
+}
+
+int32_t cdc_closure_add_sample(cdc_closure_t *c, const cdc_solver_t *base, const cdc_solver_t *plain,
+                               const cdc_solver_t *truth, double dt) {
+    if (!c) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null closure");
+    int rc;
+    if ((rc = check_solver(c, base)) || (rc = check_solver(c, plain)) || (rc = check_solver(c, truth))) return rc;
+    if (c->count >= c->cap) return set_msg(CDC_ERR_INVALID_ARGUMENT, "closure dataset is full");
+    if (!(dt > 0.0)) return set_msg(CDC_ERR_INVALID_ARGUMENT, "dt must be > 0");
+    HIP_TRY(hipStreamSynchronize(base->stream));
+    HIP_TRY(hipStreamSynchronize(plain->stream));
+    HIP_TRY(hipStreamSynchronize(truth->stream));
+    const int k = c->count;
+    if ((rc = features(c, base, c->d_feat + (size_t)k * c->P * C, c->stream))) return rc;
+    TargetArgs a{};
+    a.plain = plain->d_u[plain->cur];
+    a.truth = truth->d_u[truth->cur];
+    a.t = c->d_tgt + (size_t)k * c->P * 4;
+    a.part = c->d_part_l;
+    a.N = c->n;
+    a.inv_dt = 1.0 / dt;
+    hipLaunchKernelGGL(target_kernel, dim3(c->nlb), dim3(256), 0, c->stream, a);
+    HIP_TRY(hipGetLastError());
+    std::vector<float> part((size_t)c->nlb * 3);
+    HIP_TRY(hipMemcpyAsync(part.data(), c->d_part_l, part.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int b = 0; b < c->nlb; ++b)
+        for (int v = 0; v < 3; ++v) c->tsq[v] += part[(size_t)b * 3 + v];
+    c->count += 1;
+    return CDC_OK;
+}
+
+int32_t cdc_closure_num_samples(cdc_closure_t *c, int32_t *count, double *target_ms) {
+    if (!c) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null closure");
+    if (count) *count = c->count;
+    if (target_ms)
+        for (int v = 0; v < 3; ++v) target_ms[v] = c->count ? c->tsq[v] / ((double)c->count * c->P) : 0.0;
+    return CDC_OK;
+}
+
+int32_t cdc_closure_clear_samples(cdc_closure_t *c) {
+    if (!c) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null closure");
+    c->count = 0;
+    for (int v = 0; v < 3; ++v) c->tsq[v] = 0.0;
+    return CDC_OK;
+}
+
+int32_t cdc_closure_sample_grad(cdc_closure_t *c, int32_t k, double *loss_out, float *grad_out) {
+    if (!c) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null closure");
+    if (k < 0 || k >= c->count) return set_msg(CDC_ERR_INVALID_ARGUMENT, "no such sample");
+    int rc = ensure_hist(c, 1);
+    if (rc) return rc;
+    if ((rc = backprop(c, k, 0, c->stream))) return rc;
+    std::vector<float> g(NPAR);
+    float l = 0.0f;
+    HIP_TRY(hipMemcpyAsync(g.data(), c->d_grad, NPAR * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&l, c->d_hist, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (loss_out) *loss_out = l;
+    if (grad_out) for_each_param([&](int64_t i, int64_t d) { grad_out[i] = g[(size_t)d]; });
+    return CDC_OK;
+}
+
+int32_t cdc_closure_train(cdc_closure_t *c, int32_t steps, double lr, uint64_t seed, float *loss_out) {
+    if (!c) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null closure");
+    if (steps < 0) return set_msg(CDC_ERR_INVALID_ARGUMENT, "steps must be >= 0");
+    if (c->count < 1) return set_msg(CDC_ERR_INVALID_ARGUMENT, "closure dataset is empty");
+    int rc = ensure_hist(c, std::max(1, steps));
+    if (rc) return rc;
+    // samples in shuffled epochs
+    std::mt19937_64 rng(seed);
+    std::vector<int> order;
+    for (int t = 0; t < steps; ++t) {
+        if (order.empty()) {
+            for (int k = 0; k < c->count; ++k) order.push_back(k);
+            std::shuffle(order.begin(), order.end(), rng);
+        }
+        const int k = order.back();
+        order.pop_back();
+        if ((rc = backprop(c, k, t, c->stream))) return rc;
+        if ((rc = adam_update(c, lr, c->stream))) return rc;
+    }
+    if (loss_out && steps > 0)
+        HIP_TRY(hipMemcpyAsync(loss_out, c->d_hist, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CDC_OK;
+}
+
+int32_t cdc_closure_advance(cdc_closure_t *c, cdc_solver_t *s, double *dt_out, cdc_error_t *err) {
+    if (err) memset(err, 0, sizeof(*err));
+    if (!c) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null closure");
+    int rc = check_solver(c, s);
+    if (rc) return rc;
+    if (s->nv < 3) return set_msg(CDC_ERR_INVALID_ARGUMENT, "unsupported hydro");
+    // net on the state at the start of the step, on the solver's stream
+    if ((rc = features(c, s, c->d_act[0], s->stream))) return rc;
+    if ((rc = forward(c, s->stream))) return rc;
+    double dt = 0.0;
+    rc = cdc_solver_advance(s, 1, &dt, err);
+    if (dt_out) *dt_out = dt;
+    if (rc) return rc;
+    ApplyArgs a{};
+    a.u = s->d_u[s->cur];
+    a.y = c->d_y;
+    a.N = s->N;
+    a.dt = dt;
+    const double dx = 2.0 * s->mesh.domain_radius / (double)s->N;
+    a.idx = 1.0 / dx;
+    a.idy = 1.0 / dx;
+    for (int v = 0; v < 3; ++v) a.s[v] = c->cfg.output_scale[v];
+    hipLaunchKernelGGL(apply_kernel, dim3((unsigned)(((int64_t)s->N * s->N + 255) / 256)), dim3(256), 0, s->stream,
+                       a);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->amax_valid = false;   // the correction changed the state the last stage measured
+    s->version += 1;
+    return CDC_OK;
+}
+
+}  // extern "C"

@@ -112,5 +112,13 @@ void commit(cdc_solver *s, const Slot &out);
 // After a failed advance (err_flag = flag): the error join_all would report.
 int resolve_error(cdc_solver *s, int flag, const StepPlan &plan, cdc_error_t *err);
 
+// Mean of r values x[0], x[stride], ... (r a power of two) by a pairwise tree.
+double tree_mean(const double *x, int64_t stride, int64_t r);
+// Per-block ist / oec of a re-gridded state: copied when num_blocks is kept,
+// summed over merged blocks when a coarse block covers whole fine blocks,
+// zero otherwise (either destination may be null).
+void resample_block_sums(const cdc_mesh_t *src, const cdc_mesh_t *dst, int nv, const double *ist_src,
+                         const double *oec_src, double *ist_dst, double *oec_dst);
+
 }  // namespace host
 }  // namespace cdc

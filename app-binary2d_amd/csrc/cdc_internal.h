@@ -98,6 +98,9 @@ hipError_t launch_max_speed(int nv, const double *u, int64_t ncell, double gm, d
 hipError_t launch_aos_to_soa(int nv, int nb, int bs, const double *aos, double *soa, hipStream_t s);
 hipError_t launch_soa_to_aos(int nv, int nb, int bs, const double *soa, double *aos, hipStream_t s);
 hipError_t launch_set_int(int32_t *p, int32_t v, hipStream_t s);
+// r in 1..16 (a power of two), down = 1 for averaging, 0 for injection
+hipError_t launch_resample(int nv, const double *src, int64_t Ns, double *dst, int64_t Nd, int r, int down,
+                           hipStream_t s);
 hipError_t launch_column_max(const double *x, int n, int stride, double *out, hipStream_t s);
 
 }  // namespace cdc

@@ -841,6 +841,35 @@ __global__ void __launch_bounds__(256) soa_to_aos_kernel(int nv, int nb, int bs,
     for (int v = 0; v < nv; ++v) aos[c * nv + v] = soa[v * NN + gi * N + gj];
 }
 
+// Re-gridding between two SoA states of one domain, r a power of two:
+// injection (up) or the mean of r x r fine cells as a pairwise tree over each
+// row, then over the row means (down), the same order as the host resampler.
+__global__ void __launch_bounds__(256) resample_kernel(int nv, const double *src, int64_t Ns, double *dst,
+                                                       int64_t Nd, int r, int down) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= Nd * Nd) return;
+    const int64_t gi = c / Nd, gj = c - (c / Nd) * Nd;
+    for (int v = 0; v < nv; ++v) {
+        const double *sp = src + v * Ns * Ns;
+        if (!down) {
+            dst[v * Nd * Nd + c] = sp[(gi / r) * Ns + gj / r];
+            continue;
+        }
+        double rows[16];
+        for (int a = 0; a < r; ++a) {
+            double t[16];
+            const double *q = sp + (gi * r + a) * Ns + gj * r;
+            for (int k = 0; k < r; ++k) t[k] = q[k];
+            for (int w = r; w > 1; w /= 2)
+                for (int k = 0; k < w / 2; ++k) t[k] = t[2 * k] + t[2 * k + 1];
+            rows[a] = t[0] / (double)r;
+        }
+        for (int w = r; w > 1; w /= 2)
+            for (int k = 0; k < w / 2; ++k) rows[k] = rows[2 * k] + rows[2 * k + 1];
+        dst[v * Nd * Nd + c] = rows[0] / (double)r;
+    }
+}
+
 __global__ void set_int_kernel(int32_t *p, int32_t v) { *p = v; }
 
 // max over a strided column (the per-tile max signal speed slot), one block
@@ -948,6 +977,15 @@ hipError_t launch_soa_to_aos(int nv, int nb, int bs, const double *soa, double *
 
 hipError_t launch_column_max(const double *x, int n, int stride, double *out, hipStream_t s) {
     hipLaunchKernelGGL(column_max_kernel, dim3(1), dim3(256), 0, s, x, n, stride, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_resample(int nv, const double *src, int64_t Ns, double *dst, int64_t Nd, int r, int down,
+                           hipStream_t s) {
+    if (r < 1 || r > 16) return hipErrorInvalidValue;
+    const int64_t n = Nd * Nd;
+    hipLaunchKernelGGL(resample_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nv, src, Ns, dst, Nd, r,
+                       down);
     return hipGetLastError();
 }
 

@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -460,6 +461,7 @@ int32_t cdc_struct_size(int32_t which) {
         case 2: return (int32_t)sizeof(cdc_physics_t);
         case 3: return (int32_t)sizeof(cdc_error_t);
         case 4: return (int32_t)sizeof(cdc_cavity_model_t);
+        case 5: return (int32_t)sizeof(cdc_closure_config_t);
         default: return -1;
     }
 }
@@ -759,19 +761,8 @@ int32_t cdc_solver_get_timing(cdc_solver_t *s, int64_t *launches, double *total_
     return CDC_OK;
 }
 
-int32_t cdc_solver_advance(cdc_solver_t *s, int32_t fold, double *dt_out, cdc_error_t *err) {
-    if (err) memset(err, 0, sizeof(*err));
-    if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
-    if (fold < 0) return set_msg(CDC_ERR_INVALID_ARGUMENT, "fold must be >= 0");
-
-    // dt = cfl * dx / max signal speed (src/scheme.rs:336-340)
-    double amax = 0.0;
-    int rc = compute_amax(s, &amax);
-    if (rc) return rc;
-    const double dt = cdc::host::step_dt(s, amax);
-    if (dt_out) *dt_out = dt;
-    if (fold == 0) return CDC_OK;
-
+static int advance_at(cdc_solver_t *s, int32_t fold, double dt, cdc_error_t *err) {
+    int rc = CDC_OK;
     StepPlan plan;
     cdc::host::plan_steps(s, fold, dt, plan);
     const size_t nst = plan.args.size();
@@ -802,6 +793,65 @@ int32_t cdc_solver_advance(cdc_solver_t *s, int32_t fold, double *dt_out, cdc_er
         return CDC_OK;
     }
     return cdc::host::resolve_error(s, flag, plan, err);
+}
+
+int32_t cdc_solver_advance(cdc_solver_t *s, int32_t fold, double *dt_out, cdc_error_t *err) {
+    if (err) memset(err, 0, sizeof(*err));
+    if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
+    if (fold < 0) return set_msg(CDC_ERR_INVALID_ARGUMENT, "fold must be >= 0");
+
+    // dt = cfl * dx / max signal speed (src/scheme.rs:336-340)
+    double amax = 0.0;
+    int rc = compute_amax(s, &amax);
+    if (rc) return rc;
+    const double dt = cdc::host::step_dt(s, amax);
+    if (dt_out) *dt_out = dt;
+    if (fold == 0) return CDC_OK;
+    return advance_at(s, fold, dt, err);
+}
+
+int32_t cdc_solver_advance_dt(cdc_solver_t *s, int32_t fold, double dt, cdc_error_t *err) {
+    if (err) memset(err, 0, sizeof(*err));
+    if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
+    if (fold < 0) return set_msg(CDC_ERR_INVALID_ARGUMENT, "fold must be >= 0");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return set_msg(CDC_ERR_INVALID_ARGUMENT, "dt must be positive and finite");
+    if (fold == 0) return CDC_OK;
+    return advance_at(s, fold, dt, err);
+}
+
+int32_t cdc_solver_resample_from(cdc_solver_t *dst, const cdc_solver_t *src) {
+    if (!dst || !src) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
+    if (dst == src) return set_msg(CDC_ERR_INVALID_ARGUMENT, "source and destination are one solver");
+    if (dst->nv != src->nv) return set_msg(CDC_ERR_INVALID_ARGUMENT, "solvers must share one hydro kind");
+    if (dst->mesh.domain_radius != src->mesh.domain_radius)
+        return set_msg(CDC_ERR_INVALID_ARGUMENT, "meshes must cover one domain");
+    if (dst->device != src->device) return set_msg(CDC_ERR_INVALID_ARGUMENT, "solvers must share one device");
+    const int64_t Ns = src->N, Nd = dst->N;
+    const bool down = Ns >= Nd;
+    const int64_t r = down ? Ns / Nd : Nd / Ns;
+    if ((down ? Ns % Nd : Nd % Ns) != 0 || (r & (r - 1)) != 0 || r > 16)
+        return set_msg(CDC_ERR_INVALID_ARGUMENT, "grid sizes must differ by a power-of-two factor of at most 16");
+    HIP_TRY(hipStreamSynchronize(src->stream));
+    // write into a free buffer of dst and commit it: a failure leaves dst as it was
+    const int out = (dst->cur + 1) % NBUF;
+    HIP_TRY(cdc::launch_resample(dst->nv, src->d_u[src->cur], Ns, dst->d_u[out], Nd, (int)r, down ? 1 : 0,
+                                 dst->stream));
+    const size_t ni = (size_t)src->nblk * 7 * src->nv, ne = (size_t)src->nblk * 7 * 4;
+    const size_t di = (size_t)dst->nblk * 7 * dst->nv, de = (size_t)dst->nblk * 7 * 4;
+    std::vector<double> is(ni), es(ne), id(di), ed(de);
+    HIP_TRY(hipMemcpy(is.data(), src->d_ist[src->cur], ni * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(es.data(), src->d_oec[src->cur], ne * sizeof(double), hipMemcpyDeviceToHost));
+    cdc::host::resample_block_sums(&src->mesh, &dst->mesh, dst->nv, is.data(), es.data(), id.data(), ed.data());
+    HIP_TRY(hipMemcpyAsync(dst->d_ist[out], id.data(), di * sizeof(double), hipMemcpyHostToDevice, dst->stream));
+    HIP_TRY(hipMemcpyAsync(dst->d_oec[out], ed.data(), de * sizeof(double), hipMemcpyHostToDevice, dst->stream));
+    HIP_TRY(hipStreamSynchronize(dst->stream));
+    dst->cur = out;
+    dst->time = src->time;
+    dst->num = src->num;
+    dst->den = src->den;
+    dst->amax_valid = false;
+    dst->version += 1;
+    return CDC_OK;
 }
 
 }  // extern "C"

@@ -158,7 +158,8 @@ typedef struct cdc_solver cdc_solver_t;
 int32_t cdc_abi_version(void);
 
 /* sizeof of each boundary struct, so that bindings can verify their layout. */
-int32_t cdc_struct_size(int32_t which);  /* 0 mesh, 1 hydro, 2 physics, 3 error */
+int32_t cdc_struct_size(int32_t which);  /* 0 mesh, 1 hydro, 2 physics, 3 error,
+                                            4 cavity model, 5 closure config */
 
 /* Number of visible HIP devices (0 when none). */
 int32_t cdc_device_count(void);
@@ -206,6 +207,26 @@ int32_t cdc_solver_get_state(cdc_solver_t *solver, double *time,
  */
 int32_t cdc_solver_advance(cdc_solver_t *solver, int32_t fold, double *dt_out,
                            cdc_error_t *err);
+
+/*
+ * The same `fold` RK steps at a caller-given dt (no CFL evaluation; the
+ * reference always derives dt, src/scheme.rs:336-340).  For building pairs of
+ * runs at one time step, e.g. a coarse step of dt against 4 fine steps of
+ * dt/4.  Errors and transactionality as cdc_solver_advance.
+ */
+int32_t cdc_solver_advance_dt(cdc_solver_t *solver, int32_t fold, double dt,
+                              cdc_error_t *err);
+
+/*
+ * Re-grid the committed state of `src` into `dst` (one domain, one hydro
+ * kind, grid sizes differing by a power-of-two factor of at most 16), on the
+ * device: injection when dst is finer (the checkpoint up-sampler,
+ * upsampler/src/main.rs:27-72), the mean of r x r cells when it is coarser
+ * (pairwise-tree sums, so down(up(u)) == u exactly).  dst takes src's time
+ * and iteration; its ist / oec follow cdc_resample_state.  dst's initial
+ * state (the buffer-zone target) is not changed.
+ */
+int32_t cdc_solver_resample_from(cdc_solver_t *dst, const cdc_solver_t *src);
 
 /* State::max_signal_speed (src/state.rs:121-133) of the current state. */
 int32_t cdc_solver_max_signal_speed(cdc_solver_t *solver, double *out);
@@ -263,7 +284,81 @@ int32_t cdc_ensemble_get_tile(cdc_ensemble_t *ensemble, int32_t *tx, int32_t *ty
 int32_t cdc_ensemble_get_stats(cdc_ensemble_t *ensemble, int64_t *member_steps,
                                double *zone_updates);
 
+/* ---- learned flux closure for coarse runs -------------------------------- */
+/*
+ * No counterpart in the reference.  A periodic 3x3 convolutional net
+ * (4 -> 32 -> 32 -> 32 -> 6 channels, ReLU; bf16 matrix-core operands, fp32
+ * accumulation) maps the state at the start of a coarse step to corrections
+ * of the x- and y-face fluxes of (Sigma, Sx, Sy).  A corrected step is the
+ * plain RK step followed by
+ *   u_v += dt r_v,  r_v = -s_v ((Y_v[i,j] - Y_v[i-1,j]) / dx + (Y_{3+v}[i,j] - Y_{3+v}[i,j-1]) / dy),
+ * a flux divergence (mass and momentum conserved).  Features: a0 ln Sigma,
+ * a1 vx, a1 vy, a2 ln(p / Sigma) (0 for isothermal).  Training fits r to
+ * (u_truth - u_plain) / dt over stored samples with Adam on the loss
+ * (1/P) sum_cells sum_v w_v (r_v - target_v)^2.
+ * Parameters in canonical order: per layer W[cout][cin][3][3] then b[cout],
+ * layers (4,32), (32,32), (32,32), (32,6) as (cin, cout).
+ */
+typedef struct cdc_closure cdc_closure_t;
+
+typedef struct cdc_closure_config {
+    int32_t  grid;               /* N: the coarse grid is N x N cells */
+    int32_t  max_samples;        /* dataset capacity (0: inference only) */
+    uint64_t seed;               /* initial weights */
+    double   domain_radius;      /* the coarse mesh's (dx = 2R / N) */
+    double   feature_scale[3];   /* a0, a1, a2 */
+    double   output_scale[3];    /* s_v */
+    double   loss_weight[3];     /* w_v */
+    double   init_output_gain;   /* multiplies the last layer's initial weights */
+} cdc_closure_config_t;
+
+int32_t cdc_closure_create(const cdc_closure_config_t *config, cdc_closure_t **out);
+void    cdc_closure_destroy(cdc_closure_t *closure);
+int64_t cdc_closure_num_params(void);
+int32_t cdc_closure_get_params(cdc_closure_t *closure, float *params_out);
+int32_t cdc_closure_set_params(cdc_closure_t *closure, const float *params);   /* resets Adam */
+int32_t cdc_closure_get_config(cdc_closure_t *closure, cdc_closure_config_t *out);
+/* Either pointer may be NULL (keep). */
+int32_t cdc_closure_set_scales(cdc_closure_t *closure, const double *output_scale,
+                               const double *loss_weight);
+/* Net input (bf16-rounded, [N*N][4]) and output ([N*N][6]) on a solver's
+ * committed state (N x N global grid, i-major). */
+int32_t cdc_closure_features(cdc_closure_t *closure, const cdc_solver_t *solver, float *x_out);
+int32_t cdc_closure_forward(cdc_closure_t *closure, const cdc_solver_t *solver, float *y_out);
+/* Store one training sample: features of base's state, target
+ * (truth - plain) / dt for (Sigma, Sx, Sy). */
+int32_t cdc_closure_add_sample(cdc_closure_t *closure, const cdc_solver_t *base,
+                               const cdc_solver_t *plain, const cdc_solver_t *truth, double dt);
+/* count, and the mean square of each target channel over all samples */
+int32_t cdc_closure_num_samples(cdc_closure_t *closure, int32_t *count, double *target_ms);
+int32_t cdc_closure_clear_samples(cdc_closure_t *closure);
+/* Loss and gradient (canonical order) of sample k, no update. */
+int32_t cdc_closure_sample_grad(cdc_closure_t *closure, int32_t k, double *loss_out,
+                                float *grad_out);
+/* `steps` Adam steps (learning rate lr), one sample per step in shuffled
+ * epochs; loss_out[steps] (may be NULL). */
+int32_t cdc_closure_train(cdc_closure_t *closure, int32_t steps, double lr, uint64_t seed,
+                          float *loss_out);
+/* One corrected coarse step of solver (cdc_solver_advance(solver, 1) plus
+ * the correction).  Errors as cdc_solver_advance; with an all-zero net the
+ * result is bitwise that of cdc_solver_advance. */
+int32_t cdc_closure_advance(cdc_closure_t *closure, cdc_solver_t *solver, double *dt_out,
+                            cdc_error_t *err);
+
 /* ---- host utilities (no GPU) -------------------------------------------- */
+
+/*
+ * Re-grid a state in the boundary layout between two meshes of one domain
+ * (the same rules as cdc_solver_resample_from): u_src [nbS^2][bsS][bsS][nv]
+ * -> u_dst [nbD^2][bsD][bsD][nv].  Per-block ist [.][7][nv] and oec [.][7][4]
+ * are copied when num_blocks is kept (the up-sampler's rule), summed over
+ * merged blocks when each coarse block covers whole fine blocks, and zeroed
+ * otherwise; the ist/oec pointers may be NULL.
+ */
+int32_t cdc_resample_state(const cdc_mesh_t *src, const cdc_mesh_t *dst, int32_t nv,
+                           const double *u_src, const double *ist_src,
+                           const double *oec_src, double *u_dst, double *ist_dst,
+                           double *oec_dst);
 
 /* Physics::orbital_state_from_time (src/physics.rs:419-421). */
 int32_t cdc_orbital_state_from_time(const cdc_physics_t *physics, double time,

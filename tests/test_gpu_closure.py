@@ -1,0 +1,218 @@
+"""The learned closure's HIP kernels (matrix-core bf16 convolutions, loss,
+backward, Adam, correction) against the FP64 CPU oracle
+(oracle/closure_oracle.py).  Bars: bf16 activations and gradients with fp32
+accumulation, so features within 2^-8 relative, net outputs within 2e-2 of
+max|Y|; the loss and each layer's gradient within 2e-3 and 1e-2 (relative,
+L2) of the oracle run with bf16 rounding where the device rounds, and within
+2e-2 and 0.1 of the exact FP64 oracle; an all-zero net leaves a coarse run bitwise unchanged;
+the correction conserves total mass and momentum to rounding."""
+import numpy as np
+import pytest
+
+from cdc_amd import Solver, abi
+from cdc_amd.closure import Closure, num_params
+from oracle import closure_oracle as C
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    lib = abi.load()
+    assert lib.cdc_device_count() >= 1, "no HIP device: the HIP path must run on a GPU"
+
+
+def to_global(u, nb, bs):
+    nv = u.shape[-1]
+    return u.reshape(nb, nb, bs, bs, nv).transpose(0, 2, 1, 3, 4).reshape(nb * bs, nb * bs, nv)
+
+
+def setup(nb=2, bs=16, R=4.0):
+    mesh, hydro, ph = H.small("euler", nb=nb, bs=bs, R=R)
+    u0 = H.initial(mesh, hydro, ph)
+    return mesh, hydro, ph, u0
+
+
+def oracle_layers(params):
+    """bf16-rounded weights (the matrix-core operands), fp32 biases."""
+    return [(C.bf16(W), np.asarray(b, dtype=np.float32).astype(np.float64)) for W, b in C.unpack(params)]
+
+
+def rand_params(seed, scale=0.25):
+    rng = np.random.default_rng(seed)
+    return np.concatenate([np.concatenate([rng.normal(size=co * ci * 9) * scale / np.sqrt(ci),
+                                           rng.normal(size=co) * 0.05]) for ci, co in C.LAYERS]).astype(np.float32)
+
+
+def test_features_match_oracle():
+    mesh, hydro, ph, u0 = setup()
+    g = Solver(mesh, hydro, ph, u0)
+    N = mesh.num_blocks * mesh.block_size
+    net = Closure(N, mesh.domain_radius, feature_scale=(0.25, 0.5, 0.5))
+    got = net.features(g)
+    want = C.bf16(C.features(to_global(u0, mesh.num_blocks, mesh.block_size), 4, hydro.gamma_law_index,
+                             (0.25, 0.5, 0.5)))
+    np.testing.assert_allclose(got, want, rtol=2.0 ** -8, atol=1e-30)
+
+
+def test_forward_matches_oracle():
+    mesh, hydro, ph, u0 = setup()
+    g = Solver(mesh, hydro, ph, u0)
+    N = mesh.num_blocks * mesh.block_size
+    net = Closure(N, mesh.domain_radius)
+    p = rand_params(5)
+    net.set_params(p)
+    np.testing.assert_array_equal(net.params(), p)
+    Y = net.forward(g)
+    x0 = np.zeros((N, N, 4))
+    x0[:] = net.features(g)
+    want = C.forward(oracle_layers(p), x0)[-1]
+    err = np.max(np.abs(Y - want)) / np.max(np.abs(want))
+    assert err <= 2e-2, err
+
+
+def rough_state(u0, hydro, rng):
+    """A valid gamma-law state with O(1) cell-to-cell variation of every
+    feature (so that no gradient is a near-cancelling sum)."""
+    shape = u0.shape[:-1]
+    sig = 1e-3 * np.exp(0.7 * rng.normal(size=shape))
+    vx, vy = rng.normal(size=shape), rng.normal(size=shape)
+    p = sig * np.exp(0.7 * rng.normal(size=shape))
+    u = np.empty_like(u0)
+    u[..., 0], u[..., 1], u[..., 2] = sig, sig * vx, sig * vy
+    u[..., 3] = p / (hydro.gamma_law_index - 1.0) + 0.5 * sig * (vx * vx + vy * vy)
+    return u
+
+
+def make_sample(net, mesh, hydro, ph, u0, seed=9):
+    """base: a rough state; target (truth - plain) / dt with plain, truth
+    small perturbations of it."""
+    rng = np.random.default_rng(seed)
+    ub = rough_state(u0, hydro, rng)
+    up = ub * (1.0 + 1e-3 * rng.normal(size=ub.shape))
+    ut = ub * (1.0 + 1e-3 * rng.normal(size=ub.shape))
+    base, plain, truth = (Solver(mesh, hydro, ph, u0) for _ in range(3))
+    base.set_state(ub)
+    plain.set_state(up)
+    truth.set_state(ut)
+    dt = 1e-3
+    net.add_sample(base, plain, truth, dt)
+    T = np.zeros(ub.shape[:-1] + (4,))
+    T[..., :3] = (ut[..., :3] - up[..., :3]) / dt
+    return base, to_global(T, mesh.num_blocks, mesh.block_size), dt
+
+
+def test_sample_gradient_matches_oracle():
+    mesh, hydro, ph, u0 = setup()
+    N = mesh.num_blocks * mesh.block_size
+    net = Closure(N, mesh.domain_radius, max_samples=2)
+    base, T, dt = make_sample(net, mesh, hydro, ph, u0)
+    n, ms = net.samples()
+    assert n == 1 and np.all(ms > 0)
+    np.testing.assert_allclose(ms, np.mean(T[..., :3] ** 2, axis=(0, 1)), rtol=1e-4)
+    dx = 2 * mesh.domain_radius / N
+    s = np.sqrt(ms) * dx
+    w = 1.0 / ms
+    net.set_scales(output_scale=s, loss_weight=w)
+    p = rand_params(6)
+    net.set_params(p)
+    loss, grad = net.sample_grad(0)
+    x0 = np.zeros((N, N, 4))
+    x0[:] = net.features(base)
+    T32 = np.asarray(T, dtype=np.float32).astype(np.float64)
+    # exact FP64 gradient (the bf16 device is within 10% of it) and the
+    # bf16-emulating oracle (rounding where the device rounds: within 1e-2)
+    L, gx, _ = C.loss_and_grads(oracle_layers(p), x0, T32, s, w, dx, dx)
+    Le, gw, _ = C.loss_and_grads(oracle_layers(p), x0, T32, s, w, dx, dx, rnd=C.bf16)
+    assert loss == pytest.approx(L, rel=2e-2)
+    assert loss == pytest.approx(Le, rel=2e-3)
+    k = 0
+    for layer, (ci, co) in enumerate(C.LAYERS):
+        nw = co * ci * 9
+        aw, bw = grad[k:k + nw], gw[k:k + nw]
+        rel = np.linalg.norm(aw - bw) / np.linalg.norm(bw)
+        assert rel <= 1e-2, ("W", layer, rel)
+        assert np.linalg.norm(aw - gx[k:k + nw]) <= 0.1 * np.linalg.norm(gx[k:k + nw]), ("W exact", layer)
+        ab, bb = grad[k + nw:k + nw + co], gw[k + nw:k + nw + co]
+        if layer == len(C.LAYERS) - 1:
+            # the loss sees only differences of Y: the output bias gradient is zero
+            assert np.linalg.norm(ab) <= 1e-2 * np.linalg.norm(bw), ("b", layer)
+        else:
+            rel = np.linalg.norm(ab - bb) / np.linalg.norm(bb)
+            assert rel <= 1e-2, ("b", layer, rel)
+        k += nw + co
+
+
+def from_global(g, nb, bs):
+    nv = g.shape[-1]
+    return g.reshape(nb, bs, nb, bs, nv).transpose(0, 2, 1, 3, 4).reshape(nb * nb, bs, bs, nv)
+
+
+def test_training_recovers_a_teacher_closure():
+    """Targets made by a random teacher net of the same architecture (through
+    the oracle on the GPU's own features): the student's loss must fall well
+    below the loss of predicting no correction."""
+    mesh, hydro, ph, u0 = setup()
+    nb, bs = mesh.num_blocks, mesh.block_size
+    N = nb * bs
+    dx = 2 * mesh.domain_radius / N
+    net = Closure(N, mesh.domain_radius, max_samples=4, seed=3, init_output_gain=0.1)
+    teacher = oracle_layers(rand_params(11, scale=0.5))
+    probe = Closure(N, mesh.domain_radius)
+    dt = 1e-3
+    rng = np.random.default_rng(12)
+    for _ in range(4):
+        ub = rough_state(u0, hydro, rng)
+        base, plain, truth = (Solver(mesh, hydro, ph, u0) for _ in range(3))
+        base.set_state(ub)
+        plain.set_state(ub)
+        x0 = probe.features(base).astype(np.float64)
+        r = C.correction(C.forward(teacher, x0)[-1], [1.0, 1.0, 1.0], dx, dx)
+        ut = ub.copy()
+        ut[..., :3] += dt * from_global(r, nb, bs)
+        truth.set_state(ut)
+        net.add_sample(base, plain, truth, dt)
+    _, ms = net.samples()
+    net.set_scales(output_scale=[1.0, 1.0, 1.0], loss_weight=1.0 / ms)
+    hist = net.train(400, lr=2e-3, seed=4)
+    assert np.all(np.isfinite(hist))
+    # predicting no correction scores 3 (each channel's weighted mean square)
+    assert hist[-40:].mean() < 0.3 * 3.0, (hist[:4], hist[-40:].mean())
+
+
+def test_zero_net_is_the_plain_run():
+    mesh, hydro, ph, u0 = setup()
+    N = mesh.num_blocks * mesh.block_size
+    net = Closure(N, mesh.domain_radius)
+    net.set_params(np.zeros(num_params(), dtype=np.float32))
+    a, b = Solver(mesh, hydro, ph, u0), Solver(mesh, hydro, ph, u0)
+    for _ in range(3):
+        assert net.advance(a) == b.advance(1)
+    sa, sb = a.get_state(), b.get_state()
+    np.testing.assert_array_equal(sa["conserved"], sb["conserved"])
+    np.testing.assert_array_equal(sa["ist"], sb["ist"])
+    assert sa["time"] == sb["time"]
+
+
+def test_correction_conserves_mass_and_momentum():
+    mesh, hydro, ph, u0 = setup()
+    N = mesh.num_blocks * mesh.block_size
+    net = Closure(N, mesh.domain_radius)
+    net.set_params(rand_params(8))
+    net.set_scales(output_scale=[1e-6, 1e-6, 1e-6])
+    a, b = Solver(mesh, hydro, ph, u0), Solver(mesh, hydro, ph, u0)
+    Y = net.forward(a)
+    dt = net.advance(a)
+    b.advance(1)
+    ua, ub = a.get_state()["conserved"], b.get_state()["conserved"]
+    d = ua - ub
+    assert np.max(np.abs(d[..., :3])) > 0          # the correction acted
+    np.testing.assert_array_equal(d[..., 3], 0.0)  # energy untouched
+    for v in range(3):
+        assert abs(d[..., v].sum()) <= 1e-12 * np.abs(d[..., v]).sum()
+    # and it is the oracle's correction of the net output
+    dx = 2 * mesh.domain_radius / N
+    r = C.correction(Y.astype(np.float64), [1e-6] * 3, dx, dx)
+    np.testing.assert_allclose(to_global(d, mesh.num_blocks, mesh.block_size)[..., :3], dt * r,
+                               rtol=1e-6, atol=1e-6 * np.max(np.abs(dt * r)))
