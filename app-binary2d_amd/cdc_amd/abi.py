@@ -129,6 +129,8 @@ class CClosureConfig(ctypes.Structure):
         ("output_scale", c_double * 3),
         ("loss_weight", c_double * 3),
         ("init_output_gain", c_double),
+        ("limiter", c_double),
+        ("mask_density", c_double),
     ]
 
 
@@ -152,6 +154,13 @@ SIGNATURES = {
     "cdc_resample_state": (c_int32, [POINTER(CMesh), POINTER(CMesh), c_int32, c_void_p, c_void_p, c_void_p,
                                      c_void_p, c_void_p, c_void_p]),
     "cdc_solver_max_signal_speed": (c_int32, [c_void_p, POINTER(c_double)]),
+    "cdc_solver_get_torque": (c_int32, [c_void_p, c_void_p]),
+    "cdc_snapshot_create": (c_int32, [c_void_p, POINTER(c_void_p)]),
+    "cdc_snapshot_destroy": (None, [c_void_p]),
+    "cdc_snapshot_time": (c_int32, [c_void_p, POINTER(c_double)]),
+    "cdc_solver_save": (c_int32, [c_void_p, c_void_p]),
+    "cdc_solver_restore": (c_int32, [c_void_p, c_void_p]),
+    "cdc_solver_time_series_sample": (c_int32, [c_void_p, POINTER(c_double), c_void_p, c_void_p, c_void_p]),
     "cdc_solver_set_timing": (c_int32, [c_void_p, c_int32]),
     "cdc_solver_get_timing": (c_int32, [c_void_p, POINTER(c_int64), POINTER(c_double), POINTER(c_double)]),
     "cdc_solver_set_tile": (c_int32, [c_void_p, c_int32, c_int32]),

@@ -25,7 +25,8 @@ class Closure:
     """Net, Adam state and training samples on the GPU for one coarse grid."""
 
     def __init__(self, grid, domain_radius, max_samples=0, seed=1, feature_scale=(0.25, 0.5, 0.5),
-                 output_scale=(1.0, 1.0, 1.0), loss_weight=(1.0, 1.0, 1.0), init_output_gain=1.0):
+                 output_scale=(1.0, 1.0, 1.0), loss_weight=(1.0, 1.0, 1.0), init_output_gain=1.0, limiter=0.0,
+                 mask_density=0.0):
         self.lib = abi.load()
         c = abi.CClosureConfig()
         c.grid, c.max_samples, c.seed, c.domain_radius = int(grid), int(max_samples), int(seed), float(domain_radius)
@@ -33,6 +34,8 @@ class Closure:
         c.output_scale[:] = list(output_scale)
         c.loss_weight[:] = list(loss_weight)
         c.init_output_gain = float(init_output_gain)
+        c.limiter = float(limiter)
+        c.mask_density = float(mask_density)
         h = c_void_p()
         _check(self.lib, self.lib.cdc_closure_create(byref(c), byref(h)))
         self.handle = h
@@ -112,3 +115,146 @@ class Closure:
             raise HydroError(err)
         _check(self.lib, rc)
         return dt.value
+
+
+ORBIT = 2.0 * math.pi
+
+
+def accreted(solver):
+    """Mass taken by both sinks so far: minus the sum over blocks of the
+    sink items' density component of integrated_source_terms."""
+    ist = solver.get_state()["ist"]
+    return -float(ist[:, 0, 0].sum() + ist[:, 1, 0].sum())
+
+
+class Stalled(RuntimeError):
+    """The time step collapsed (a run that would never reach its end)."""
+
+
+def run_to(step, solver, t_end, log=None, every=2000, min_dt_ratio=1e-2):
+    """Advance with `step(solver)` (which returns the time advanced) until
+    the solver's time reaches t_end.  Raises Stalled when a step advances
+    less than min_dt_ratio of the first one.  Returns (steps, wall seconds)."""
+    n = 0
+    t0 = time.perf_counter()
+    t = solver.time()
+    first = None
+    while t < t_end:
+        d = step(solver)
+        first = d if first is None else first
+        if d < min_dt_ratio * first:
+            raise Stalled(f"time step collapsed to {d:.3e} (first {first:.3e}) at t = {t / ORBIT:.4f} orbits")
+        t += d
+        n += 1
+        if n % 64 == 0:
+            t = solver.time()      # the exact committed time
+        if log and n % every == 0:
+            log(f"  t = {t / ORBIT:.3f} orbits after {n} steps")
+    return n, time.perf_counter() - t0
+
+
+def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, sample_every=100,
+                    max_samples=200, train_steps=4000, lr=1e-3, seed=5, limiter=0.1,
+                    variants=((0.1, 1.0),), log=print):
+    """Train a closure for the coarse grid on pairs from the fine workload,
+    then run plain fine, plain coarse and learned coarse runs over the
+    held-out window [train_orbits, train_orbits + eval_orbits] from the same
+    fine state; one learned run per (limiter kappa, damping, mask density)
+    variant, the damping scaling the trained output.  Returns a dict of wall times per
+    orbit and accretion rates."""
+    from .config import Mesh
+    from .solver import Solver, resample_state
+    from . import workloads
+
+    fm, hydro, ph = fine_w["mesh"], fine_w["hydro"], fine_w["physics"]
+    u0 = workloads.initial_state(fine_w)
+    cm = Mesh(coarse_blocks, fm.num_blocks * fm.block_size // (4 * coarse_blocks), fm.domain_radius)
+    cu0, _, _ = resample_state(fm, cm, u0)
+    fine = Solver(fm, hydro, ph, u0)
+    base, plain, truth = (Solver(cm, hydro, ph, cu0) for _ in range(3))
+    N = cm.num_blocks * cm.block_size
+    dxc = 2.0 * cm.domain_radius / N
+    net = Closure(N, cm.domain_radius, max_samples=max_samples, seed=seed, init_output_gain=0.1,
+                  limiter=limiter)
+
+    # training pairs along the fine run
+    t_train = train_orbits * ORBIT
+    log(f"training pairs: fine {fm.num_blocks * fm.block_size}^2 to {train_orbits} orbits, coarse {N}^2")
+    tp0 = time.perf_counter()
+    nfine = 0
+    while fine.time() < t_train:
+        for _ in range(max(1, sample_every // 10)):
+            fine.advance(10)
+        nfine += max(1, sample_every // 10) * 10
+        if net.samples()[0] >= max_samples:
+            continue
+        base.resample_from(fine)
+        plain.resample_from(fine)
+        dtc = min(plain.advance(0), 4.0 * fine.advance(0))
+        plain.advance_dt(dtc)
+        fine.advance_dt(dtc / 4.0, 4)
+        nfine += 4
+        truth.resample_from(fine)
+        net.add_sample(base, plain, truth, dtc)
+        k = net.samples()[0]
+        if k % 20 == 0:
+            log(f"  {k} samples, t = {fine.time() / ORBIT:.3f} orbits")
+    tpairs = time.perf_counter() - tp0
+    nsamp, ms = net.samples()
+    net.set_scales(output_scale=np.sqrt(ms) * dxc, loss_weight=1.0 / ms)
+    log(f"{nsamp} samples in {tpairs:.1f} s; target rms {np.sqrt(ms)}")
+
+    tt0 = time.perf_counter()
+    hist = net.train(train_steps, lr=lr, seed=seed)
+    ttrain = time.perf_counter() - tt0
+    params = net.params()
+    log(f"trained {train_steps} steps in {ttrain:.1f} s: loss {hist[:50].mean():.4f} -> {hist[-200:].mean():.4f}")
+
+    # held-out window from the fine state at the end of training
+    t_start = fine.time()
+    t_end = t_start + eval_orbits * ORBIT
+    start = Solver(cm, hydro, ph, cu0)
+    start.resample_from(fine)
+    coarse = Solver(cm, hydro, ph, cu0)
+    coarse.resample_from(fine)
+    trained_scale = np.sqrt(ms) * dxc
+    out = {"fine_grid": fm.num_blocks * fm.block_size, "coarse_grid": N, "samples": nsamp,
+           "train_steps": train_steps, "loss_first50": float(hist[:50].mean()),
+           "loss_last200": float(hist[-200:].mean()), "pairs_s": tpairs, "train_s": ttrain,
+           "window_orbits": [t_start / ORBIT, t_end / ORBIT]}
+    # plain runs step as the reference's driver does (fold = 10 steps per
+    # advance at one dt); the learned run corrects every step
+    runs = [("fine", fine, lambda s: 10 * s.advance(10), None),
+            ("coarse", coarse, lambda s: 10 * s.advance(10), None)]
+    for v in variants:
+        kappa, damping, mask = (tuple(v) + (0.0,))[:3]
+        runs.append((f"learned_k{kappa:g}_d{damping:g}_m{mask:g}", None, lambda s: net.advance(s),
+                     (kappa, damping, mask)))
+    for name, solver, step, var in runs:
+        if var is not None:
+            net.close()
+            net = Closure(N, cm.domain_radius, limiter=var[0], mask_density=var[2])
+            net.set_params(params)
+            net.set_scales(output_scale=trained_scale * var[1])
+            solver = Solver(cm, hydro, ph, cu0)
+            solver.resample_from(start)
+        m0, ta = accreted(solver), solver.time()
+        try:
+            n, wall = run_to(step, solver, t_end, log=lambda m: log(f"{name}:{m}"), every=1000)
+        except (HydroError, Stalled) as e:
+            log(f"{name}: failed: {e}")
+            out[name] = {"failed": str(e)}
+            continue
+        tb = solver.time()
+        mdot = (accreted(solver) - m0) / (tb - ta)
+        out[name] = {"advance_calls": n, "wall_s": wall, "orbits": (tb - ta) / ORBIT,
+                     "wall_per_orbit_s": wall / ((tb - ta) / ORBIT), "mdot": mdot}
+        log(f"{name}: {n} steps, {wall:.2f} s, mdot {mdot:.6e}")
+    if "mdot" in out.get("fine", {}):
+        ref = out["fine"]["mdot"]
+        for name in [r[0] for r in runs[1:]]:
+            if "mdot" in out.get(name, {}):
+                out[name]["mdot_rel_err"] = abs(out[name]["mdot"] - ref) / abs(ref)
+                out[name]["speedup_vs_fine"] = out["fine"]["wall_per_orbit_s"] / out[name]["wall_per_orbit_s"]
+    net.close()
+    return out

@@ -170,6 +170,29 @@ class Solver:
         """This solver's committed state := src's, re-gridded on the device."""
         _check(self.lib, self.lib.cdc_solver_resample_from(self.handle, src.handle))
 
+    def save(self, snapshot):
+        _check(self.lib, self.lib.cdc_solver_save(self.handle, snapshot.handle))
+
+    def restore(self, snapshot):
+        _check(self.lib, self.lib.cdc_solver_restore(self.handle, snapshot.handle))
+
+    def torque(self):
+        """Per-block torque impulses on body 1 and 2, [nblk][2]."""
+        out = np.empty((self.nblk, 2))
+        _check(self.lib, self.lib.cdc_solver_get_torque(self.handle, _ptr(out)))
+        return out
+
+    def time_series_sample(self):
+        """State::time_series_sample: dict(time, ist [7][nv], oec [7][4],
+        torque [2]) summed over blocks."""
+        t = c_double()
+        ist = np.empty((7, self.nv))
+        oec = np.empty((7, 4))
+        tq = np.empty(2)
+        _check(self.lib, self.lib.cdc_solver_time_series_sample(self.handle, byref(t), _ptr(ist), _ptr(oec),
+                                                               _ptr(tq)))
+        return {"time": t.value, "ist": ist, "oec": oec, "torque": tq}
+
     def max_signal_speed(self):
         a = c_double()
         _check(self.lib, self.lib.cdc_solver_max_signal_speed(self.handle, byref(a)))
@@ -197,6 +220,32 @@ class Solver:
         n, ms, cells = c_int64(), c_double(), c_double()
         _check(self.lib, self.lib.cdc_solver_get_timing(self.handle, byref(n), byref(ms), byref(cells)))
         return n.value, ms.value, cells.value
+
+
+class Snapshot:
+    """A device copy of a solver's committed state (cdc_snapshot_*)."""
+
+    def __init__(self, solver):
+        self.lib = abi.load()
+        h = c_void_p()
+        _check(self.lib, self.lib.cdc_snapshot_create(solver.handle, byref(h)))
+        self.handle = h
+
+    def time(self):
+        t = c_double()
+        _check(self.lib, self.lib.cdc_snapshot_time(self.handle, byref(t)))
+        return t.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.cdc_snapshot_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Ensemble:

@@ -4,12 +4,14 @@
 // whose 6 output channels are corrections to the x- and y-face fluxes of
 // (Sigma, Sx, Sy).  No counterpart in the reference.
 //
+// Features (clamped to [-16, 16]): a0 ln Sigma, a1 vx, a1 vy, a2 ln(p/Sigma).
 // A corrected coarse step: features of the state u_n at the step's start ->
 // net -> Y; the plain RK step u_n -> u* (cdc_solver_advance); then
 //   u_v = u*_v + dt r_v,  r_v = -s_v ((Y_v[i,j] - Y_v[i-1,j]) / dx
 //                                   + (Y_{3+v}[i,j] - Y_{3+v}[i,j-1]) / dy)
 // for v = Sigma, Sx, Sy: a flux divergence, so mass and momentum are
-// conserved.  Training fits r to (u_fine - u_plain) / dt on pairs made from a
+// conserved; each face's corrections are scaled down where they would move
+// more than a fraction kappa of a neighbour's mass (limit_kernel).  Training fits r to (u_fine - u_plain) / dt on pairs made from a
 // down-sampled fine run.
 //
 // Layouts: activations [pixel][32] bf16 (channel fastest, 64 B per pixel),
@@ -298,7 +300,7 @@ __global__ void __launch_bounds__(256) features_kernel(const FeatArgs a) {
     }
     uint16_t *o = a.x + p * C;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) o[c] = f2bf(f[c]);
+    for (int c = 0; c < 4; ++c) o[c] = f2bf(fminf(fmaxf(f[c], -16.0f), 16.0f));
     for (int c = 4; c < C; ++c) o[c] = 0;
 }
 
@@ -336,10 +338,58 @@ __global__ void __launch_bounds__(256) target_kernel(const TargetArgs a) {
 struct ApplyArgs {
     double *u;            // SoA [nv][N][N], corrected in place
     const float *y;       // [N*N][YS]
-    int N;
-    double dt, idx, idy;
+    double *lam;          // [N*N][2] face limiter factors (x+, y+ face of each cell)
+    int N, nv;
+    double dt, idx, idy, kappa, gm, mask;
     double s[3];
 };
+
+// Limiter, one pass before the update (so that it reads the uncorrected
+// state): the three corrections of a face are scaled by the largest
+// lambda <= 1 with
+//   dt |s_0 Y_Sigma| / dx        <= kappa min(Sigma_L, Sigma_R)
+//   dt |s_v Y_Sv| / dx (v = 1,2) <= kappa min(Sigma_L, Sigma_R) max(a_L, a_R)
+// (a = |v| + c_s, the cell's signal speed; c_s = 0 for isothermal), so that
+// no face moves more than the fraction kappa of a neighbour's mass, or of its
+// mass times the local signal speed in momentum.  kappa <= 0 disables it.
+// Faces where either density is below mask_density get no correction.
+__device__ inline double signal_speed(const ApplyArgs &a, int64_t NN, int64_t q) {
+    const double sig = a.u[q];
+    const double vx = a.u[NN + q] / sig, vy = a.u[2 * NN + q] / sig;
+    double c2 = 0.0;
+    if (a.nv == 4) c2 = a.gm * (a.u[3 * NN + q] - 0.5 * sig * (vx * vx + vy * vy)) * (a.gm - 1.0) / sig;
+    return sqrt(vx * vx + vy * vy) + sqrt(fmax(c2, 0.0));
+}
+
+__global__ void __launch_bounds__(256) limit_kernel(const ApplyArgs a) {
+    const int64_t NN = (int64_t)a.N * a.N;
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= NN) return;
+    const int i = (int)(p / a.N), j = (int)(p - (p / a.N) * a.N);
+    const int64_t nb[2] = {(int64_t)wrap(i + 1, a.N) * a.N + j, (int64_t)i * a.N + wrap(j + 1, a.N)};
+    const double id[2] = {a.idx, a.idy};
+    double l[2] = {1.0, 1.0};
+    for (int d = 0; d < 2; ++d)
+        if (fmin(a.u[p], a.u[nb[d]]) < a.mask) l[d] = 0.0;
+    if (a.kappa > 0.0) {
+        const double sig = a.u[p];
+        const double ap = signal_speed(a, NN, p);
+        for (int d = 0; d < 2; ++d) {
+            if (l[d] == 0.0) continue;
+            const double cap = a.kappa * fmax(fmin(sig, a.u[nb[d]]), 0.0);
+            const double capm = cap * fmax(ap, signal_speed(a, NN, nb[d]));
+            double lam = 1.0;
+            for (int v = 0; v < 3; ++v) {
+                const double m = a.dt * fabs(a.s[v] * (double)a.y[p * YS + 3 * d + v]) * id[d];
+                const double c = v == 0 ? cap : capm;
+                if (m > c) lam = fmin(lam, c / m);
+            }
+            l[d] = lam;
+        }
+    }
+    a.lam[p * 2] = l[0];
+    a.lam[p * 2 + 1] = l[1];
+}
 
 __global__ void __launch_bounds__(256) apply_kernel(const ApplyArgs a) {
     const int64_t NN = (int64_t)a.N * a.N;
@@ -347,9 +397,10 @@ __global__ void __launch_bounds__(256) apply_kernel(const ApplyArgs a) {
     if (p >= NN) return;
     const int i = (int)(p / a.N), j = (int)(p - (p / a.N) * a.N);
     const int64_t xm = (int64_t)wrap(i - 1, a.N) * a.N + j, ym = (int64_t)i * a.N + wrap(j - 1, a.N);
+    const double lxp = a.lam[p * 2], lxm = a.lam[xm * 2], lyp = a.lam[p * 2 + 1], lym = a.lam[ym * 2 + 1];
     for (int v = 0; v < 3; ++v) {
-        const double fx = (double)a.y[p * YS + v] - (double)a.y[xm * YS + v];
-        const double fy = (double)a.y[p * YS + 3 + v] - (double)a.y[ym * YS + 3 + v];
+        const double fx = lxp * (double)a.y[p * YS + v] - lxm * (double)a.y[xm * YS + v];
+        const double fy = lyp * (double)a.y[p * YS + 3 + v] - lym * (double)a.y[ym * YS + 3 + v];
         const double r = -a.s[v] * (fx * a.idx + fy * a.idy);
         a.u[v * NN + p] += a.dt * r;
     }
@@ -369,6 +420,7 @@ struct cdc_closure {
     uint16_t *d_act[4] = {};      // X0, A1, A2, A3
     uint16_t *d_g[2] = {};        // gradient ping-pong [P][32]
     float *d_y = nullptr;         // [P][YS]
+    double *d_lam = nullptr;      // [P][2] limiter factors
     float *d_part_w = nullptr, *d_part_b = nullptr, *d_part_l = nullptr;
     int nwg = 0, chunk = 0, nlb = 0;
     // dataset
@@ -387,6 +439,7 @@ void free_closure(cdc_closure *c) {
     float *fl[] = {c->d_par, c->d_grad, c->d_m, c->d_v, c->d_y, c->d_part_w, c->d_part_b, c->d_part_l, c->d_tgt,
                    c->d_hist};
     for (float *p : fl) (void)hipFree(p);
+    (void)hipFree(c->d_lam);
     uint16_t *hl[] = {c->d_wf, c->d_wb, c->d_act[0], c->d_act[1], c->d_act[2], c->d_act[3], c->d_g[0], c->d_g[1],
                       c->d_feat};
     for (uint16_t *p : hl) (void)hipFree(p);
@@ -603,6 +656,7 @@ int32_t cdc_closure_create(const cdc_closure_config_t *cfg, cdc_closure_t **out)
     for (int i = 0; i < 4 && ok; ++i) ok = hipMalloc((void **)&c->d_act[i], act) == hipSuccess;
     for (int i = 0; i < 2 && ok; ++i) ok = hipMalloc((void **)&c->d_g[i], act) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->d_y, (size_t)c->P * YS * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->d_lam, (size_t)c->P * 2 * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->d_part_w, (size_t)c->nwg * 9 * C * C * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->d_part_b, (size_t)c->nwg * C * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->d_part_l, (size_t)c->nlb * 3 * sizeof(float)) == hipSuccess;
@@ -817,14 +871,21 @@ int32_t cdc_closure_advance(cdc_closure_t *c, cdc_solver_t *s, double *dt_out, c
     ApplyArgs a{};
     a.u = s->d_u[s->cur];
     a.y = c->d_y;
+    a.lam = c->d_lam;
     a.N = s->N;
     a.dt = dt;
     const double dx = 2.0 * s->mesh.domain_radius / (double)s->N;
     a.idx = 1.0 / dx;
     a.idy = 1.0 / dx;
+    a.kappa = c->cfg.limiter;
+    a.mask = c->cfg.mask_density;
+    a.nv = s->nv;
+    a.gm = s->hydro.kind == CDC_EULER ? s->hydro.gamma_law_index : 1.0;
     for (int v = 0; v < 3; ++v) a.s[v] = c->cfg.output_scale[v];
-    hipLaunchKernelGGL(apply_kernel, dim3((unsigned)(((int64_t)s->N * s->N + 255) / 256)), dim3(256), 0, s->stream,
-                       a);
+    const dim3 grid((unsigned)(((int64_t)s->N * s->N + 255) / 256));
+    hipLaunchKernelGGL(limit_kernel, grid, dim3(256), 0, s->stream, a);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(apply_kernel, grid, dim3(256), 0, s->stream, a);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->amax_valid = false;   // the correction changed the state the last stage measured

@@ -60,6 +60,7 @@ struct cdc_solver {
     double *d_u[cdc::host::NBUF] = {};
     double *d_ist[cdc::host::NBUF] = {};
     double *d_oec[cdc::host::NBUF] = {};
+    double *d_tq[cdc::host::NBUF] = {};   // [nblk][2] torque impulses (extension)
     double *d_tile_sums = nullptr;
     cdc::TileErr *d_tile_err = nullptr;
     int32_t *d_perr = nullptr;

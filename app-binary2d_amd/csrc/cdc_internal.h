@@ -62,6 +62,9 @@ struct StageArgs {
     const double *ist_c, *ist_b;   // [nblk][7][nv]
     const double *oec_c, *oec_b;   // [nblk][7][4]
     double *ist_o, *oec_o;
+    // per-block torque impulses on body 1, 2 about the origin ([nblk][2])
+    const double *tq_c, *tq_b;
+    double *tq_o;
     int32_t *perr;                 // [nblk] perturbation error of the block
     int32_t N, nb, bs, ntiles, tpb;  // tpb: tiles per block
     int32_t stage_id, read_base, want_amax, reuse_prim;

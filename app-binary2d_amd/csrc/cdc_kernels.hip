@@ -729,6 +729,21 @@ __device__ __forceinline__ void finalize_block(const StageArgs &A, const int b, 
         return;
     }
     if (tid == 0) A.perr[b] = 0;
+    if (tid >= 32 && tid < 34) {
+        // torque impulse on body k about the origin: x_k x (-dp_k), dp_k the
+        // gas momentum change from body k's gravity (no reference counterpart)
+        const int k = tid - 32;
+        const double *g = s_ds + (CDC_ITEM_GRAV1 + k) * NV;
+        const double x = k == 0 ? A.orb.x1 : A.orb.x2, y = k == 0 ? A.orb.y1 : A.orb.y2;
+        double tau;
+        {
+CDC_NO_CONTRACT
+            tau = y * g[1] - x * g[2];
+        }
+        const size_t ti = (size_t)b * 2 + k;
+        const double s1 = A.tq_c[ti] + tau;
+        A.tq_o[ti] = A.read_base ? s1 * A.c1 + A.tq_b[ti] * A.c0 : s1;
+    }
     const size_t oi = (size_t)b * 7 * NV, oe = (size_t)b * 7 * 4;
     if (tid < 7 * 4) {
         if (tid < 7 * NV) {

@@ -152,6 +152,7 @@ void free_all(cdc_solver *s) {
         (void)hipFree(s->d_u[i]);
         (void)hipFree(s->d_ist[i]);
         (void)hipFree(s->d_oec[i]);
+        (void)hipFree(s->d_tq[i]);
     }
     (void)hipFree(s->d_tile_sums);
     (void)hipFree(s->d_tile_err);
@@ -330,6 +331,9 @@ void plan_steps(const cdc_solver *s, int fold, double dt, StepPlan &plan) {
         a.oec_b = s->d_oec[base.idx];
         a.ist_o = s->d_ist[out.idx];
         a.oec_o = s->d_oec[out.idx];
+        a.tq_c = s->d_tq[in.idx];
+        a.tq_b = s->d_tq[base.idx];
+        a.tq_o = s->d_tq[out.idx];
         a.perr = s->d_perr;
         a.reuse_prim = s->hydro.has_density_floor ? 0 : 1;
         a.N = s->N;
@@ -555,6 +559,7 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
         ALLOC(s->d_u[i], sb);
         ALLOC(s->d_ist[i], (size_t)s->nblk * 7 * s->nv * sizeof(double));
         ALLOC(s->d_oec[i], (size_t)s->nblk * 7 * 4 * sizeof(double));
+        ALLOC(s->d_tq[i], (size_t)s->nblk * 2 * sizeof(double));
     }
     ALLOC(s->d_tile_sums, (size_t)s->max_tiles * (s->nsum + 1) * sizeof(double));
     ALLOC(s->d_tile_err, (size_t)s->max_tiles * sizeof(cdc::TileErr));
@@ -572,6 +577,7 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
     if (hipMemcpyAsync(s->d_u[0], s->d_init, sb, hipMemcpyDeviceToDevice, s->stream) != hipSuccess ||
         hipMemsetAsync(s->d_ist[0], 0, (size_t)s->nblk * 7 * s->nv * sizeof(double), s->stream) != hipSuccess ||
         hipMemsetAsync(s->d_oec[0], 0, (size_t)s->nblk * 7 * 4 * sizeof(double), s->stream) != hipSuccess ||
+        hipMemsetAsync(s->d_tq[0], 0, (size_t)s->nblk * 2 * sizeof(double), s->stream) != hipSuccess ||
         hipStreamSynchronize(s->stream) != hipSuccess)
         return fail(CDC_ERR_HIP, "initial state upload failed");
     rc = compute_aux(s);
@@ -611,6 +617,7 @@ int32_t cdc_solver_set_state(cdc_solver_t *s, double time, int64_t num, int64_t 
     else HIP_TRY(hipMemsetAsync(s->d_ist[s->cur], 0, ni, s->stream));
     if (oec) HIP_TRY(hipMemcpyAsync(s->d_oec[s->cur], oec, ne, hipMemcpyHostToDevice, s->stream));
     else HIP_TRY(hipMemsetAsync(s->d_oec[s->cur], 0, ne, s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_tq[s->cur], 0, (size_t)s->nblk * 2 * sizeof(double), s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->time = time;
     s->num = num;
@@ -753,6 +760,136 @@ int32_t cdc_solver_set_timing(cdc_solver_t *s, int32_t enable) {
     return CDC_OK;
 }
 
+// A copy of a solver's committed state in device memory (the fallback
+// states of the driver loop, src/main.rs:148-200, without a host round trip).
+struct cdc_snapshot {
+    int nv = 0, nblk = 0;
+    size_t ncell = 0;
+    double *u = nullptr, *ist = nullptr, *oec = nullptr, *tq = nullptr;
+    double time = 0.0;
+    int64_t num = 0, den = 1;
+    bool valid = false;
+};
+
+int32_t cdc_snapshot_create(const cdc_solver_t *s, cdc_snapshot_t **out) {
+    if (!s || !out) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null pointer");
+    *out = nullptr;
+    cdc_snapshot *p = new cdc_snapshot();
+    p->nv = s->nv;
+    p->nblk = s->nblk;
+    p->ncell = s->ncell();
+    if (hipMalloc((void **)&p->u, s->nstate() * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&p->ist, (size_t)s->nblk * 7 * s->nv * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&p->oec, (size_t)s->nblk * 7 * 4 * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&p->tq, (size_t)s->nblk * 2 * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        cdc_snapshot_destroy(p);
+        return set_msg(CDC_ERR_OUT_OF_MEMORY, "snapshot buffers");
+    }
+    *out = p;
+    return CDC_OK;
+}
+
+void cdc_snapshot_destroy(cdc_snapshot_t *p) {
+    if (!p) return;
+    (void)hipFree(p->u);
+    (void)hipFree(p->ist);
+    (void)hipFree(p->oec);
+    (void)hipFree(p->tq);
+    delete p;
+}
+
+static bool snapshot_fits(const cdc_solver *s, const cdc_snapshot *p) {
+    return p->nv == s->nv && p->nblk == s->nblk && p->ncell == s->ncell();
+}
+
+int32_t cdc_solver_save(cdc_solver_t *s, cdc_snapshot_t *p) {
+    if (!s || !p) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null pointer");
+    if (!snapshot_fits(s, p)) return set_msg(CDC_ERR_INVALID_ARGUMENT, "snapshot made for another mesh");
+    const int c = s->cur;
+    HIP_TRY(hipMemcpyAsync(p->u, s->d_u[c], s->nstate() * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(p->ist, s->d_ist[c], (size_t)s->nblk * 7 * s->nv * sizeof(double),
+                           hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(p->oec, s->d_oec[c], (size_t)s->nblk * 7 * 4 * sizeof(double), hipMemcpyDeviceToDevice,
+                           s->stream));
+    HIP_TRY(hipMemcpyAsync(p->tq, s->d_tq[c], (size_t)s->nblk * 2 * sizeof(double), hipMemcpyDeviceToDevice,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    p->time = s->time;
+    p->num = s->num;
+    p->den = s->den;
+    p->valid = true;
+    return CDC_OK;
+}
+
+int32_t cdc_solver_restore(cdc_solver_t *s, const cdc_snapshot_t *p) {
+    if (!s || !p) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null pointer");
+    if (!snapshot_fits(s, p)) return set_msg(CDC_ERR_INVALID_ARGUMENT, "snapshot made for another mesh");
+    if (!p->valid) return set_msg(CDC_ERR_INVALID_ARGUMENT, "snapshot holds no state");
+    const int c = (s->cur + 1) % NBUF;
+    HIP_TRY(hipMemcpyAsync(s->d_u[c], p->u, s->nstate() * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_ist[c], p->ist, (size_t)s->nblk * 7 * s->nv * sizeof(double),
+                           hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_oec[c], p->oec, (size_t)s->nblk * 7 * 4 * sizeof(double), hipMemcpyDeviceToDevice,
+                           s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_tq[c], p->tq, (size_t)s->nblk * 2 * sizeof(double), hipMemcpyDeviceToDevice,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->cur = c;
+    s->time = p->time;
+    s->num = p->num;
+    s->den = p->den;
+    s->amax_valid = false;
+    s->version += 1;
+    return CDC_OK;
+}
+
+int32_t cdc_snapshot_time(const cdc_snapshot_t *p, double *time) {
+    if (!p || !time) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null pointer");
+    *time = p->time;
+    return CDC_OK;
+}
+
+int32_t cdc_solver_get_torque(cdc_solver_t *s, double *torque) {
+    if (!s || !torque) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null pointer");
+    HIP_TRY(hipMemcpyAsync(torque, s->d_tq[s->cur], (size_t)s->nblk * 2 * sizeof(double), hipMemcpyDeviceToHost,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return CDC_OK;
+}
+
+int32_t cdc_solver_time_series_sample(cdc_solver_t *s, double *time, double *ist_total, double *oec_total,
+                                      double *torque_total) {
+    if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
+    const size_t ni = (size_t)s->nblk * 7 * s->nv, ne = (size_t)s->nblk * 7 * 4, nt = (size_t)s->nblk * 2;
+    std::vector<double> ist(ni), oec(ne), tq(nt);
+    HIP_TRY(hipMemcpyAsync(ist.data(), s->d_ist[s->cur], ni * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(oec.data(), s->d_oec[s->cur], ne * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(tq.data(), s->d_tq[s->cur], nt * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (time) *time = s->time;
+    // sums over blocks in Mesh::block_indexes() order
+    if (ist_total)
+        for (size_t k = 0; k < (size_t)7 * s->nv; ++k) {
+            double a = 0.0;
+            for (int b = 0; b < s->nblk; ++b) a += ist[(size_t)b * 7 * s->nv + k];
+            ist_total[k] = a;
+        }
+    if (oec_total)
+        for (size_t k = 0; k < 7 * 4; ++k) {
+            double a = 0.0;
+            for (int b = 0; b < s->nblk; ++b) a += oec[(size_t)b * 7 * 4 + k];
+            oec_total[k] = a;
+        }
+    if (torque_total)
+        for (size_t k = 0; k < 2; ++k) {
+            double a = 0.0;
+            for (int b = 0; b < s->nblk; ++b) a += tq[(size_t)b * 2 + k];
+            torque_total[k] = a;
+        }
+    return CDC_OK;
+}
+
 int32_t cdc_solver_get_timing(cdc_solver_t *s, int64_t *launches, double *total_ms, double *cells_per_launch) {
     if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
     if (launches) *launches = s->launches;
@@ -844,6 +981,7 @@ int32_t cdc_solver_resample_from(cdc_solver_t *dst, const cdc_solver_t *src) {
     cdc::host::resample_block_sums(&src->mesh, &dst->mesh, dst->nv, is.data(), es.data(), id.data(), ed.data());
     HIP_TRY(hipMemcpyAsync(dst->d_ist[out], id.data(), di * sizeof(double), hipMemcpyHostToDevice, dst->stream));
     HIP_TRY(hipMemcpyAsync(dst->d_oec[out], ed.data(), de * sizeof(double), hipMemcpyHostToDevice, dst->stream));
+    HIP_TRY(hipMemsetAsync(dst->d_tq[out], 0, (size_t)dst->nblk * 2 * sizeof(double), dst->stream));
     HIP_TRY(hipStreamSynchronize(dst->stream));
     dst->cur = out;
     dst->time = src->time;

@@ -228,6 +228,39 @@ int32_t cdc_solver_advance_dt(cdc_solver_t *solver, int32_t fold, double dt,
  */
 int32_t cdc_solver_resample_from(cdc_solver_t *dst, const cdc_solver_t *src);
 
+/*
+ * Per-block torque impulses on body 1 and 2 about the origin, [block][2]:
+ * the sum over stages of x_k x (-dp_k), dp_k the block's gas momentum change
+ * from body k's gravity (the grav items) and x_k body k's position at the
+ * stage's time, carried through the RK combine like integrated_source_terms.
+ * No counterpart in the reference (its torque is derived from time series
+ * offline); zeroed by cdc_solver_set_state.
+ */
+int32_t cdc_solver_get_torque(cdc_solver_t *solver, double *torque);
+
+/*
+ * State::time_series_sample (src/state.rs:135-146): the time and the sums
+ * over blocks (in block order) of integrated_source_terms [7][nv] and
+ * orbital_elements_change [7][4], plus the summed torque impulses [2].  Any
+ * output may be NULL.
+ */
+int32_t cdc_solver_time_series_sample(cdc_solver_t *solver, double *time, double *ist_total,
+                                      double *oec_total, double *torque_total);
+
+/*
+ * Device-resident copies of the committed state (conserved, ist, oec,
+ * torque, time, iteration): the fallback states the driver loop keeps
+ * (src/main.rs:148-200 clones the State per advance), copied at device
+ * bandwidth instead of through the host.  A snapshot fits solvers of the
+ * mesh and hydro kind it was created for.
+ */
+typedef struct cdc_snapshot cdc_snapshot_t;
+int32_t cdc_snapshot_create(const cdc_solver_t *solver, cdc_snapshot_t **out);
+void    cdc_snapshot_destroy(cdc_snapshot_t *snapshot);
+int32_t cdc_snapshot_time(const cdc_snapshot_t *snapshot, double *time);
+int32_t cdc_solver_save(cdc_solver_t *solver, cdc_snapshot_t *snapshot);
+int32_t cdc_solver_restore(cdc_solver_t *solver, const cdc_snapshot_t *snapshot);
+
 /* State::max_signal_speed (src/state.rs:121-133) of the current state. */
 int32_t cdc_solver_max_signal_speed(cdc_solver_t *solver, double *out);
 
@@ -292,8 +325,11 @@ int32_t cdc_ensemble_get_stats(cdc_ensemble_t *ensemble, int64_t *member_steps,
  * of the x- and y-face fluxes of (Sigma, Sx, Sy).  A corrected step is the
  * plain RK step followed by
  *   u_v += dt r_v,  r_v = -s_v ((Y_v[i,j] - Y_v[i-1,j]) / dx + (Y_{3+v}[i,j] - Y_{3+v}[i,j-1]) / dy),
- * a flux divergence (mass and momentum conserved).  Features: a0 ln Sigma,
- * a1 vx, a1 vy, a2 ln(p / Sigma) (0 for isothermal).  Training fits r to
+ * a flux divergence (mass and momentum conserved), with each face's three
+ * corrections scaled down so that a face moves at most kappa min(Sigma_L,
+ * Sigma_R) of mass, and that times the larger signal speed |v| + c_s of the
+ * two cells of momentum, per step.  Features: a0 ln Sigma,
+ * a1 vx, a1 vy, a2 ln(p / Sigma) (0 for isothermal), each clamped to [-16, 16].  Training fits r to
  * (u_truth - u_plain) / dt over stored samples with Adam on the loss
  * (1/P) sum_cells sum_v w_v (r_v - target_v)^2.
  * Parameters in canonical order: per layer W[cout][cin][3][3] then b[cout],
@@ -310,6 +346,8 @@ typedef struct cdc_closure_config {
     double   output_scale[3];    /* s_v */
     double   loss_weight[3];     /* w_v */
     double   init_output_gain;   /* multiplies the last layer's initial weights */
+    double   limiter;            /* kappa of the face limiter above; 0 = off */
+    double   mask_density;       /* faces with min(Sigma_L, Sigma_R) below it are not corrected */
 } cdc_closure_config_t;
 
 int32_t cdc_closure_create(const cdc_closure_config_t *config, cdc_closure_t **out);

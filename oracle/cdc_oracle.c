@@ -73,6 +73,7 @@ typedef struct {
     double *u;           /* [nblk][bs][bs][nv] */
     double *ist;         /* [nblk][7][nv] */
     double *oec;         /* [nblk][7][4] */
+    double *tq;          /* [nblk][2] torque impulse on body 1, 2 (extension) */
 } ostate;
 
 typedef struct oracle {
@@ -707,12 +708,14 @@ static void st_alloc(oracle_t *o, ostate *s) {
     s->u = (double *)calloc((size_t)o->nblk * o->ncell * o->nv, sizeof(double));
     s->ist = (double *)calloc((size_t)o->nblk * NITEM * o->nv, sizeof(double));
     s->oec = (double *)calloc((size_t)o->nblk * NITEM * 4, sizeof(double));
+    s->tq = (double *)calloc((size_t)o->nblk * 2, sizeof(double));
 }
 
 static void st_free(ostate *s) {
     free(s->u);
     free(s->ist);
     free(s->oec);
+    free(s->tq);
 }
 
 static void st_copy(const oracle_t *o, ostate *d, const ostate *s) {
@@ -722,6 +725,7 @@ static void st_copy(const oracle_t *o, ostate *d, const ostate *s) {
     memcpy(d->u, s->u, sizeof(double) * (size_t)o->nblk * o->ncell * o->nv);
     memcpy(d->ist, s->ist, sizeof(double) * (size_t)o->nblk * NITEM * o->nv);
     memcpy(d->oec, s->oec, sizeof(double) * (size_t)o->nblk * NITEM * 4);
+    memcpy(d->tq, s->tq, sizeof(double) * (size_t)o->nblk * 2);
 }
 
 oracle_t *oracle_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, const cdc_physics_t *phys,
@@ -773,6 +777,7 @@ void oracle_set_state(oracle_t *o, double time, int64_t num, int64_t den, const 
     else memset(o->st.ist, 0, sizeof(double) * (size_t)o->nblk * NITEM * o->nv);
     if (oec) memcpy(o->st.oec, oec, sizeof(double) * (size_t)o->nblk * NITEM * 4);
     else memset(o->st.oec, 0, sizeof(double) * (size_t)o->nblk * NITEM * 4);
+    memset(o->st.tq, 0, sizeof(double) * (size_t)o->nblk * 2);
 }
 
 void oracle_get_state(const oracle_t *o, double *time, int64_t *num, int64_t *den, double *u,
@@ -783,6 +788,11 @@ void oracle_get_state(const oracle_t *o, double *time, int64_t *num, int64_t *de
     if (u) memcpy(u, o->st.u, sizeof(double) * (size_t)o->nblk * o->ncell * o->nv);
     if (ist) memcpy(ist, o->st.ist, sizeof(double) * (size_t)o->nblk * NITEM * o->nv);
     if (oec) memcpy(oec, o->st.oec, sizeof(double) * (size_t)o->nblk * NITEM * 4);
+}
+
+/* Per-block torque impulses on body 1 and 2 ([nblk][2]; extension). */
+void oracle_get_torque(const oracle_t *o, double *tq) {
+    memcpy(tq, o->st.tq, sizeof(double) * (size_t)o->nblk * 2);
 }
 
 /* State::max_signal_speed, src/state.rs:121-133 (max is order independent) */
@@ -1007,6 +1017,13 @@ static int block_update(oracle_t *o, int b, const ostate *in, ostate *out, doubl
         for (int k = 0; k < 4; ++k)
             out->oec[((size_t)b * NITEM + it) * 4 + k] = in->oec[((size_t)b * NITEM + it) * 4 + k] + de[it].e[k];
     }
+    /* torque impulse on body k about the origin: x_k x (-dp), dp the gas
+       momentum change from body k's gravity (no reference counterpart) */
+    for (int k = 0; k < 2; ++k) {
+        const vec4 *g = &ds.it[CDC_ITEM_GRAV1 + k];
+        const double tau = tbs->p[k].y * g->v[1] - tbs->p[k].x * g->v[2];
+        out->tq[(size_t)b * 2 + k] = in->tq[(size_t)b * 2 + k] + tau;
+    }
     return 0;
 }
 
@@ -1102,6 +1119,7 @@ static void weighted_average(oracle_t *o, ostate *s1, int64_t bn, int64_t bd, co
     for (size_t k = 0; k < ni; ++k) s1->ist[k] = s1->ist[k] * c1 + s0->ist[k] * bf;
     size_t ne = (size_t)o->nblk * NITEM * 4;
     for (size_t k = 0; k < ne; ++k) s1->oec[k] = s1->oec[k] * c1 + s0->oec[k] * bf;
+    for (size_t k = 0; k < (size_t)o->nblk * 2; ++k) s1->tq[k] = s1->tq[k] * c1 + s0->tq[k] * bf;
     s1->time = s1->time * c1 + s0->time * bf;
     int64_t on, od;
     rat_wavg(s1->num, s1->den, s0->num, s0->den, bn, bd, &on, &od);

@@ -16,6 +16,8 @@ Net (grid arrays are [i][j][channel], i the x index, j the y index, periodic):
 Correction of one step of length dt (v = 0, 1, 2: Sigma, Sx, Sy):
   r_v = -s_v ((Y_v[i, j] - Y_v[i-1, j]) / dx + (Y_{3+v}[i, j] - Y_{3+v}[i, j-1]) / dy)
   u_v += dt r_v
+with each face's three corrections scaled by a positivity limiter when one
+is set (`limiter`); training uses the unlimited r.
 Training loss on a sample (features of the base state, target T):
   L = (1/P) sum_{cells, v} w_v (r_v - T_v)^2
 """
@@ -59,7 +61,7 @@ def pack(layers):
 def features(u_global, nv, gamma, scales):
     """u_global: [N][N][nv] conserved.  Channels: a0 ln Sigma, a1 vx, a1 vy,
     a2 ln(p / Sigma) (Euler; 0 for isothermal).  Arguments of the logs are
-    floored at 1e-30."""
+    floored at 1e-30 and every feature is clamped to [-16, 16]."""
     a0, a1, a2 = scales[0], scales[1], scales[2]
     sig = u_global[..., 0]
     vx = u_global[..., 1] / sig
@@ -71,7 +73,7 @@ def features(u_global, nv, gamma, scales):
     if nv == 4:
         p = (u_global[..., 3] - 0.5 * sig * (vx * vx + vy * vy)) * (gamma - 1.0)
         f[..., 3] = a2 * np.log(np.maximum(p / sig, 1e-30))
-    return f
+    return np.clip(f, -16.0, 16.0)
 
 
 def conv(x, W, b=None):
@@ -124,11 +126,42 @@ def forward(layers, x0, rnd=None):
     return acts          # [X0, A1, A2, A3, Y]
 
 
-def correction(Y, s, dx, dy):
-    """r_v, v = 0..2, from the 6 output channels."""
+def limiter(Y, s, dx, dy, dt, u, kappa, nv=4, gamma=5.0 / 3.0, mask=0.0):
+    """Face factors (x+, y+ face of each cell): the largest lambda <= 1 with
+    dt |s_0 Y_Sigma| / h <= kappa min(Sigma_L, Sigma_R) and, for the momentum
+    channels, dt |s_v Y_v| / h <= kappa min(Sigma_L, Sigma_R) max(a_L, a_R),
+    a = |v| + c_s the cells' signal speed (u: [N][N][nv] conserved); faces
+    where min(Sigma_L, Sigma_R) < mask get 0."""
+    lam = np.ones(Y.shape[:2] + (2,))
+    sig = u[..., 0]
+    for d, axis in enumerate((0, 1)):
+        lam[..., d] = np.where(np.minimum(sig, np.roll(sig, -1, axis=axis)) < mask, 0.0, 1.0)
+    if kappa <= 0:
+        return lam
+    vx, vy = u[..., 1] / sig, u[..., 2] / sig
+    c2 = np.zeros_like(sig)
+    if nv == 4:
+        c2 = gamma * (u[..., 3] - 0.5 * sig * (vx * vx + vy * vy)) * (gamma - 1.0) / sig
+    a = np.sqrt(vx * vx + vy * vy) + np.sqrt(np.maximum(c2, 0.0))
+    for d, (axis, h) in enumerate(((0, dx), (1, dy))):
+        cap = kappa * np.maximum(np.minimum(sig, np.roll(sig, -1, axis=axis)), 0.0)
+        capm = cap * np.maximum(a, np.roll(a, -1, axis=axis))
+        l = lam[..., d].copy()
+        for v in range(3):
+            m = dt * np.abs(s[v] * Y[..., 3 * d + v]) / h
+            c = cap if v == 0 else capm
+            l = np.where(m > c, np.minimum(l, c / np.where(m > c, m, 1.0)), l)
+        lam[..., d] = l
+    return lam
+
+
+def correction(Y, s, dx, dy, lam=None):
+    """r_v, v = 0..2, from the 6 output channels (face factors lam optional)."""
     r = np.zeros(Y.shape[:2] + (3,))
+    lx = 1.0 if lam is None else lam[..., 0]
+    ly = 1.0 if lam is None else lam[..., 1]
     for v in range(3):
-        fx, fy = Y[..., v], Y[..., 3 + v]
+        fx, fy = lx * Y[..., v], ly * Y[..., 3 + v]
         r[..., v] = -s[v] * ((fx - np.roll(fx, 1, axis=0)) / dx + (fy - np.roll(fy, 1, axis=1)) / dy)
     return r
 

@@ -33,6 +33,7 @@ def load():
         "oracle_get_state": (None, [c_void_p, D, POINTER(c_int64), POINTER(c_int64), c_void_p, c_void_p,
                                     c_void_p]),
         "oracle_advance": (c_int, [c_void_p, c_int, D, c_void_p]),
+        "oracle_get_torque": (None, [c_void_p, c_void_p]),
         "oracle_max_signal_speed": (c_double, [c_void_p]),
         "oracle_set_threads": (None, [c_int]),
         "oracle_get_threads": (c_int, []),
@@ -126,6 +127,12 @@ class Oracle:
         t, n, d = c_double(), c_int64(), c_int64()
         self.lib.oracle_get_state(self.h, byref(t), byref(n), byref(d), _p(u), _p(ist), _p(oec))
         return {"time": t.value, "iteration": (n.value, d.value), "conserved": u, "ist": ist, "oec": oec}
+
+    def get_torque(self):
+        """Per-block torque impulses on body 1 and 2, [nblk][2]."""
+        tq = np.empty((self.nblk, 2))
+        self.lib.oracle_get_torque(self.h, _p(tq))
+        return tq
 
     def advance(self, fold=1):
         from ctypes import create_string_buffer

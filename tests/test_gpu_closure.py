@@ -216,3 +216,32 @@ def test_correction_conserves_mass_and_momentum():
     r = C.correction(Y.astype(np.float64), [1e-6] * 3, dx, dx)
     np.testing.assert_allclose(to_global(d, mesh.num_blocks, mesh.block_size)[..., :3], dt * r,
                                rtol=1e-6, atol=1e-6 * np.max(np.abs(dt * r)))
+
+
+def test_limited_correction_matches_oracle_and_conserves():
+    """Large corrections with the limiter engaged: the applied change is the
+    oracle's limited correction, still a flux divergence, and no face moves
+    more than kappa of a neighbour's density."""
+    mesh, hydro, ph, u0 = setup()
+    N = mesh.num_blocks * mesh.block_size
+    kappa = 0.1
+    net = Closure(N, mesh.domain_radius, limiter=kappa)
+    net.set_params(rand_params(8))
+    net.set_scales(output_scale=[1e-2, 1e-2, 1e-2])
+    a, b = Solver(mesh, hydro, ph, u0), Solver(mesh, hydro, ph, u0)
+    Y = net.forward(a).astype(np.float64)
+    dt = net.advance(a)
+    b.advance(1)
+    ua, ub = a.get_state()["conserved"], b.get_state()["conserved"]
+    dx = 2 * mesh.domain_radius / N
+    ug = to_global(ub, mesh.num_blocks, mesh.block_size)
+    sig = ug[..., 0]
+    lam = C.limiter(Y, [1e-2] * 3, dx, dx, dt, ug, kappa, 4, hydro.gamma_law_index)
+    assert lam.min() < 0.5                       # the limiter acted
+    r = C.correction(Y, [1e-2] * 3, dx, dx, lam)
+    d = to_global(ua - ub, mesh.num_blocks, mesh.block_size)
+    np.testing.assert_allclose(d[..., :3], dt * r, rtol=1e-6, atol=1e-9 * np.max(np.abs(dt * r)))
+    for v in range(3):
+        assert abs(d[..., v].sum()) <= 1e-12 * np.abs(d[..., v]).sum()
+    # each cell loses at most 4 kappa of its density through its four faces
+    assert np.all(d[..., 0] >= -4 * kappa * sig * (1 + 1e-9))

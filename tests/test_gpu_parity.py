@@ -37,6 +37,7 @@ def check_states(g, o, tol=1e-12, ist_tol=1e-10, one_body=False):
     err = H.field_err(sg["conserved"], so["conserved"])
     assert err <= tol, f"state error {err:.3e}"
     check_items(g, sg, so, ist_tol)
+    check_torque(g, o, ist_tol, one_body)
     d = np.abs(sg["oec"] - so["oec"])
     # the binary's elements are O(1): absolute bar.  one_body's elements are
     # (1e-12, 1, 1, 0): mean motion 1e18, so a one-ulp difference in a stage
@@ -47,6 +48,23 @@ def check_states(g, o, tol=1e-12, ist_tol=1e-10, one_body=False):
     assert np.all(d <= bar), (f"orbital element change error {d[k]:.3e} at block {k[0]} item {k[1]} "
                               f"element {k[2]}: gpu {sg['oec'][k]:.17e} oracle {so['oec'][k]:.17e}")
     return err
+
+
+def check_torque(g, o, tol=1e-10, one_body=False):
+    """Per-block torque impulses: relative to the largest block's.  With
+    one_body the bodies sit 5e-13 from the origin at a phase scrambled by
+    the mean motion 1e18 (see check_states), so only the size is compared."""
+    tg, to = g.torque(), o.get_torque()
+    scale = max(np.abs(to).max(), 1e-300)
+    if one_body:
+        # |x_k| = 5e-13: the torque is bounded by that times the block's kick
+        kick = np.abs(g.get_state()["ist"][:, 2:4, 1:3]).max()
+        assert np.all(np.isfinite(tg)) and np.abs(tg).max() <= 1e-11 * kick
+        return
+    err = np.abs(tg - to).max() / scale
+    assert err <= tol, f"torque error {err:.3e}"
+    ts = g.time_series_sample()
+    np.testing.assert_allclose(ts["torque"], tg.sum(axis=0), rtol=1e-12, atol=1e-14 * scale)
 
 
 def check_items(g, sg, so, ist_tol=1e-10):

@@ -442,3 +442,37 @@ def test_negative_density_error_is_transactional():
     s = o.get_state()
     assert s["time"] == 0.5 and s["iteration"] == (3, 1)
     np.testing.assert_array_equal(s["conserved"], bad)
+
+
+# ---- torque impulses (extension): central-force identity ------------------------
+
+def test_torque_is_minus_the_sum_of_cell_torques():
+    """Body k's torque impulse x_k x (-F_k) equals minus the sum over cells of
+    x x f_k, because each f_k points along x - x_k: checked against a
+    per-cell reduction of the Plummer force on one RK1 step."""
+    from tests import helpers as H
+    mesh, hydro, ph = H.small("iso", nb=2, bs=8, R=4.0, rk_order="RK1", binary_eccentricity=0.4,
+                              binary_mass_ratio=0.6, buffer_rate=0.0, sink_rate=0.0)
+    u0 = H.initial(mesh, hydro, ph)
+    o = H.oracle_for(mesh, hydro, ph, u0)
+    dt = o.advance(1)
+    tq = o.get_torque()
+    orb = O.orbital_state(ph.to_c(), 0.0)
+    mc = mesh.to_c()
+    rs = ph.softening()
+    nb, bs = mesh.num_blocks, mesh.block_size
+    dA = mesh.cell_spacing() ** 2
+    for b in range(nb * nb):
+        xv, yv = O.block_vertices(mc, b // nb), O.block_vertices(mc, b % nb)
+        xc, yc = 0.5 * (xv[:-1] + xv[1:]), 0.5 * (yv[:-1] + yv[1:])
+        want = np.zeros(2)
+        mag = np.zeros(2)
+        for i in range(bs):
+            for j in range(bs):
+                sig = u0[b, i, j, 0]
+                _, acc = O.gravity(orb, xc[i], yc[j], rs)
+                for k in range(2):
+                    fx, fy = sig * acc[2 * k] * dt * dA, sig * acc[2 * k + 1] * dt * dA
+                    want[k] -= xc[i] * fy - yc[j] * fx
+                    mag[k] += abs(xc[i] * fy) + abs(yc[j] * fx)
+        np.testing.assert_allclose(tq[b], want, rtol=0, atol=1e-12 * mag.max())
