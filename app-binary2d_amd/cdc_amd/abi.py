@@ -165,6 +165,8 @@ SIGNATURES = {
     "cdc_solver_get_timing": (c_int32, [c_void_p, POINTER(c_int64), POINTER(c_double), POINTER(c_double)]),
     "cdc_solver_set_tile": (c_int32, [c_void_p, c_int32, c_int32]),
     "cdc_solver_get_tile": (c_int32, [c_void_p, POINTER(c_int32), POINTER(c_int32)]),
+    "cdc_solver_set_kernel": (c_int32, [c_void_p, c_int32, c_int32, c_int32]),
+    "cdc_solver_get_kernel": (c_int32, [c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
     "cdc_solver_autotune": (c_int32, [c_void_p, c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_double)]),
     "cdc_ensemble_create": (c_int32, [POINTER(c_void_p), c_int32, POINTER(c_void_p)]),
     "cdc_ensemble_destroy": (None, [c_void_p]),

@@ -206,6 +206,15 @@ class Solver:
         _check(self.lib, self.lib.cdc_solver_get_tile(self.handle, byref(tx), byref(ty)))
         return tx.value, ty.value
 
+    def set_kernel(self, tx=0, ty=0, threads=0):
+        _check(self.lib, self.lib.cdc_solver_set_kernel(self.handle, int(tx), int(ty), int(threads)))
+
+    def get_kernel(self):
+        """(tx, ty, threads) of the stage kernel in use."""
+        tx, ty, nt = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(self.lib, self.lib.cdc_solver_get_kernel(self.handle, byref(tx), byref(ty), byref(nt)))
+        return tx.value, ty.value, nt.value
+
     def autotune(self, reps=3):
         """Time every supported tile shape on the current state; keep the fastest.
         Returns (tx, ty, ms per stage launch)."""

@@ -147,7 +147,7 @@ int32_t cdc_ensemble_create(cdc_solver_t *const *members, int32_t n, cdc_ensembl
     // divides every block size (block sizes are multiples of 8)
     const cdc::TileShape ts = cdc::tile_shape_for(bs_gcd, m0->nv);
     for (int i = 0; i < n; ++i) {
-        const int rc = cdc_solver_set_tile(members[i], ts.tx, ts.ty);
+        const int rc = cdc_solver_set_kernel(members[i], ts.tx, ts.ty, ts.nt);
         if (rc) return rc;
     }
     cdc_ensemble *e = new cdc_ensemble();
@@ -217,7 +217,7 @@ int32_t cdc_ensemble_advance(cdc_ensemble_t *e, const int32_t *active, double *d
     std::vector<int32_t> tile_begin((size_t)nrun + 1, 0), block_begin((size_t)nrun + 1, 0);
     for (int r = 0; r < nrun; ++r) {
         cdc_solver *m = e->members[(size_t)runs[r]];
-        if (m->ts.tx != e->ts.tx || m->ts.ty != e->ts.ty)
+        if (m->ts.tx != e->ts.tx || m->ts.ty != e->ts.ty || m->ts.nt != e->ts.nt)
             return set_msg(CDC_ERR_INVALID_ARGUMENT, "a member's tile shape was changed after joining the ensemble");
         if (m->phys.rk_order != e->rk)
             return set_msg(CDC_ERR_INVALID_ARGUMENT, "a member's rk_order was changed after joining the ensemble");

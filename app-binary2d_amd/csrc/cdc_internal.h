@@ -77,7 +77,7 @@ struct StageArgs {
 
 // Tile shape chosen for a block size; see DESIGN.md for the LDS budget.
 struct TileShape {
-    int tx, ty, nt;
+    int tx, ty, nt;   // tile of tx x ty cells, nt threads
 };
 TileShape tile_shape_for(int bs, int nv);
 int tile_shapes(const TileShape **out);

@@ -62,6 +62,15 @@ int nsum_for(int nv) { return nv == 4 ? SumMap<4>::n : SumMap<3>::n; }
 
 namespace dev {
 
+// Device memory reached through a pointer loaded from the per-run argument
+// array is not known to the compiler to be global memory; these casts let it
+// use global (not flat) loads and stores, which wait on vmcnt alone instead
+// of also on the LDS / scalar counter.
+typedef __attribute__((address_space(1))) const double gcdouble;
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ __forceinline__ gcdouble *gptr(const double *p) { return (gcdouble *)p; }
+__device__ __forceinline__ gdouble *gptr(double *p) { return (gdouble *)p; }
+
 // ---- refined hardware estimates -------------------------------------------
 
 __device__ __forceinline__ double frcp(double x) {
@@ -187,7 +196,7 @@ __device__ __forceinline__ void face_flux(const StageArgs &A, int axis, const do
                                           const double *GX, const double *GY, int eL, int eR, int rL,
                                           int rR, double fxp, double fyp, double *F) {
     const StageConsts &k = A.k;
-    const double *GN = axis == 0 ? GX : GY;
+    const double *GN = axis ? GY : GX;
     double pL[NV], pR[NV], pl[NV], pr[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
@@ -232,21 +241,20 @@ __device__ __forceinline__ void face_flux(const StageArgs &A, int axis, const do
         const double offL = gxvyL + gyvxL, offR = gxvyR + gyvxR;
         const double divL = gxvxL + gyvyL, divR = gxvxR + gyvyR;
         const double nsL = pL[0] * nuL, nsR = pR[0] * nuR;
-        if (axis == 0) {
-            const double sxxL = nsL * ((4.0 / 3.0) * gxvxL - (2.0 / 3.0) * gyvyL + k.lam * divL);
-            const double sxxR = nsR * ((4.0 / 3.0) * gxvxR - (2.0 / 3.0) * gyvyR + k.lam * divR);
-            tau_x = 0.5 * (sxxL + sxxR);
-            tau_y = 0.5 * (nsL * offL + nsR * offR);
-        } else {
-            const double syyL = nsL * ((4.0 / 3.0) * gyvyL - (2.0 / 3.0) * gxvxL + k.lam * divL);
-            const double syyR = nsR * ((4.0 / 3.0) * gyvyR - (2.0 / 3.0) * gxvxR + k.lam * divR);
-            tau_x = 0.5 * (nsL * offL + nsR * offR);
-            tau_y = 0.5 * (syyL + syyR);
-        }
+        // normal-normal component (sxx on an x-face, syy on a y-face) and the
+        // off-diagonal one, by selects so that both face kinds share one path
+        const double gnnL = axis ? gyvyL : gxvxL, gttL = axis ? gxvxL : gyvyL;
+        const double gnnR = axis ? gyvyR : gxvxR, gttR = axis ? gxvxR : gyvyR;
+        const double snnL = nsL * ((4.0 / 3.0) * gnnL - (2.0 / 3.0) * gttL + k.lam * divL);
+        const double snnR = nsR * ((4.0 / 3.0) * gnnR - (2.0 / 3.0) * gttR + k.lam * divR);
+        const double tau_n = 0.5 * (snnL + snnR);
+        const double tau_o = 0.5 * (nsL * offL + nsR * offR);
+        tau_x = axis ? tau_o : tau_n;
+        tau_y = axis ? tau_n : tau_o;
     }
 
     // HLLE (Toro ch. 10), outer wave speeds v_n -/+ c_s
-    const double vnl = pl[1 + axis], vnr = pr[1 + axis];
+    const double vnl = axis ? pl[2] : pl[1], vnr = axis ? pr[2] : pr[1];
     double ul[NV], ur[NV], fl[NV], fr[NV], csl, csr, ppl, ppr;
     ul[0] = pl[0]; ul[1] = pl[0] * pl[1]; ul[2] = pl[0] * pl[2];
     ur[0] = pr[0]; ur[1] = pr[0] * pr[1]; ur[2] = pr[0] * pr[2];
@@ -270,8 +278,10 @@ __device__ __forceinline__ void face_flux(const StageArgs &A, int axis, const do
         fl[v] = ul[v] * vnl;
         fr[v] = ur[v] * vnr;
     }
-    fl[1 + axis] += ppl;
-    fr[1 + axis] += ppr;
+    fl[1] = axis ? fl[1] : fl[1] + ppl;
+    fl[2] = axis ? fl[2] + ppl : fl[2];
+    fr[1] = axis ? fr[1] : fr[1] + ppr;
+    fr[2] = axis ? fr[2] + ppr : fr[2];
     const double ap = fmax(fmax(vnl + csl, vnr + csr), 0.0);
     const double am = fmin(fmin(vnl - csl, vnr - csr), 0.0);
     const double inv = frcp(ap - am);
@@ -423,6 +433,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
     const int li0 = (lt / tpby) * TX, lj0 = (lt - (lt / tpby) * tpby) * TY;
     const int gi0 = bi * bs + li0, gj0 = bj * bs + lj0;
     const size_t NN = (size_t)N * N;
+    dev::gcdouble *guc = dev::gptr(A.uc), *gu0 = dev::gptr(A.u0), *gub = dev::gptr(A.ub), *gaux = dev::gptr(A.aux);
+    dev::gdouble *guo = dev::gptr(A.uo);
     const dev::Axis axx = dev::block_axis(A.R, A.L, bi, bs);
     const dev::Axis axy = dev::block_axis(A.R, A.L, bj, bs);
     const bool alpha = A.k.viscosity_model == CDC_VISC_ALPHA;
@@ -450,7 +462,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
             gj = gj < 0 ? gj + N : (gj >= N ? gj - N : gj);
             const size_t g = (size_t)gi * N + gj;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) ue[q][v] = A.uc[v * NN + g];
+            for (int v = 0; v < NV; ++v) ue[q][v] = guc[v * NN + g];
         }
     }
     double pu0[CPT][NV], pub[CPT][NV], pbr[CPT], pom[CPT];
@@ -461,11 +473,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
         const size_t g = (size_t)(gi0 + ci) * N + (gj0 + cj);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            pu0[q][v] = A.u0[v * NN + g];
-            pub[q][v] = A.read_base ? A.ub[v * NN + g] : 0.0;
+            pu0[q][v] = gu0[v * NN + g];
+            pub[q][v] = A.read_base ? gub[v * NN + g] : 0.0;
         }
-        pbr[q] = A.aux[g];
-        pom[q] = A.k.cooling_model == CDC_COOLING_BETA ? A.aux[NN + g] : 0.0;
+        pbr[q] = gaux[g];
+        pom[q] = A.k.cooling_model == CDC_COOLING_BETA ? gaux[NN + g] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < KE; ++q) {
@@ -507,23 +519,23 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
     __syncthreads();
 
     // ---- phase 3: face fluxes ----------------------------------------------
+    // x-faces f < NFX, then y-faces; one code path for both kinds (the axis
+    // enters by selects), so a wave holding both does not run both.
     double fl[KF][NV];
 #pragma unroll
     for (int q = 0; q < KF; ++q) {
         const int f = tid + q * NT;
-        if (f < NFX) {
-            const int fi = f / TY, fj = f - (f / TY) * TY;
-            const int eL = (fi + 1) * EY + fj + 2;
-            const int rL = fi * RY + fj + 1;
-            dev::face_flux<NV, EXY, RXY>(A, 0, P, C, GX, GY, eL, eL + EY, rL, rL + RY,
-                                         axx.v(li0 + fi), axy.c(lj0 + fj), fl[q]);
-        } else if (f < NF) {
-            const int g = f - NFX;
-            const int fi = g / (TY + 1), fj = g - (g / (TY + 1)) * (TY + 1);
-            const int eL = (fi + 2) * EY + fj + 1;
-            const int rL = (fi + 1) * RY + fj;
-            dev::face_flux<NV, EXY, RXY>(A, 1, P, C, GX, GY, eL, eL + 1, rL, rL + 1,
-                                         axx.c(li0 + fi), axy.v(lj0 + fj), fl[q]);
+        if (f < NF) {
+            const int isy = f >= NFX;
+            const int g = isy ? f - NFX : f;
+            const int fi = isy ? g / (TY + 1) : g / TY;
+            const int fj = g - fi * (isy ? TY + 1 : TY);
+            const int eL = isy ? (fi + 2) * EY + fj + 1 : (fi + 1) * EY + fj + 2;
+            const int rL = isy ? (fi + 1) * RY + fj : fi * RY + fj + 1;
+            const int de = isy ? 1 : EY, dr = isy ? 1 : RY;
+            const double fxp = isy ? axx.c(li0 + fi) : axx.v(li0 + fi);
+            const double fyp = isy ? axy.v(lj0 + fj) : axy.c(lj0 + fj);
+            dev::face_flux<NV, EXY, RXY>(A, isy, P, C, GX, GY, eL, eL + de, rL, rL + dr, fxp, fyp, fl[q]);
         }
     }
     __syncthreads();
@@ -555,7 +567,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
         double uc[NV], du[NV];
         const double *u0 = pu0[q], *ub = pub[q];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) uc[v] = A.uc[v * NN + g];
+        for (int v = 0; v < NV; ++v) uc[v] = guc[v * NN + g];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const double *fx = FX + v * NFX + ci * TY + cj;
@@ -588,7 +600,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
             else total = total + it[CDC_ITEM_FAKE_MASS][v];
             const double u1 = uc[v] + du[v] + total;
             out[v] = A.read_base ? u1 * A.c1 + ub[v] * A.c0 : u1;
-            A.uo[v * NN + g] = out[v];
+            guo[v * NN + g] = out[v];
         }
 #pragma unroll
         for (int s = 0; s < NS; ++s) acc[s] += it[SumMap<NV>::item(s)][SumMap<NV>::comp(s)];
@@ -633,7 +645,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
         } else {
             for (int j = 1; j < CH; ++j) t += q[j];
         }
-        A.tile_sums[(size_t)tile * NR + s] = t;
+        dev::gptr(A.tile_sums)[(size_t)tile * NR + s] = t;
     }
     const int sk = s_key;
     if (sk == INT_MAX) {
@@ -697,7 +709,7 @@ __device__ __forceinline__ void finalize_block(const StageArgs &A, const int b, 
     const int tid = threadIdx.x;
     if (tid < NS) {
         double t = 0.0;
-        for (int j = 0; j < tpb; ++j) t += A.tile_sums[((size_t)b * tpb + j) * NR + tid];
+        for (int j = 0; j < tpb; ++j) t += dev::gptr(A.tile_sums)[((size_t)b * tpb + j) * NR + tid];
         s_ds[SumMap<NV>::item(tid) * NV + SumMap<NV>::comp(tid)] = t * A.area;
     } else if (tid < NS + 7 * NV) {
         // structurally zero components
@@ -921,8 +933,7 @@ TileShape tile_shape_for(int bs, int nv) {
 }
 
 template <int NV, int TX, int TY, int NT>
-static hipError_t launch_batch_t(const StageArgs *args, const int32_t *begin, int nruns, int ntiles,
-                                 hipStream_t s) {
+static hipError_t launch_batch_t(const StageArgs *args, const int32_t *begin, int nruns, int ntiles, hipStream_t s) {
     hipLaunchKernelGGL((stage_kernel_batch<NV, TX, TY, NT>), dim3(ntiles), dim3(NT), 0, s, args, begin, nruns);
     return hipGetLastError();
 }

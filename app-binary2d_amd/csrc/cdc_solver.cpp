@@ -648,15 +648,18 @@ int32_t cdc_solver_get_state(cdc_solver_t *s, double *time, int64_t *num, int64_
     return CDC_OK;
 }
 
-int32_t cdc_solver_set_tile(cdc_solver_t *s, int32_t tx, int32_t ty) {
+int32_t cdc_solver_set_kernel(cdc_solver_t *s, int32_t tx, int32_t ty, int32_t threads) {
     if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
     cdc::TileShape t = cdc::tile_shape_for(s->bs, s->nv);
     if (tx != 0 || ty != 0) {
         const cdc::TileShape *shapes;
         const int n = cdc::tile_shapes(&shapes);
         bool found = false;
-        for (int i = 0; i < n; ++i)
-            if (shapes[i].tx == tx && shapes[i].ty == ty) { t = shapes[i]; found = true; }
+        for (int i = 0; i < n && !found; ++i)
+            if (shapes[i].tx == tx && shapes[i].ty == ty && (threads == 0 || shapes[i].nt == threads)) {
+                t = shapes[i];
+                found = true;
+            }
         if (!found) return set_msg(CDC_ERR_INVALID_ARGUMENT, "unsupported tile shape");
         if (s->bs % tx != 0 || s->bs % ty != 0)
             return set_msg(CDC_ERR_INVALID_ARGUMENT, "tile shape must divide block_size");
@@ -666,6 +669,16 @@ int32_t cdc_solver_set_tile(cdc_solver_t *s, int32_t tx, int32_t ty) {
     s->tpb = (s->bs / t.tx) * (s->bs / t.ty);
     s->ntiles = s->nblk * s->tpb;
     s->amax_valid = false;   // the per-tile max-speed column was written with the old tiling
+    return CDC_OK;
+}
+
+int32_t cdc_solver_set_tile(cdc_solver_t *s, int32_t tx, int32_t ty) { return cdc_solver_set_kernel(s, tx, ty, 0); }
+
+int32_t cdc_solver_get_kernel(cdc_solver_t *s, int32_t *tx, int32_t *ty, int32_t *threads) {
+    if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
+    if (tx) *tx = s->ts.tx;
+    if (ty) *ty = s->ts.ty;
+    if (threads) *threads = s->ts.nt;
     return CDC_OK;
 }
 
@@ -738,7 +751,7 @@ int32_t cdc_solver_autotune(cdc_solver_t *s, int32_t reps, int32_t *tx, int32_t 
     for (auto &m : marks) (void)hipEventDestroy(m);
     if (rc) return rc;
     HIP_TRY(cdc::launch_set_int(s->d_err_flag, 0, s->stream));
-    rc = cdc_solver_set_tile(s, pick.tx, pick.ty);
+    rc = cdc_solver_set_kernel(s, pick.tx, pick.ty, pick.nt);
     if (rc) return rc;
     if (tx) *tx = pick.tx;
     if (ty) *ty = pick.ty;

@@ -288,18 +288,18 @@ def test_timing_counts_stage_launches():
     assert n == 6 and ms > 0 and cells == 64 * 64
 
 
-SHAPES = [(16, 32), (32, 16), (16, 16), (8, 32), (32, 8), (8, 8)]
+SHAPES = [(16, 32, 256), (32, 16, 256), (16, 16, 256), (8, 32, 256), (32, 8, 256), (8, 8, 64)]
 
 
 @pytest.mark.parametrize("kind", ["iso", "euler"])
-@pytest.mark.parametrize("shape", SHAPES, ids=[f"{a}x{b}" for a, b in SHAPES])
+@pytest.mark.parametrize("shape", SHAPES, ids=[f"{a}x{b}x{c}" for a, b, c in SHAPES])
 def test_every_tile_shape_matches_oracle(kind, shape):
     kw = dict(rk_order="RK3", viscosity={"alpha": 0.1}, binary_eccentricity=0.3, binary_mass_ratio=0.7)
     mesh, hydro, ph = H.small(kind, nb=2, bs=32, R=4.0, **kw)
     u0 = H.initial(mesh, hydro, ph)
     g, o = pair(mesh, hydro, ph, u0)
-    g.set_tile(*shape)
-    assert g.get_tile() == shape
+    g.set_kernel(*shape)
+    assert g.get_kernel() == shape and g.get_tile() == shape[:2]
     for _ in range(3):
         assert g.advance(1) == pytest.approx(o.advance(1), rel=1e-12)
         check_states(g, o)
@@ -312,7 +312,7 @@ def test_autotune_keeps_state_and_picks_a_valid_shape():
     g.advance(1); o.advance(1)
     before = g.get_state()
     tx, ty, ms = g.autotune(reps=2)
-    assert (tx, ty) in SHAPES and ms > 0 and g.get_tile() == (tx, ty)
+    assert g.get_kernel() in SHAPES and ms > 0 and g.get_tile() == (tx, ty)
     after = g.get_state()
     np.testing.assert_array_equal(before["conserved"], after["conserved"])
     np.testing.assert_array_equal(before["ist"], after["ist"])
