@@ -154,9 +154,10 @@ def run_to(step, solver, t_end, log=None, every=2000, min_dt_ratio=1e-2):
 
 
 def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, sample_every=100,
-                    max_samples=200, train_steps=4000, lr=1e-3, seed=5, limiter=0.1,
-                    variants=((0.1, 1.0),), log=print):
-    """Train a closure for the coarse grid on pairs from the fine workload,
+                    max_samples=200, train_steps=4000, lr=1e-3, seed=5, limiter=0.1, train_mask=0.05,
+                    variants=((0.1, 1.0, 0.05),), log=print):
+    """Train a closure for the coarse grid on pairs from the fine workload
+    (the loss over cells with density >= train_mask),
     then run plain fine, plain coarse and learned coarse runs over the
     held-out window [train_orbits, train_orbits + eval_orbits] from the same
     fine state; one learned run per (limiter kappa, damping, mask density)
@@ -175,7 +176,7 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
     N = cm.num_blocks * cm.block_size
     dxc = 2.0 * cm.domain_radius / N
     net = Closure(N, cm.domain_radius, max_samples=max_samples, seed=seed, init_output_gain=0.1,
-                  limiter=limiter)
+                  limiter=limiter, mask_density=train_mask)
 
     # training pairs along the fine run
     t_train = train_orbits * ORBIT

@@ -217,11 +217,14 @@ __global__ void __launch_bounds__(256) loss_kernel(const LossArgs a) {
         const int i = rr / a.n, j = rr - (rr / a.n) * a.n;
         const int ip = wrap(i + 1, a.n), jp = wrap(j + 1, a.n);
         uint16_t *g = a.gy + (size_t)p * C;
+        // cell weights (target channel 3) scale each cell's error
+        const size_t px = (size_t)(f * nn + ip * a.n + j), py = (size_t)(f * nn + i * a.n + jp);
+        const float c0 = a.t[(size_t)p * 4 + 3], cx = a.t[px * 4 + 3], cy = a.t[py * 4 + 3];
 #pragma unroll
         for (int v = 0; v < 3; ++v) {
-            const float e0 = corr_r(a, f, i, j, v) - a.t[(size_t)p * 4 + v];
-            const float ex = corr_r(a, f, ip, j, v) - a.t[(size_t)(f * nn + ip * a.n + j) * 4 + v];
-            const float ey = corr_r(a, f, i, jp, v) - a.t[(size_t)(f * nn + i * a.n + jp) * 4 + v];
+            const float e0 = c0 * (corr_r(a, f, i, j, v) - a.t[(size_t)p * 4 + v]);
+            const float ex = cx * (corr_r(a, f, ip, j, v) - a.t[px * 4 + v]);
+            const float ey = cy * (corr_r(a, f, i, jp, v) - a.t[py * 4 + v]);
             l += a.w[v] * e0 * e0;
             const float k = 2.0f * a.inv_p * a.w[v] * a.s[v];
             g[v] = f2bf(-k * a.idx * (e0 - ex));
@@ -306,10 +309,11 @@ __global__ void __launch_bounds__(256) features_kernel(const FeatArgs a) {
 
 struct TargetArgs {
     const double *plain, *truth;   // SoA [nv][N][N]
-    float *t;                      // [N*N][4]
-    float *part;                   // [nblocks][3] sums of t_v^2
+    const double *base;            // SoA state the features come from
+    float *t;                      // [N*N][4]: 3 targets and the cell weight
+    float *part;                   // [nblocks][3] weighted sums of t_v^2
     int N;
-    double inv_dt;
+    double inv_dt, mask;           // cell weight 1 where base Sigma >= mask, else 0
 };
 
 __global__ void __launch_bounds__(256) target_kernel(const TargetArgs a) {
@@ -318,12 +322,13 @@ __global__ void __launch_bounds__(256) target_kernel(const TargetArgs a) {
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
     float sq[3] = {0.0f, 0.0f, 0.0f};
     if (p < NN) {
+        const float wc = a.base[p] >= a.mask ? 1.0f : 0.0f;
         for (int v = 0; v < 3; ++v) {
             const float d = (float)((a.truth[v * NN + p] - a.plain[v * NN + p]) * a.inv_dt);
             a.t[p * 4 + v] = d;
-            sq[v] = d * d;
+            sq[v] = wc * d * d;
         }
-        a.t[p * 4 + 3] = 0.0f;
+        a.t[p * 4 + 3] = wc;
     }
     for (int v = 0; v < 3; ++v) red[v][threadIdx.x] = sq[v];
     __syncthreads();
@@ -784,6 +789,8 @@ int32_t cdc_closure_add_sample(cdc_closure_t *c, const cdc_solver_t *base, const
     TargetArgs a{};
     a.plain = plain->d_u[plain->cur];
     a.truth = truth->d_u[truth->cur];
+    a.base = base->d_u[base->cur];
+    a.mask = c->cfg.mask_density;
     a.t = c->d_tgt + (size_t)k * c->P * 4;
     a.part = c->d_part_l;
     a.N = c->n;

@@ -171,9 +171,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tile", default=None, help="TXxTY stage-kernel tile (skips the autotuner)")
     ap.add_argument("--train-orbits", type=float, default=2.0)
-    ap.add_argument("--eval-orbits", type=float, default=1.0)
+    ap.add_argument("--eval-orbits", type=float, default=2.0)
     ap.add_argument("--train-steps", type=int, default=4000)
-    ap.add_argument("--variants", default="0.1:1:0,0.05:1:0.05,0.02:0.5:0.05",
+    ap.add_argument("--variants", default="0.1:1:0.05,0.05:1:0.05,0.02:0.5:0.05",
                     help="learned runs as kappa:damping:mask_density,...")
     args = ap.parse_args()
     if args.gpus != 1:

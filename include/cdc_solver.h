@@ -335,7 +335,8 @@ int32_t cdc_ensemble_get_stats(cdc_ensemble_t *ensemble, int64_t *member_steps,
  * two cells of momentum, per step.  Features: a0 ln Sigma,
  * a1 vx, a1 vy, a2 ln(p / Sigma) (0 for isothermal), each clamped to [-16, 16].  Training fits r to
  * (u_truth - u_plain) / dt over stored samples with Adam on the loss
- * (1/P) sum_cells sum_v w_v (r_v - target_v)^2.
+ * (1/P) sum_cells c sum_v w_v (r_v - target_v)^2, c = 1 where the base density
+ * is at least mask_density (the cells the correction is applied to), else 0.
  * Parameters in canonical order: per layer W[cout][cin][3][3] then b[cout],
  * layers (4,32), (32,32), (32,32), (32,6) as (cin, cout).
  */

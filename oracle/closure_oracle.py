@@ -18,8 +18,9 @@ Correction of one step of length dt (v = 0, 1, 2: Sigma, Sx, Sy):
   u_v += dt r_v
 with each face's three corrections scaled by a positivity limiter when one
 is set (`limiter`); training uses the unlimited r.
-Training loss on a sample (features of the base state, target T):
-  L = (1/P) sum_{cells, v} w_v (r_v - T_v)^2
+Training loss on a sample (features of the base state, target T, cell
+weights c = 1 where the base density is at least the mask density, else 0):
+  L = (1/P) sum_{cells, v} w_v c^2 (r_v - T_v)^2
 """
 import numpy as np
 
@@ -166,16 +167,21 @@ def correction(Y, s, dx, dy, lam=None):
     return r
 
 
-def loss_and_grads(layers, x0, T, s, w, dx, dy, rnd=None):
+def loss_and_grads(layers, x0, T, s, w, dx, dy, rnd=None, cw=None):
     """Loss and canonical flat gradient (analytic backward).  rnd (optional)
     rounds the stored activations, dL/dY and every propagated activation
-    gradient, emulating a device that keeps them in bf16."""
+    gradient, emulating a device that keeps them in bf16.  cw (optional,
+    [N][N]) weights each cell's error (the loss over masked cells)."""
     acts = forward(layers, x0, rnd)
     Y = acts[-1]
     P = Y.shape[0] * Y.shape[1]
     e = correction(Y, s, dx, dy) - T[..., :3]
+    if cw is not None:
+        e = e * np.asarray(cw)[..., None]
     L = float(np.sum(np.asarray(w)[None, None, :] * e * e) / P)
-    g = 2.0 * np.asarray(w) * e / P                 # dL/dr
+    g = 2.0 * np.asarray(w) * e / P                 # dL/dr (times the cell weight)
+    if cw is not None:
+        g = g * np.asarray(cw)[..., None]
     gY = np.zeros_like(Y)
     for v in range(3):
         # r_v[i,j] = -s_v (fx[i,j] - fx[i-1,j]) / dx - ...

@@ -103,14 +103,18 @@ def make_sample(net, mesh, hydro, ph, u0, seed=9):
     return base, to_global(T, mesh.num_blocks, mesh.block_size), dt
 
 
-def test_sample_gradient_matches_oracle():
+@pytest.mark.parametrize("mask", [0.0, 1e-3], ids=["all-cells", "masked"])
+def test_sample_gradient_matches_oracle(mask):
     mesh, hydro, ph, u0 = setup()
     N = mesh.num_blocks * mesh.block_size
-    net = Closure(N, mesh.domain_radius, max_samples=2)
+    net = Closure(N, mesh.domain_radius, max_samples=2, mask_density=mask)
     base, T, dt = make_sample(net, mesh, hydro, ph, u0)
+    sig = to_global(base.get_state()["conserved"], mesh.num_blocks, mesh.block_size)[..., 0]
+    cw = (sig >= mask).astype(np.float64)
+    assert 0.2 < cw.mean() <= 1.0
     n, ms = net.samples()
     assert n == 1 and np.all(ms > 0)
-    np.testing.assert_allclose(ms, np.mean(T[..., :3] ** 2, axis=(0, 1)), rtol=1e-4)
+    np.testing.assert_allclose(ms, np.mean(cw[..., None] * T[..., :3] ** 2, axis=(0, 1)), rtol=1e-4)
     dx = 2 * mesh.domain_radius / N
     s = np.sqrt(ms) * dx
     w = 1.0 / ms
@@ -123,8 +127,8 @@ def test_sample_gradient_matches_oracle():
     T32 = np.asarray(T, dtype=np.float32).astype(np.float64)
     # exact FP64 gradient (the bf16 device is within 10% of it) and the
     # bf16-emulating oracle (rounding where the device rounds: within 1e-2)
-    L, gx, _ = C.loss_and_grads(oracle_layers(p), x0, T32, s, w, dx, dx)
-    Le, gw, _ = C.loss_and_grads(oracle_layers(p), x0, T32, s, w, dx, dx, rnd=C.bf16)
+    L, gx, _ = C.loss_and_grads(oracle_layers(p), x0, T32, s, w, dx, dx, cw=cw)
+    Le, gw, _ = C.loss_and_grads(oracle_layers(p), x0, T32, s, w, dx, dx, rnd=C.bf16, cw=cw)
     assert loss == pytest.approx(L, rel=2e-2)
     assert loss == pytest.approx(Le, rel=2e-3)
     k = 0
