@@ -17,6 +17,8 @@ tree with its own CBOR codec, following serde's data model:
   Tasks          -> {"write_checkpoint": {"count", "next_time"}, ...} (src/tasks.rs:10-50)
   TimeSeries     -> {"isothermal" | "euler": [{"time", "integrated_source_terms",
                                                "orbital_elements_change"}, ...]}
+                    (plus "torque": [tau_1, tau_2] per sample when recorded, an
+                    extension field that serde skips)
 The layout is restated from those conventions; no reference checkpoint file
 exists to pin it, so it is covered by round-trip tests.  The configuration's
 `model` is written as given (a reference model, or this build's seeded
@@ -77,8 +79,13 @@ def pack(state, mesh, hydro, physics, model=None, control=None, tasks=None, time
         }
     tag = hydro_tag(hydro)
     num, den = state["iteration"]
-    series = [{"time": float(s["time"]), "integrated_source_terms": _items(s["ist"]),
-               "orbital_elements_change": _items(s["oec"])} for s in (time_series or [])]
+    series = []
+    for s in time_series or []:
+        e = {"time": float(s["time"]), "integrated_source_terms": _items(s["ist"]),
+             "orbital_elements_change": _items(s["oec"])}
+        if "torque" in s:      # extension field; serde ignores unknown fields here
+            e["torque"] = [float(v) for v in s["torque"]]
+        series.append(e)
     return {
         "state": {tag: {"time": float(state["time"]), "iteration": [int(num), int(den)], "solution": solution}},
         "tasks": tasks if tasks is not None else new_tasks(),
@@ -116,8 +123,11 @@ def unpack(app):
     num, den = st["iteration"]
     series = []
     for s in app.get("time_series", {}).get(tag, []):
-        series.append({"time": float(s["time"]), "ist": _unitems(s["integrated_source_terms"], nv),
-                       "oec": _unitems(s["orbital_elements_change"], 4)})
+        e = {"time": float(s["time"]), "ist": _unitems(s["integrated_source_terms"], nv),
+             "oec": _unitems(s["orbital_elements_change"], 4)}
+        if "torque" in s:
+            e["torque"] = np.asarray(s["torque"], dtype=np.float64)
+        series.append(e)
     return {"state": {"time": float(st["time"]), "iteration": (int(num), int(den)), "conserved": u,
                       "ist": ist, "oec": oec},
             "hydro": tag, "num_blocks": nb, "block_size": bs,

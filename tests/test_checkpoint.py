@@ -77,3 +77,33 @@ def test_checkpoint_tree_round_trip(tmp_path, hydro):
     np.testing.assert_array_equal(blk["conserved"]["data"], st["conserved"][5].reshape(64, nv))
     assert list(blk["integrated_source_terms"]) == list(ckpt.ITEMS)
     assert tree["config"]["hydro"] == {ckpt.hydro_tag(hydro): ckpt._plain(hydro)}
+
+
+def test_time_series_rates_and_checkpoint_round_trip(tmp_path):
+    """Accretion and torque rates from samples with linear ramps are exact,
+    and the samples (with the torque extension) survive a checkpoint."""
+    from cdc_amd import timeseries as ts
+    times = np.array([0.0, 0.5, 1.25, 2.0])
+    samples = []
+    for t in times:
+        ist = np.zeros((7, 3))
+        ist[0, 0] = -0.25 * t          # sink 1 accretes at 0.25
+        ist[1, 0] = -0.75 * t          # sink 2 at 0.75
+        samples.append({"time": t, "ist": ist, "oec": np.zeros((7, 4)), "torque": np.array([2.0 * t, -t])})
+    s = ts.extract(samples)
+    tm, mdot, tau = ts.rates(s)
+    np.testing.assert_allclose(tm, [0.25, 0.875, 1.625])
+    np.testing.assert_allclose(mdot, [[0.25, 0.75]] * 3, rtol=1e-15)
+    np.testing.assert_allclose(tau, [[2.0, -1.0]] * 3, rtol=1e-15)
+    np.testing.assert_allclose(ts.accreted_mass(s)[-1], [0.5, 1.5])
+    mesh = Mesh(1, 8, 4.0)
+    st = {"time": 2.0, "iteration": (4, 1), "conserved": np.ones((1, 8, 8, 3)), "ist": np.zeros((1, 7, 3)),
+          "oec": np.zeros((1, 7, 4))}
+    p = tmp_path / "chkpt.0001.cbor"
+    ckpt.write(str(p), st, mesh, Isothermal(mach_number=10.0), workloads.base_physics(), time_series=samples)
+    back = ts.from_checkpoint(str(p))
+    np.testing.assert_array_equal(back["torque"], s["torque"])
+    np.testing.assert_array_equal(back["ist"], s["ist"])
+    ts.main([str(p), "--output", str(tmp_path / "series.npz")])
+    z = np.load(tmp_path / "series.npz")
+    np.testing.assert_allclose(z["mdot"], mdot)
