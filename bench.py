@@ -62,8 +62,18 @@ def cpu_baseline(w, u0, seconds, threads):
             break
     o.close()
     zones = w["mesh"].total_zones()
+    cpu = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": n * zones / el, "unit": "zone-updates/s", "cores": O.get_threads(), "kind": "port",
-            "sample": f"{n} RK steps of the same workload ({el:.1f} s), C oracle, OpenMP over blocks"}
+            "sample": f"{n} RK steps of the same workload ({el:.1f} s), C oracle, OpenMP over blocks, {cpu}",
+            "mzps": n * zones / el * 1e-6}
 
 
 def run_ensemble(args):
