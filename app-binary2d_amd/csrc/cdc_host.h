@@ -81,7 +81,7 @@ struct cdc_solver {
     // device copy, and the batch-of-one offsets {0, ntiles, 0, nblk}
     cdc::StageArgs *h_args = nullptr, *d_args = nullptr;
     size_t cap_args = 0;
-    int32_t *d_begin = nullptr;
+    int32_t *h_begin = nullptr, *d_begin = nullptr;   // pinned staging and device copy
 
     bool timing = false;
     int64_t launches = 0;

@@ -709,7 +709,17 @@ __device__ __forceinline__ void finalize_block(const StageArgs &A, const int b, 
     const int tid = threadIdx.x;
     if (tid < NS) {
         double t = 0.0;
-        for (int j = 0; j < tpb; ++j) t += dev::gptr(A.tile_sums)[((size_t)b * tpb + j) * NR + tid];
+        // tiles in order; unrolled so that the loads are in flight together
+        const auto *ts = dev::gptr(A.tile_sums) + (size_t)b * tpb * NR + tid;
+        int j = 0;
+        for (; j + 8 <= tpb; j += 8) {
+            double v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = ts[(size_t)(j + q) * NR];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) t += v[q];
+        }
+        for (; j < tpb; ++j) t += ts[(size_t)j * NR];
         s_ds[SumMap<NV>::item(tid) * NV + SumMap<NV>::comp(tid)] = t * A.area;
     } else if (tid < NS + 7 * NV) {
         // structurally zero components

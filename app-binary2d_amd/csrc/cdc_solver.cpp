@@ -164,6 +164,7 @@ void free_all(cdc_solver *s) {
     (void)hipFree(s->d_args);
     (void)hipFree(s->d_begin);
     if (s->h_args) (void)hipHostFree(s->h_args);
+    if (s->h_begin) (void)hipHostFree(s->h_begin);
     for (auto e : s->ev) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
@@ -244,8 +245,11 @@ int launch_stages(cdc_solver *s, const StageArgs *args, size_t n, const cdc::Til
     // the staging buffer is free: every earlier call synchronised its stream
     memcpy(s->h_args, args, n * sizeof(StageArgs));
     HIP_TRY(hipMemcpyAsync(s->d_args, s->h_args, n * sizeof(StageArgs), hipMemcpyHostToDevice, s->stream));
-    const int32_t begin[4] = {0, args[0].ntiles, 0, s->nblk};
-    HIP_TRY(hipMemcpyAsync(s->d_begin, begin, sizeof(begin), hipMemcpyHostToDevice, s->stream));
+    s->h_begin[0] = 0;
+    s->h_begin[1] = args[0].ntiles;
+    s->h_begin[2] = 0;
+    s->h_begin[3] = s->nblk;
+    HIP_TRY(hipMemcpyAsync(s->d_begin, s->h_begin, 4 * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
     for (size_t i = 0; i < n; ++i) {
         if (ev) HIP_TRY(hipEventRecord(ev[2 * i], s->stream));
         HIP_TRY(cdc::launch_stage_batch(s->nv, ts, s->d_args + i, s->d_begin, 1, args[i].ntiles, s->stream));
@@ -570,6 +574,10 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
     s->namax = 1024;
     ALLOC(s->d_amax, (size_t)s->namax * sizeof(double));
     ALLOC(s->d_begin, 4 * sizeof(int32_t));
+    if (hipHostMalloc((void **)&s->h_begin, 4 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(CDC_ERR_OUT_OF_MEMORY, "pinned staging");
+    }
 #undef ALLOC
 
     rc = upload_conserved(s, initial_conserved, s->d_init);

@@ -31,19 +31,29 @@ def algorithmic_bytes_per_step(nv, rk):
     return 8 * nv * (3 + 4 * (rk - 1))
 
 
-def measured_traffic(workload, tile):
-    """HBM bytes per stage launch from the committed PMC summary
-    (FETCH_SIZE and WRITE_SIZE passes calibrated on a plain copy kernel),
-    when it covers this workload and tile; None otherwise."""
+def committed_profile(workload, tile):
+    """The committed PMC summary of the stage kernel (HBM traffic from
+    calibrated FETCH_SIZE / WRITE_SIZE passes, VALU instructions per wave)
+    when it covers this workload and tile; {} otherwise."""
     path = os.path.join(ROOT, "profiles", f"stage_kernel_{workload}_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
-        return None
+        return {}
     if t.get("workload") != workload or t.get("tile") != tile:
+        return {}
+    return t
+
+
+def valu_issue(prof, avg_launch_ms):
+    """The launch's FP64-VALU issue floor (instructions x 4 cycles per wave64
+    instruction over all SIMDs) and the fraction of it achieved."""
+    if not prof or "valu_insts_per_wave" not in prof:
         return None
-    return t.get("traffic_bytes_per_launch")
+    cycles = prof["waves_per_launch"] * prof["valu_insts_per_wave"] * prof["valu_cycles_per_wave_instruction"]
+    floor_ms = cycles / prof["simds"] / (prof["clock_ghz"] * 1e9) * 1e3
+    return {"floor_ms": floor_ms, "frac": floor_ms / avg_launch_ms, "source": prof.get("pmc_source")}
 
 
 def cpu_baseline(w, u0, seconds, threads):
@@ -221,6 +231,7 @@ def main():
     t1 = time.perf_counter()
     launches, kernel_ms, cells = g.get_timing()
     wall = t1 - t0
+    prof = committed_profile(args.workload, "%dx%d" % g.get_tile())
     rk = ph.rk_substeps()
     bytes_total = algorithmic_bytes_per_step(hydro.nv, rk) * zones * steps
     achieved = bytes_total / (kernel_ms * 1e-3) / 1e9
@@ -243,10 +254,11 @@ def main():
                    "tile": "%dx%d" % g.get_tile()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.workload, "%dx%d" % g.get_tile()),
+                     "traffic": prof.get("traffic_bytes_per_launch"),
                      "kernel": "stage_kernel_batch", "launches": launches,
                      "avg_launch_ms": kernel_ms / max(launches, 1),
-                     "bytes_per_launch": bytes_total / max(launches, 1)},
+                     "bytes_per_launch": bytes_total / max(launches, 1),
+                     "valu_issue": valu_issue(prof, kernel_ms / max(launches, 1))},
     }
     g.close()
     if not args.no_cpu:
