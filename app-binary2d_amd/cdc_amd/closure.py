@@ -155,9 +155,10 @@ def run_to(step, solver, t_end, log=None, every=2000, min_dt_ratio=1e-2):
 
 def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, sample_every=100,
                     max_samples=200, train_steps=4000, lr=1e-3, seed=5, limiter=0.1, train_mask=0.05,
-                    variants=((0.1, 1.0, 0.05),), log=print):
+                    pair_steps=1, variants=((0.1, 1.0, 0.05),), log=print):
     """Train a closure for the coarse grid on pairs from the fine workload
-    (the loss over cells with density >= train_mask),
+    (the loss over cells with density >= train_mask; each pair compares
+    pair_steps coarse steps with the fine run over the same time),
     then run plain fine, plain coarse and learned coarse runs over the
     held-out window [train_orbits, train_orbits + eval_orbits] from the same
     fine state; one learned run per (limiter kappa, damping, mask density)
@@ -191,12 +192,14 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
             continue
         base.resample_from(fine)
         plain.resample_from(fine)
+        # pair_steps coarse steps of dtc against 4 pair_steps fine steps of dtc / 4:
+        # the target is the mean rate of the difference over that window
         dtc = min(plain.advance(0), 4.0 * fine.advance(0))
-        plain.advance_dt(dtc)
-        fine.advance_dt(dtc / 4.0, 4)
-        nfine += 4
+        plain.advance_dt(dtc, pair_steps)
+        fine.advance_dt(dtc / 4.0, 4 * pair_steps)
+        nfine += 4 * pair_steps
         truth.resample_from(fine)
-        net.add_sample(base, plain, truth, dtc)
+        net.add_sample(base, plain, truth, dtc * pair_steps)
         k = net.samples()[0]
         if k % 20 == 0:
             log(f"  {k} samples, t = {fine.time() / ORBIT:.3f} orbits")
