@@ -208,7 +208,7 @@ def main():
     if args.workload == "learned512":
         run_learned(args)
         return
-    from cdc_amd import Solver, workloads
+    from cdc_amd import HydroError, Solver, workloads
     w = workloads.get(args.workload)
     u0 = workloads.initial_state(w)
     mesh, hydro, ph = w["mesh"], w["hydro"], w["physics"]
@@ -225,10 +225,18 @@ def main():
         g.set_tile(*(int(v) for v in args.tile.lower().split("x")))
     elif not args.no_autotune:
         g.autotune(reps=3)
+    if args.workload == "euler2048" and args.warmup + steps > 400:
+        print("bench.py: the 2048^2 alpha-viscosity workload loses stability after about 470 steps "
+              "(the CFL step has no viscous bound); keep warmup + steps below 400", file=sys.stderr)
     g.set_timing(True)
     t0 = time.perf_counter()
-    for _ in range(steps // fold):
-        g.advance(fold)
+    try:
+        for _ in range(steps // fold):
+            g.advance(fold)
+    except HydroError as e:
+        print(json.dumps({"metric": "f64 zone-updates/s", "value": None, "error": str(e),
+                          "config": {"workload": args.workload}}), flush=True)
+        sys.exit(1)
     t1 = time.perf_counter()
     launches, kernel_ms, cells = g.get_timing()
     wall = t1 - t0
