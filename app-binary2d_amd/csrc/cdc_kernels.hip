@@ -699,6 +699,10 @@ stage_kernel_batch(const StageArgs *__restrict__ args, const int32_t *__restrict
 // (ItemizedChange::perturbation, src/state.rs:258-282) one per lane, and the
 // RK combine of integrated_source_terms / orbital_elements_change
 // (src/scheme.rs:307-315, src/state.rs:195-197).
+// A wave that only waits on memory: every load that does not depend on the
+// tile sums (the error flag, the block's current and base ist / oec / torque
+// values) is issued with the tile-sum loads, so the block costs two memory
+// round trips (loads, then stores) around the perturbation arithmetic.
 template <int NV>
 __device__ __forceinline__ void finalize_block(const StageArgs &A, const int b, const int tpb) {
     constexpr int NS = SumMap<NV>::n, NR = NS + 1;
@@ -707,6 +711,20 @@ __device__ __forceinline__ void finalize_block(const StageArgs &A, const int b, 
     __shared__ double s_de[7 * 4];
     __shared__ int s_perr;
     const int tid = threadIdx.x;
+    const int ef = *(volatile const int32_t *)A.err_flag;
+    const size_t oi = (size_t)b * 7 * NV, oe = (size_t)b * 7 * 4;
+    double ist_c = 0.0, ist_b = 0.0, oec_c = 0.0, oec_b = 0.0, tq_c = 0.0, tq_b = 0.0;
+    if (tid < 7 * 4) {
+        if (tid < 7 * NV) {
+            ist_c = A.ist_c[oi + tid];
+            if (A.read_base) ist_b = A.ist_b[oi + tid];
+        }
+        oec_c = A.oec_c[oe + tid];
+        if (A.read_base) oec_b = A.oec_b[oe + tid];
+    } else if (tid >= 32 && tid < 34) {
+        tq_c = A.tq_c[(size_t)b * 2 + (tid - 32)];
+        if (A.read_base) tq_b = A.tq_b[(size_t)b * 2 + (tid - 32)];
+    }
     if (tid < NS) {
         double t = 0.0;
         // tiles in order; unrolled so that the loads are in flight together
@@ -729,6 +747,9 @@ __device__ __forceinline__ void finalize_block(const StageArgs &A, const int b, 
         if (!nonzero) s_ds[z] = 0.0;
     }
     if (tid == 0) s_perr = 0;
+    // a stage that already failed (err_flag holds an earlier stage id) writes
+    // nothing; every lane reads the same flag, so the return is uniform
+    if (ef != 0 && ef < A.stage_id) return;
     __syncthreads();
     if (tid < 4) {
         const int item = tid == 0 ? CDC_ITEM_SINK1 : tid == 1 ? CDC_ITEM_SINK2 : tid == 2 ? CDC_ITEM_GRAV1 : CDC_ITEM_GRAV2;
@@ -763,28 +784,27 @@ CDC_NO_CONTRACT
             tau = y * g[1] - x * g[2];
         }
         const size_t ti = (size_t)b * 2 + k;
-        const double s1 = A.tq_c[ti] + tau;
-        A.tq_o[ti] = A.read_base ? s1 * A.c1 + A.tq_b[ti] * A.c0 : s1;
+        const double s1 = tq_c + tau;
+        A.tq_o[ti] = A.read_base ? s1 * A.c1 + tq_b * A.c0 : s1;
     }
-    const size_t oi = (size_t)b * 7 * NV, oe = (size_t)b * 7 * 4;
     if (tid < 7 * 4) {
         if (tid < 7 * NV) {
-            const double s1 = A.ist_c[oi + tid] + s_ds[tid];
-            A.ist_o[oi + tid] = A.read_base ? s1 * A.c1 + A.ist_b[oi + tid] * A.c0 : s1;
+            const double s1 = ist_c + s_ds[tid];
+            A.ist_o[oi + tid] = A.read_base ? s1 * A.c1 + ist_b * A.c0 : s1;
         }
-        const double s1 = A.oec_c[oe + tid] + s_de[tid];
-        A.oec_o[oe + tid] = A.read_base ? s1 * A.c1 + A.oec_b[oe + tid] * A.c0 : s1;
+        const double s1 = oec_c + s_de[tid];
+        A.oec_o[oe + tid] = A.read_base ? s1 * A.c1 + oec_b * A.c0 : s1;
     }
 }
 
-// Blocks of run r are global blocks begin[r] .. begin[r+1]-1.
+// Blocks of run r are global blocks begin[r] .. begin[r+1]-1; the offsets
+// start at begin[0] = 0, so a single run needs no search and no load.
 template <int NV>
 __global__ void __launch_bounds__(64)
 finalize_kernel_batch(const StageArgs *__restrict__ args, const int32_t *__restrict__ begin, int nruns) {
-    const int r = run_of(begin, nruns, blockIdx.x);
+    const int r = nruns == 1 ? 0 : run_of(begin, nruns, blockIdx.x);
     const StageArgs &A = args[r];
-    if (stage_skipped(A)) return;
-    finalize_block<NV>(A, blockIdx.x - begin[r], A.tpb);
+    finalize_block<NV>(A, nruns == 1 ? (int)blockIdx.x : (int)blockIdx.x - begin[r], A.tpb);
 }
 
 // Batch prologue: clear every run's error flag (a kernel, not a memset, so
