@@ -61,7 +61,12 @@ struct cdc_solver {
     double *d_ist[cdc::host::NBUF] = {};
     double *d_oec[cdc::host::NBUF] = {};
     double *d_tq[cdc::host::NBUF] = {};   // [nblk][2] torque impulses (extension)
+    // two buffers of [max_tiles][nsum + 1] per-tile sums, used by stage
+    // parity, so that stage i + 1 writes while the finalize of stage i,
+    // fused into the same launch, reads
     double *d_tile_sums = nullptr;
+    size_t ts_stride = 0;
+    const double *ts_last = nullptr;   // buffer of the last committed stage (max-speed column)
     cdc::TileErr *d_tile_err = nullptr;
     int32_t *d_perr = nullptr;
     int32_t *d_err_flag = nullptr;
@@ -74,7 +79,7 @@ struct cdc_solver {
     int cur = 0;                 // committed buffer
     double time = 0.0;
     int64_t num = 0, den = 1;
-    bool amax_valid = false;     // tile_sums[:, nsum] hold the committed state's max speed
+    bool amax_valid = false;     // ts_last[:, nsum] hold the committed state's max speed
     uint64_t version = 0;        // bumped whenever the committed state changes
 
     // launch arguments of the stages of one advance: pinned staging and its
@@ -97,9 +102,14 @@ namespace host {
 
 // Stage (and, when finalize, per-block finalize) launches of args[0..n) for
 // this one run (a batch of one), after one upload of the arguments; events
-// ev[2i], ev[2i+1] bracket stage launch i when ev is given.
+// ev[2i], ev[2i+1] bracket stage launch i when ev is given.  With fuse, the
+// finalize of stage i runs as extra workgroups of stage i + 1's launch (one
+// launch and one dependent-launch gap fewer per stage) and only the last
+// stage's finalize is a launch of its own; a stage whose predecessor's
+// finalize failed is then not reliably skipped, so an advance that raises
+// its error flag is repeated unfused to name the reference's first error.
 int launch_stages(cdc_solver *s, const StageArgs *args, size_t n, const cdc::TileShape &ts, hipEvent_t *ev,
-                  bool finalize);
+                  bool finalize, bool fuse = false);
 // Clears the run's error flag.
 int launch_set_int_checked(cdc_solver *s);
 // Global max signal speed of the committed state (State::max_signal_speed).
@@ -109,7 +119,7 @@ double step_dt(const cdc_solver *s, double amax);
 // Launch arguments of `fold` RK steps at dt from the committed state.
 void plan_steps(const cdc_solver *s, int fold, double dt, StepPlan &plan);
 // Makes the plan's result the committed state.
-void commit(cdc_solver *s, const Slot &out);
+void commit(cdc_solver *s, const StepPlan &plan);
 // After a failed advance (err_flag = flag): the error join_all would report.
 int resolve_error(cdc_solver *s, int flag, const StepPlan &plan, cdc_error_t *err);
 

@@ -23,6 +23,7 @@
 // FP64 reciprocals and square roots use the hardware estimates (v_rcp_f64,
 // v_rsq_f64) refined by Newton-Raphson steps to within an ulp or so of the
 // IEEE results, at a third of the instructions of the IEEE expansions.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <limits.h>
 #include <math.h>
@@ -402,10 +403,25 @@ __device__ __forceinline__ int xcd_tile(int wg, int ntiles) {
     return (ntiles % 8 == 0) ? (wg % 8) * (ntiles / 8) + wg / 8 : wg;
 }
 
-// One RK stage on one TX x TY tile of one block (the P, F and U phases of
-// try_advance_rk, src/scheme.rs:88-173, restricted to the tile).
+// LDS doubles of one tile: primitives on the extended tile, the cell c_s^2
+// plane (Euler), gradients on the 1-zone ring; the face fluxes and the tile
+// reductions reuse the same space.
 template <int NV, int TX, int TY, int NT>
-__device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
+struct TileLds {
+    static constexpr int EXY = (TX + 4) * (TY + 4), RXY = (TX + 2) * (TY + 2);
+    static constexpr int NR = SumMap<NV>::n + 1, CH = 16;
+    static constexpr int A = NV * EXY + (NV == 4 ? EXY : 0) + 2 * NV * RXY;
+    static constexpr int CHUNK_FIT = (A - NR * CH) / NT;
+    static constexpr int CHUNK = CHUNK_FIT >= NR ? NR : (CHUNK_FIT < 1 ? 1 : CHUNK_FIT);
+    static constexpr int R = NR * CH + CHUNK * NT;
+    static constexpr int size = A > R ? A : R;
+};
+
+// One RK stage on one TX x TY tile of one block (the P, F and U phases of
+// try_advance_rk, src/scheme.rs:88-173, restricted to the tile), in the
+// workgroup's LDS array lds[TileLds::size].
+template <int NV, int TX, int TY, int NT>
+__device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile, double *lds, int &s_key) {
     constexpr int EX = TX + 4, EY = TY + 4, EXY = EX * EY;
     constexpr int RX = TX + 2, RY = TY + 2, RXY = RX * RY;
     constexpr int NFX = (TX + 1) * TY, NFY = TX * (TY + 1), NF = NFX + NFY;
@@ -414,16 +430,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
     constexpr int NS = SumMap<NV>::n, NR = NS + 1;
     constexpr int CH = 16, CL = NT / CH;
     constexpr int NC = NV == 4 ? EXY : 0;       // cell c_s^2 plane (Euler alpha viscosity)
-    constexpr int LDS_A = NV * EXY + NC + 2 * NV * RXY;
-    constexpr int CHUNK_FIT = (LDS_A - NR * CH) / NT;
-    constexpr int CHUNK = CHUNK_FIT >= NR ? NR : (CHUNK_FIT < 1 ? 1 : CHUNK_FIT);
+    constexpr int CHUNK = TileLds<NV, TX, TY, NT>::CHUNK;
     constexpr int NPASS = (NR + CHUNK - 1) / CHUNK;
-    constexpr int LDS_R = NR * CH + CHUNK * NT;
-    constexpr int LDS = LDS_A > LDS_R ? LDS_A : LDS_R;
     static_assert(NV * NF <= 2 * NV * RXY, "flux staging must fit the gradient region");
     static_assert((TX * TY) % NT == 0 && NT % CH == 0, "tile / thread mismatch");
-    __shared__ double lds[LDS];
-    __shared__ int s_key;
 
     const int tid = threadIdx.x;
     const int bs = A.bs, N = A.N;
@@ -472,12 +482,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
         const int ci = c / TY, cj = c - (c / TY) * TY;
         const size_t g = (size_t)(gi0 + ci) * N + (gj0 + cj);
 #pragma unroll
+        // read once per stage: streamed, so that L2 keeps the halo lines the
+        // neighbouring tiles re-read
         for (int v = 0; v < NV; ++v) {
-            pu0[q][v] = gu0[v * NN + g];
-            pub[q][v] = A.read_base ? gub[v * NN + g] : 0.0;
+            pu0[q][v] = __builtin_nontemporal_load(&gu0[v * NN + g]);
+            pub[q][v] = A.read_base ? __builtin_nontemporal_load(&gub[v * NN + g]) : 0.0;
         }
-        pbr[q] = gaux[g];
-        pom[q] = A.k.cooling_model == CDC_COOLING_BETA ? gaux[NN + g] : 0.0;
+        pbr[q] = __builtin_nontemporal_load(&gaux[g]);
+        pom[q] = A.k.cooling_model == CDC_COOLING_BETA ? __builtin_nontemporal_load(&gaux[NN + g]) : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < KE; ++q) {
@@ -600,7 +612,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
             else total = total + it[CDC_ITEM_FAKE_MASS][v];
             const double u1 = uc[v] + du[v] + total;
             out[v] = A.read_base ? u1 * A.c1 + ub[v] * A.c0 : u1;
-            guo[v * NN + g] = out[v];
+            // streamed past L2: the next stage reads it back through its halo
+            // loads, and dirty lines left in L2 cost more than they save
+            __builtin_nontemporal_store(out[v], &guo[v * NN + g]);
         }
 #pragma unroll
         for (int s = 0; s < NS; ++s) acc[s] += it[SumMap<NV>::item(s)][SumMap<NV>::comp(s)];
@@ -678,22 +692,6 @@ __device__ __forceinline__ bool stage_skipped(const StageArgs &A) {
     return ef != 0 && ef < A.stage_id;
 }
 
-// One RK stage of a batch of runs in one launch (a single run is a batch of
-// one, so every run executes the same machine code): tiles of run r are
-// global tiles begin[r] .. begin[r + 1] - 1, and each run reads its own
-// StageArgs.  One-cell-per-thread tiles fit four 256-thread workgroups per
-// CU in LDS; ask the register allocator for the matching four waves per SIMD.
-template <int NV, int TX, int TY, int NT>
-__global__ void __launch_bounds__(NT)
-__attribute__((amdgpu_waves_per_eu((TX * TY == NT) ? 4 : 2)))
-stage_kernel_batch(const StageArgs *__restrict__ args, const int32_t *__restrict__ begin, int nruns) {
-    const int t = xcd_tile(blockIdx.x, begin[nruns]);
-    const int r = run_of(begin, nruns, t);
-    const StageArgs &A = args[r];
-    if (stage_skipped(A)) return;
-    stage_tile<NV, TX, TY, NT>(A, t - begin[r]);
-}
-
 // The per-block tail of a stage, one wave per block: the block's tile sums
 // in tile order, scaled by dx*dy, the four orbital-element perturbations
 // (ItemizedChange::perturbation, src/state.rs:258-282) one per lane, and the
@@ -703,79 +701,87 @@ stage_kernel_batch(const StageArgs *__restrict__ args, const int32_t *__restrict
 // tile sums (the error flag, the block's current and base ist / oec / torque
 // values) is issued with the tile-sum loads, so the block costs two memory
 // round trips (loads, then stores) around the perturbation arithmetic.
+// One wave per block, lane = lane index; every wave of the workgroup calls
+// this (an inactive one, past the last block, only takes the two barriers).
+// Scratch in LDS: s_ds[7 nv], s_de[7 * 4], *s_perr, per wave.
+constexpr int kFinLds = 7 * 4 + 7 * 4 + 1;   // doubles of per-wave scratch (nv <= 4)
+
 template <int NV>
-__device__ __forceinline__ void finalize_block(const StageArgs &A, const int b, const int tpb) {
+__device__ __forceinline__ void finalize_block(const StageArgs &A, const int b, const int tpb, const int lane,
+                                               const bool active, double *s_ds, double *s_de, int *s_perr) {
     constexpr int NS = SumMap<NV>::n, NR = NS + 1;
     static_assert(NS + 7 * NV <= 64, "one wave per block");
-    __shared__ double s_ds[7 * NV];
-    __shared__ double s_de[7 * 4];
-    __shared__ int s_perr;
-    const int tid = threadIdx.x;
     const int ef = *(volatile const int32_t *)A.err_flag;
     const size_t oi = (size_t)b * 7 * NV, oe = (size_t)b * 7 * 4;
     double ist_c = 0.0, ist_b = 0.0, oec_c = 0.0, oec_b = 0.0, tq_c = 0.0, tq_b = 0.0;
-    if (tid < 7 * 4) {
-        if (tid < 7 * NV) {
-            ist_c = A.ist_c[oi + tid];
-            if (A.read_base) ist_b = A.ist_b[oi + tid];
+    if (active) {
+        if (lane < 7 * 4) {
+            if (lane < 7 * NV) {
+                ist_c = A.ist_c[oi + lane];
+                if (A.read_base) ist_b = A.ist_b[oi + lane];
+            }
+            oec_c = A.oec_c[oe + lane];
+            if (A.read_base) oec_b = A.oec_b[oe + lane];
+        } else if (lane >= 32 && lane < 34) {
+            tq_c = A.tq_c[(size_t)b * 2 + (lane - 32)];
+            if (A.read_base) tq_b = A.tq_b[(size_t)b * 2 + (lane - 32)];
         }
-        oec_c = A.oec_c[oe + tid];
-        if (A.read_base) oec_b = A.oec_b[oe + tid];
-    } else if (tid >= 32 && tid < 34) {
-        tq_c = A.tq_c[(size_t)b * 2 + (tid - 32)];
-        if (A.read_base) tq_b = A.tq_b[(size_t)b * 2 + (tid - 32)];
-    }
-    if (tid < NS) {
-        double t = 0.0;
-        // tiles in order; unrolled so that the loads are in flight together
-        const auto *ts = dev::gptr(A.tile_sums) + (size_t)b * tpb * NR + tid;
-        int j = 0;
-        for (; j + 8 <= tpb; j += 8) {
-            double v[8];
+        if (lane < NS) {
+            double t = 0.0;
+            // tiles in order; unrolled so that the loads are in flight together
+            const auto *ts = dev::gptr(A.tile_sums) + (size_t)b * tpb * NR + lane;
+            int j = 0;
+            for (; j + 8 <= tpb; j += 8) {
+                double v[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = ts[(size_t)(j + q) * NR];
+                for (int q = 0; q < 8; ++q) v[q] = ts[(size_t)(j + q) * NR];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) t += v[q];
+                for (int q = 0; q < 8; ++q) t += v[q];
+            }
+            for (; j < tpb; ++j) t += ts[(size_t)j * NR];
+            s_ds[SumMap<NV>::item(lane) * NV + SumMap<NV>::comp(lane)] = t * A.area;
+        } else if (lane < NS + 7 * NV) {
+            // structurally zero components
+            const int z = lane - NS;
+            bool nonzero = false;
+            for (int s = 0; s < NS; ++s) nonzero |= (SumMap<NV>::item(s) * NV + SumMap<NV>::comp(s)) == z;
+            if (!nonzero) s_ds[z] = 0.0;
         }
-        for (; j < tpb; ++j) t += ts[(size_t)j * NR];
-        s_ds[SumMap<NV>::item(tid) * NV + SumMap<NV>::comp(tid)] = t * A.area;
-    } else if (tid < NS + 7 * NV) {
-        // structurally zero components
-        const int z = tid - NS;
-        bool nonzero = false;
-        for (int s = 0; s < NS; ++s) nonzero |= (SumMap<NV>::item(s) * NV + SumMap<NV>::comp(s)) == z;
-        if (!nonzero) s_ds[z] = 0.0;
     }
-    if (tid == 0) s_perr = 0;
+    if (lane == 0) *s_perr = 0;
     // a stage that already failed (err_flag holds an earlier stage id) writes
-    // nothing; every lane reads the same flag, so the return is uniform
+    // nothing; every lane of the workgroup reads the same flag, so the return
+    // is uniform
     if (ef != 0 && ef < A.stage_id) return;
     __syncthreads();
-    if (tid < 4) {
-        const int item = tid == 0 ? CDC_ITEM_SINK1 : tid == 1 ? CDC_ITEM_SINK2 : tid == 2 ? CDC_ITEM_GRAV1 : CDC_ITEM_GRAV2;
-        const double *d = s_ds + item * NV;
-        double p[4];
-        const bool ok = (tid % 2 == 0) ? perturb(A.orb, -d[0], 0.0, -d[1], 0.0, -d[2], 0.0, p)
-                                       : perturb(A.orb, 0.0, -d[0], 0.0, -d[1], 0.0, -d[2], p);
-        for (int c = 0; c < 4; ++c) s_de[item * 4 + c] = ok ? p[c] - A.el[c] : 0.0;
-        if (!ok) atomicOr(&s_perr, 1);
-    } else if (tid < 4 + 3 * 4) {
-        const int z = tid - 4;   // buffer, cooling, fake_mass items: no orbital change
-        s_de[(CDC_ITEM_BUFFER + z / 4) * 4 + (z % 4)] = 0.0;
+    if (active) {
+        if (lane < 4) {
+            const int item = lane == 0 ? CDC_ITEM_SINK1 : lane == 1 ? CDC_ITEM_SINK2 : lane == 2 ? CDC_ITEM_GRAV1 : CDC_ITEM_GRAV2;
+            const double *d = s_ds + item * NV;
+            double p[4];
+            const bool ok = (lane % 2 == 0) ? perturb(A.orb, -d[0], 0.0, -d[1], 0.0, -d[2], 0.0, p)
+                                            : perturb(A.orb, 0.0, -d[0], 0.0, -d[1], 0.0, -d[2], p);
+            for (int c = 0; c < 4; ++c) s_de[item * 4 + c] = ok ? p[c] - A.el[c] : 0.0;
+            if (!ok) atomicOr(s_perr, 1);
+        } else if (lane < 4 + 3 * 4) {
+            const int z = lane - 4;   // buffer, cooling, fake_mass items: no orbital change
+            s_de[(CDC_ITEM_BUFFER + z / 4) * 4 + (z % 4)] = 0.0;
+        }
     }
     __syncthreads();
-    if (s_perr) {
-        if (tid == 0) {
+    if (!active) return;
+    if (*s_perr) {
+        if (lane == 0) {
             A.perr[b] = CDC_ERR_UNBOUND_ORBIT;
             atomicCAS(A.err_flag, 0, A.stage_id);
         }
         return;
     }
-    if (tid == 0) A.perr[b] = 0;
-    if (tid >= 32 && tid < 34) {
+    if (lane == 0) A.perr[b] = 0;
+    if (lane >= 32 && lane < 34) {
         // torque impulse on body k about the origin: x_k x (-dp_k), dp_k the
         // gas momentum change from body k's gravity (no reference counterpart)
-        const int k = tid - 32;
+        const int k = lane - 32;
         const double *g = s_ds + (CDC_ITEM_GRAV1 + k) * NV;
         const double x = k == 0 ? A.orb.x1 : A.orb.x2, y = k == 0 ? A.orb.y1 : A.orb.y2;
         double tau;
@@ -787,13 +793,13 @@ CDC_NO_CONTRACT
         const double s1 = tq_c + tau;
         A.tq_o[ti] = A.read_base ? s1 * A.c1 + tq_b * A.c0 : s1;
     }
-    if (tid < 7 * 4) {
-        if (tid < 7 * NV) {
-            const double s1 = ist_c + s_ds[tid];
-            A.ist_o[oi + tid] = A.read_base ? s1 * A.c1 + ist_b * A.c0 : s1;
+    if (lane < 7 * 4) {
+        if (lane < 7 * NV) {
+            const double s1 = ist_c + s_ds[lane];
+            A.ist_o[oi + lane] = A.read_base ? s1 * A.c1 + ist_b * A.c0 : s1;
         }
-        const double s1 = oec_c + s_de[tid];
-        A.oec_o[oe + tid] = A.read_base ? s1 * A.c1 + oec_b * A.c0 : s1;
+        const double s1 = oec_c + s_de[lane];
+        A.oec_o[oe + lane] = A.read_base ? s1 * A.c1 + oec_b * A.c0 : s1;
     }
 }
 
@@ -802,9 +808,43 @@ CDC_NO_CONTRACT
 template <int NV>
 __global__ void __launch_bounds__(64)
 finalize_kernel_batch(const StageArgs *__restrict__ args, const int32_t *__restrict__ begin, int nruns) {
+    __shared__ double scratch[kFinLds];
     const int r = nruns == 1 ? 0 : run_of(begin, nruns, blockIdx.x);
     const StageArgs &A = args[r];
-    finalize_block<NV>(A, nruns == 1 ? (int)blockIdx.x : (int)blockIdx.x - begin[r], A.tpb);
+    finalize_block<NV>(A, nruns == 1 ? (int)blockIdx.x : (int)blockIdx.x - begin[r], A.tpb, threadIdx.x, true,
+                       scratch, scratch + 7 * 4, (int *)(scratch + 7 * 4 + 7 * 4));
+}
+
+// One RK stage of a batch of runs in one launch (a single run is a batch of
+// one, so every run executes the same machine code): tiles of run r are
+// global tiles begin[r] .. begin[r + 1] - 1, and each run reads its own
+// StageArgs.  One-cell-per-thread tiles fit four 256-thread workgroups per
+// CU in LDS; ask the register allocator for the matching four waves per SIMD.
+// With fin (a single run), the first nfin workgroups finalize the earlier
+// stage whose arguments fin holds, one block per wave, ahead of the tiles;
+// nfin is a multiple of 8, so the tiles keep their XCD mapping.  The two
+// stages use different tile-sum buffers and the tiles read nothing the
+// finalize writes.
+template <int NV, int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT)
+__attribute__((amdgpu_waves_per_eu((TX * TY == NT) ? 4 : 2)))
+stage_kernel_batch(const StageArgs *__restrict__ args, const int32_t *__restrict__ begin, int nruns,
+                   const StageArgs *__restrict__ fin, int nfin, int fin_blocks) {
+    static_assert((NT / 64) * kFinLds <= TileLds<NV, TX, TY, NT>::size, "finalize scratch fits the tile LDS");
+    __shared__ double lds[TileLds<NV, TX, TY, NT>::size];
+    __shared__ int s_key;
+    if ((int)blockIdx.x < nfin) {
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int b = blockIdx.x * (NT / 64) + w;
+        double *sc = lds + w * kFinLds;
+        finalize_block<NV>(*fin, b, fin->tpb, lane, b < fin_blocks, sc, sc + 7 * 4, (int *)(sc + 7 * 4 + 7 * 4));
+        return;
+    }
+    const int t = xcd_tile((int)blockIdx.x - nfin, begin[nruns]);
+    const int r = run_of(begin, nruns, t);
+    const StageArgs &A = args[r];
+    if (stage_skipped(A)) return;
+    stage_tile<NV, TX, TY, NT>(A, t - begin[r], lds, s_key);
 }
 
 // Batch prologue: clear every run's error flag (a kernel, not a memset, so
@@ -962,28 +1002,45 @@ TileShape tile_shape_for(int bs, int nv) {
     return {8, 8, 64};
 }
 
+// With timing events, they are bound to the dispatch itself
+// (hipExtLaunchKernelGGL): no marker packets between dependent launches.
 template <int NV, int TX, int TY, int NT>
-static hipError_t launch_batch_t(const StageArgs *args, const int32_t *begin, int nruns, int ntiles, hipStream_t s) {
-    hipLaunchKernelGGL((stage_kernel_batch<NV, TX, TY, NT>), dim3(ntiles), dim3(NT), 0, s, args, begin, nruns);
+static hipError_t launch_batch_t(const StageArgs *args, const int32_t *begin, int nruns, int ntiles,
+                                 const StageArgs *fin, int nblk, hipEvent_t ev0, hipEvent_t ev1, hipStream_t s) {
+    constexpr int WPG = NT / 64;   // blocks finalized per workgroup
+    const int nfin = fin ? ((nblk + WPG - 1) / WPG + 7) / 8 * 8 : 0;
+    if (ev0 || ev1)
+        hipExtLaunchKernelGGL((stage_kernel_batch<NV, TX, TY, NT>), dim3(ntiles + nfin), dim3(NT), 0, s, ev0, ev1, 0,
+                              args, begin, nruns, fin, nfin, nblk);
+    else
+        hipLaunchKernelGGL((stage_kernel_batch<NV, TX, TY, NT>), dim3(ntiles + nfin), dim3(NT), 0, s, args, begin,
+                           nruns, fin, nfin, nblk);
     return hipGetLastError();
 }
 
 template <int NV>
 static hipError_t launch_batch_nv(const TileShape &ts, const StageArgs *args, const int32_t *begin, int nruns,
-                                  int ntiles, hipStream_t s) {
-    if (ts.tx == 16 && ts.ty == 32) return launch_batch_t<NV, 16, 32, 256>(args, begin, nruns, ntiles, s);
-    if (ts.tx == 32 && ts.ty == 16) return launch_batch_t<NV, 32, 16, 256>(args, begin, nruns, ntiles, s);
-    if (ts.tx == 16 && ts.ty == 16) return launch_batch_t<NV, 16, 16, 256>(args, begin, nruns, ntiles, s);
-    if (ts.tx == 8 && ts.ty == 32) return launch_batch_t<NV, 8, 32, 256>(args, begin, nruns, ntiles, s);
-    if (ts.tx == 32 && ts.ty == 8) return launch_batch_t<NV, 32, 8, 256>(args, begin, nruns, ntiles, s);
-    if (ts.tx == 8 && ts.ty == 8) return launch_batch_t<NV, 8, 8, 64>(args, begin, nruns, ntiles, s);
+                                  int ntiles, const StageArgs *fin, int nblk, hipEvent_t e0, hipEvent_t e1,
+                                  hipStream_t s) {
+#define CDC_SHAPE(X, Y, T)                                                                        \
+    if (ts.tx == X && ts.ty == Y && ts.nt == T)                                                   \
+        return launch_batch_t<NV, X, Y, T>(args, begin, nruns, ntiles, fin, nblk, e0, e1, s);
+    CDC_SHAPE(16, 32, 256)
+    CDC_SHAPE(32, 16, 256)
+    CDC_SHAPE(16, 16, 256)
+    CDC_SHAPE(8, 32, 256)
+    CDC_SHAPE(32, 8, 256)
+    CDC_SHAPE(8, 8, 64)
+#undef CDC_SHAPE
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_stage_batch(int nv, const TileShape &ts, const StageArgs *args, const int32_t *tile_begin,
-                              int nruns, int ntiles, hipStream_t s) {
-    return nv == 3 ? launch_batch_nv<3>(ts, args, tile_begin, nruns, ntiles, s)
-                   : launch_batch_nv<4>(ts, args, tile_begin, nruns, ntiles, s);
+                              int nruns, int ntiles, const StageArgs *fin, int nblk, hipEvent_t ev0, hipEvent_t ev1,
+                              hipStream_t s) {
+    if (fin && nruns != 1) return hipErrorInvalidValue;
+    return nv == 3 ? launch_batch_nv<3>(ts, args, tile_begin, nruns, ntiles, fin, nblk, ev0, ev1, s)
+                   : launch_batch_nv<4>(ts, args, tile_begin, nruns, ntiles, fin, nblk, ev0, ev1, s);
 }
 
 hipError_t launch_finalize_batch(int nv, const StageArgs *args, const int32_t *block_begin, int nruns,

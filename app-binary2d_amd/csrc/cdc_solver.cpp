@@ -229,7 +229,7 @@ int launch_set_int_checked(cdc_solver *s) {
 }
 
 int launch_stages(cdc_solver *s, const StageArgs *args, size_t n, const cdc::TileShape &ts, hipEvent_t *ev,
-                  bool finalize) {
+                  bool finalize, bool fuse) {
     if (n > s->cap_args) {
         HIP_TRY(hipStreamSynchronize(s->stream));
         if (s->h_args) (void)hipHostFree(s->h_args);
@@ -250,13 +250,18 @@ int launch_stages(cdc_solver *s, const StageArgs *args, size_t n, const cdc::Til
     s->h_begin[2] = 0;
     s->h_begin[3] = s->nblk;
     HIP_TRY(hipMemcpyAsync(s->d_begin, s->h_begin, 4 * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+    fuse = fuse && finalize;
     for (size_t i = 0; i < n; ++i) {
-        if (ev) HIP_TRY(hipEventRecord(ev[2 * i], s->stream));
-        HIP_TRY(cdc::launch_stage_batch(s->nv, ts, s->d_args + i, s->d_begin, 1, args[i].ntiles, s->stream));
-        if (ev) HIP_TRY(hipEventRecord(ev[2 * i + 1], s->stream));
-        if (finalize)
+        // fused: this launch also finalizes stage i - 1 (its tile sums are in
+        // the other parity buffer, its ist / oec inputs are complete)
+        const StageArgs *fin = fuse && i > 0 ? s->d_args + i - 1 : nullptr;
+        HIP_TRY(cdc::launch_stage_batch(s->nv, ts, s->d_args + i, s->d_begin, 1, args[i].ntiles, fin, s->nblk,
+                                        ev ? ev[2 * i] : nullptr, ev ? ev[2 * i + 1] : nullptr, s->stream));
+        if (finalize && !fuse)
             HIP_TRY(cdc::launch_finalize_batch(s->nv, s->d_args + i, s->d_begin + 2, 1, s->nblk, s->stream));
     }
+    if (fuse && n > 0)
+        HIP_TRY(cdc::launch_finalize_batch(s->nv, s->d_args + n - 1, s->d_begin + 2, 1, s->nblk, s->stream));
     return CDC_OK;
 }
 
@@ -266,7 +271,7 @@ int compute_amax(cdc_solver *s, double *out) {
         // max signal speed of the committed state, left by the last stage
         const int nr = s->nsum + 1;
         double a = 0.0;
-        HIP_TRY(cdc::launch_column_max(s->d_tile_sums + s->nsum, s->ntiles, nr, s->d_amax, s->stream));
+        HIP_TRY(cdc::launch_column_max(s->ts_last + s->nsum, s->ntiles, nr, s->d_amax, s->stream));
         HIP_TRY(hipMemcpyAsync(&a, s->d_amax, sizeof(double), hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
         *out = a;
@@ -326,7 +331,7 @@ void plan_steps(const cdc_solver *s, int fold, double dt, StepPlan &plan) {
         a.ub = s->d_u[base.idx];
         a.uo = s->d_u[out.idx];
         a.aux = s->d_aux;
-        a.tile_sums = s->d_tile_sums;
+        a.tile_sums = s->d_tile_sums + (size_t)((sid - 1) & 1) * s->ts_stride;
         a.tile_err = s->d_tile_err;
         a.err_flag = s->d_err_flag;
         a.ist_c = s->d_ist[in.idx];
@@ -395,7 +400,9 @@ void plan_steps(const cdc_solver *s, int fold, double dt, StepPlan &plan) {
     plan.out = cur;
 }
 
-void commit(cdc_solver *s, const Slot &out) {
+void commit(cdc_solver *s, const StepPlan &plan) {
+    const Slot &out = plan.out;
+    s->ts_last = plan.args.back().tile_sums;
     s->cur = out.idx;
     s->time = out.time;
     s->num = out.num;
@@ -565,7 +572,9 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
         ALLOC(s->d_oec[i], (size_t)s->nblk * 7 * 4 * sizeof(double));
         ALLOC(s->d_tq[i], (size_t)s->nblk * 2 * sizeof(double));
     }
-    ALLOC(s->d_tile_sums, (size_t)s->max_tiles * (s->nsum + 1) * sizeof(double));
+    s->ts_stride = (size_t)s->max_tiles * (s->nsum + 1);
+    ALLOC(s->d_tile_sums, 2 * s->ts_stride * sizeof(double));
+    s->ts_last = s->d_tile_sums;
     ALLOC(s->d_tile_err, (size_t)s->max_tiles * sizeof(cdc::TileErr));
     ALLOC(s->d_perr, (size_t)s->nblk * sizeof(int32_t));
     ALLOC(s->d_err_flag, sizeof(int32_t));
@@ -723,7 +732,8 @@ int32_t cdc_solver_autotune(cdc_solver_t *s, int32_t reps, int32_t *tx, int32_t 
         a.ub = s->d_u[s->cur];
         a.aux = s->d_aux;
         a.uo = s->d_u[scratch];
-        a.tile_sums = s->d_tile_sums;
+        // the tile-sum buffer that does not hold the committed max speed
+        a.tile_sums = s->ts_last == s->d_tile_sums ? s->d_tile_sums + s->ts_stride : s->d_tile_sums;
         a.tile_err = s->d_tile_err;
         a.err_flag = s->d_err_flag;
         a.N = s->N;
@@ -932,12 +942,23 @@ static int advance_at(cdc_solver_t *s, int32_t fold, double dt, cdc_error_t *err
         }
     }
     HIP_TRY(cdc::launch_set_int(s->d_err_flag, 0, s->stream));
-    rc = cdc::host::launch_stages(s, plan.args.data(), nst, s->ts, s->timing ? s->ev.data() : nullptr, true);
+    rc = cdc::host::launch_stages(s, plan.args.data(), nst, s->ts, s->timing ? s->ev.data() : nullptr, true, true);
     if (rc) return rc;
 
     int32_t flag = 0;
     HIP_TRY(hipMemcpyAsync(&flag, s->d_err_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    if (flag != 0 && nst > 1) {
+        // The fused launches may have run stages after a failed finalize;
+        // repeat the call unfused from the untouched committed state, where
+        // a failed stage skips the rest, so that the flag and the records
+        // name the first error as the reference reports it.
+        HIP_TRY(cdc::launch_set_int(s->d_err_flag, 0, s->stream));
+        rc = cdc::host::launch_stages(s, plan.args.data(), nst, s->ts, nullptr, true, false);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(&flag, s->d_err_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
     if (s->timing) {
         for (size_t i = 0; i < nst; ++i) {
             float ms = 0.0f;
@@ -947,7 +968,7 @@ static int advance_at(cdc_solver_t *s, int32_t fold, double dt, cdc_error_t *err
         }
     }
     if (flag == 0) {
-        cdc::host::commit(s, plan.out);
+        cdc::host::commit(s, plan);
         return CDC_OK;
     }
     return cdc::host::resolve_error(s, flag, plan, err);
