@@ -87,6 +87,18 @@ struct cdc_solver {
     cdc::StageArgs *h_args = nullptr, *d_args = nullptr;
     size_t cap_args = 0;
     int32_t *h_begin = nullptr, *d_begin = nullptr;   // pinned staging and device copy
+    int32_t *h_flag = nullptr;   // pinned read-back of the error flag
+
+    // the device work of one untimed advance as a replayable graph, per
+    // stage count and tile shape (the arguments are re-read from the pinned
+    // staging at every replay); replays start dependent launches back to back
+    struct AdvanceGraph {
+        size_t n;
+        cdc::TileShape ts;
+        hipGraphExec_t exec;
+    };
+    std::vector<AdvanceGraph> graphs;
+    bool use_graph = true;
 
     bool timing = false;
     int64_t launches = 0;
@@ -110,6 +122,11 @@ namespace host {
 // its error flag is repeated unfused to name the reference's first error.
 int launch_stages(cdc_solver *s, const StageArgs *args, size_t n, const cdc::TileShape &ts, hipEvent_t *ev,
                   bool finalize, bool fuse = false);
+// Copies args[0..n) into the pinned staging, growing it (and dropping the
+// cached graphs, which hold its old addresses) when needed.
+int stage_args(cdc_solver *s, const StageArgs *args, size_t n);
+// Drops the cached advance graphs.
+void drop_graphs(cdc_solver *s);
 // Clears the run's error flag.
 int launch_set_int_checked(cdc_solver *s);
 // Global max signal speed of the committed state (State::max_signal_speed).

@@ -147,6 +147,8 @@ cdc::StageConsts make_consts(const cdc_solver *s, double time, double dt) {
 
 void free_all(cdc_solver *s) {
     if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    cdc::host::drop_graphs(s);
     (void)hipFree(s->d_init);
     for (int i = 0; i < NBUF; ++i) {
         (void)hipFree(s->d_u[i]);
@@ -165,6 +167,7 @@ void free_all(cdc_solver *s) {
     (void)hipFree(s->d_begin);
     if (s->h_args) (void)hipHostFree(s->h_args);
     if (s->h_begin) (void)hipHostFree(s->h_begin);
+    if (s->h_flag) (void)hipHostFree(s->h_flag);
     for (auto e : s->ev) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
@@ -228,10 +231,15 @@ int launch_set_int_checked(cdc_solver *s) {
     return CDC_OK;
 }
 
-int launch_stages(cdc_solver *s, const StageArgs *args, size_t n, const cdc::TileShape &ts, hipEvent_t *ev,
-                  bool finalize, bool fuse) {
+void drop_graphs(cdc_solver *s) {
+    for (auto &g : s->graphs) (void)hipGraphExecDestroy(g.exec);
+    s->graphs.clear();
+}
+
+int stage_args(cdc_solver *s, const StageArgs *args, size_t n) {
     if (n > s->cap_args) {
         HIP_TRY(hipStreamSynchronize(s->stream));
+        drop_graphs(s);
         if (s->h_args) (void)hipHostFree(s->h_args);
         if (s->d_args) (void)hipFree(s->d_args);
         s->h_args = nullptr;
@@ -244,24 +252,86 @@ int launch_stages(cdc_solver *s, const StageArgs *args, size_t n, const cdc::Til
     }
     // the staging buffer is free: every earlier call synchronised its stream
     memcpy(s->h_args, args, n * sizeof(StageArgs));
-    HIP_TRY(hipMemcpyAsync(s->d_args, s->h_args, n * sizeof(StageArgs), hipMemcpyHostToDevice, s->stream));
     s->h_begin[0] = 0;
     s->h_begin[1] = args[0].ntiles;
     s->h_begin[2] = 0;
     s->h_begin[3] = s->nblk;
+    return CDC_OK;
+}
+
+// The stage launches of n staged stages: argument upload, then per stage the
+// stage kernel (fused: also finalizing the stage before) and, unfused, its
+// finalize; fused, the last stage's finalize at the end.  Timing events are
+// bound to each stage dispatch.
+static int enqueue_stages(cdc_solver *s, size_t n, int ntiles, const cdc::TileShape &ts, hipEvent_t *ev,
+                          bool finalize, bool fuse) {
+    HIP_TRY(hipMemcpyAsync(s->d_args, s->h_args, n * sizeof(StageArgs), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_begin, s->h_begin, 4 * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
     fuse = fuse && finalize;
     for (size_t i = 0; i < n; ++i) {
         // fused: this launch also finalizes stage i - 1 (its tile sums are in
         // the other parity buffer, its ist / oec inputs are complete)
         const StageArgs *fin = fuse && i > 0 ? s->d_args + i - 1 : nullptr;
-        HIP_TRY(cdc::launch_stage_batch(s->nv, ts, s->d_args + i, s->d_begin, 1, args[i].ntiles, fin, s->nblk,
+        HIP_TRY(cdc::launch_stage_batch(s->nv, ts, s->d_args + i, s->d_begin, 1, ntiles, fin, s->nblk,
                                         ev ? ev[2 * i] : nullptr, ev ? ev[2 * i + 1] : nullptr, s->stream));
         if (finalize && !fuse)
             HIP_TRY(cdc::launch_finalize_batch(s->nv, s->d_args + i, s->d_begin + 2, 1, s->nblk, s->stream));
     }
     if (fuse && n > 0)
         HIP_TRY(cdc::launch_finalize_batch(s->nv, s->d_args + n - 1, s->d_begin + 2, 1, s->nblk, s->stream));
+    return CDC_OK;
+}
+
+int launch_stages(cdc_solver *s, const StageArgs *args, size_t n, const cdc::TileShape &ts, hipEvent_t *ev,
+                  bool finalize, bool fuse) {
+    int rc = stage_args(s, args, n);
+    if (rc) return rc;
+    return enqueue_stages(s, n, args[0].ntiles, ts, ev, finalize, fuse);
+}
+
+// One advance's device work, fused, from the staged arguments: flag reset,
+// stages, the flag read into s->h_flag.  Replayed from a cached graph when
+// capture works on this device, else launched directly.  With timing events
+// it is launched directly, the events bound to each stage dispatch: events
+// recorded as nodes of a captured graph cannot be timed on ROCm.
+static int enqueue_advance(cdc_solver *s, size_t n, int ntiles, hipEvent_t *ev) {
+    const bool timed = ev != nullptr;
+    if (s->use_graph && !timed) {
+        for (const auto &g : s->graphs)
+            if (g.n == n && g.ts.tx == s->ts.tx && g.ts.ty == s->ts.ty && g.ts.nt == s->ts.nt) {
+                HIP_TRY(hipGraphLaunch(g.exec, s->stream));
+                return CDC_OK;
+            }
+        HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        int rc = CDC_OK;
+        if (cdc::launch_set_int(s->d_err_flag, 0, s->stream) != hipSuccess) rc = set_msg(CDC_ERR_HIP, "set_int");
+        if (!rc) rc = enqueue_stages(s, n, ntiles, s->ts, nullptr, true, true);
+        if (!rc && hipMemcpyAsync(s->h_flag, s->d_err_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream) !=
+                       hipSuccess)
+            rc = set_msg(CDC_ERR_HIP, "flag read-back");
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(s->stream, &graph);
+        hipGraphExec_t exec = nullptr;
+        if (!rc && ec == hipSuccess && graph &&
+            hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            if (s->graphs.size() >= 4) {
+                (void)hipGraphExecDestroy(s->graphs.front().exec);
+                s->graphs.erase(s->graphs.begin());
+            }
+            s->graphs.push_back({n, s->ts, exec});
+            HIP_TRY(hipGraphLaunch(exec, s->stream));
+            return CDC_OK;
+        }
+        // capture is not available here: launch directly from now on
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        s->use_graph = false;
+    }
+    HIP_TRY(cdc::launch_set_int(s->d_err_flag, 0, s->stream));
+    int rc = enqueue_stages(s, n, ntiles, s->ts, ev, true, true);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(s->h_flag, s->d_err_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
     return CDC_OK;
 }
 
@@ -583,7 +653,8 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
     s->namax = 1024;
     ALLOC(s->d_amax, (size_t)s->namax * sizeof(double));
     ALLOC(s->d_begin, 4 * sizeof(int32_t));
-    if (hipHostMalloc((void **)&s->h_begin, 4 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&s->h_begin, 4 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&s->h_flag, sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         return fail(CDC_ERR_OUT_OF_MEMORY, "pinned staging");
     }
@@ -941,13 +1012,12 @@ static int advance_at(cdc_solver_t *s, int32_t fold, double dt, cdc_error_t *err
             s->ev.push_back(e);
         }
     }
-    HIP_TRY(cdc::launch_set_int(s->d_err_flag, 0, s->stream));
-    rc = cdc::host::launch_stages(s, plan.args.data(), nst, s->ts, s->timing ? s->ev.data() : nullptr, true, true);
+    rc = cdc::host::stage_args(s, plan.args.data(), nst);
     if (rc) return rc;
-
-    int32_t flag = 0;
-    HIP_TRY(hipMemcpyAsync(&flag, s->d_err_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+    rc = cdc::host::enqueue_advance(s, nst, plan.args[0].ntiles, s->timing ? s->ev.data() : nullptr);
+    if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(s->stream));
+    int32_t flag = *s->h_flag;
     if (flag != 0 && nst > 1) {
         // The fused launches may have run stages after a failed finalize;
         // repeat the call unfused from the untouched committed state, where
