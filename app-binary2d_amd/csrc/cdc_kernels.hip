@@ -987,8 +987,8 @@ __global__ void __launch_bounds__(256) column_max_kernel(const double *x, int n,
 // Host launchers
 
 // Supported tile shapes (TX x TY cells, NT threads), largest first
-static const TileShape kShapes[] = {{16, 32, 256}, {32, 16, 256}, {16, 16, 256},
-                                    {8, 32, 256},  {32, 8, 256},  {8, 8, 64}};
+static const TileShape kShapes[] = {{16, 32, 256}, {32, 16, 256}, {16, 16, 256}, {8, 32, 256},
+                                    {32, 8, 256},  {8, 16, 128},  {16, 8, 128},  {8, 8, 64}};
 
 int tile_shapes(const TileShape **out) {
     *out = kShapes;
@@ -1030,6 +1030,8 @@ static hipError_t launch_batch_nv(const TileShape &ts, const StageArgs *args, co
     CDC_SHAPE(16, 16, 256)
     CDC_SHAPE(8, 32, 256)
     CDC_SHAPE(32, 8, 256)
+    CDC_SHAPE(8, 16, 128)
+    CDC_SHAPE(16, 8, 128)
     CDC_SHAPE(8, 8, 64)
 #undef CDC_SHAPE
     return hipErrorInvalidValue;

@@ -317,7 +317,8 @@ def test_timing_counts_stage_launches():
     assert n == 6 and ms > 0 and cells == 64 * 64
 
 
-SHAPES = [(16, 32, 256), (32, 16, 256), (16, 16, 256), (8, 32, 256), (32, 8, 256), (8, 8, 64)]
+SHAPES = [(16, 32, 256), (32, 16, 256), (16, 16, 256), (8, 32, 256), (32, 8, 256), (8, 16, 128), (16, 8, 128),
+          (8, 8, 64)]
 
 
 @pytest.mark.parametrize("kind", ["iso", "euler"])
