@@ -266,8 +266,8 @@ int32_t cdc_solver_max_signal_speed(cdc_solver_t *solver, double *out);
 
 /* Tile shape (tx rows along x by ty columns along y, inside one block) of
  * the fused stage kernel.  Supported: 16x32, 32x16, 16x16, 8x32, 32x8 (256
- * threads), 8x8 (64 threads), each dividing block_size; (0, 0) restores the
- * default.  No reference counterpart: the reference has no tiles.
+ * threads), 8x16, 16x8 (128 threads), 8x8 (64 threads), each dividing
+ * block_size; (0, 0) restores the default.  No reference counterpart: the reference has no tiles.
  * set_kernel / get_kernel also name the thread count (threads = 0: the
  * shape's only one). */
 int32_t cdc_solver_set_tile(cdc_solver_t *solver, int32_t tx, int32_t ty);
