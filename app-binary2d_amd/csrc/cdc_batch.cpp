@@ -109,8 +109,8 @@ int enqueue_step(cdc_ensemble *e, int nrun, int total_tiles, int total_blocks) {
     HIP_TRY(cdc::launch_batch_reset(e->d_args, nrun, e->stream));
     for (int k = 0; k < e->rk; ++k) {
         const cdc::StageArgs *a = e->d_args + (size_t)k * nrun;
-        HIP_TRY(cdc::launch_stage_batch(e->nv, e->ts, a, e->d_tile_begin, nrun, total_tiles, nullptr, 0, nullptr,
-                                        nullptr, e->stream));
+        HIP_TRY(cdc::launch_stage_batch(e->nv, e->ts, a, e->d_tile_begin, nrun, total_tiles, nullptr, nullptr, 0,
+                                        nullptr, nullptr, e->stream));
         HIP_TRY(cdc::launch_finalize_batch(e->nv, a, e->d_block_begin, nrun, total_blocks, e->stream));
     }
     HIP_TRY(cdc::launch_batch_tail(e->nv, e->d_args + (size_t)(e->rk - 1) * nrun, nrun, e->d_amax, e->d_flags,

@@ -89,12 +89,13 @@ hipError_t launch_aux(const AuxArgs &a, hipStream_t s);
 // One RK stage (fused stage kernel, then the per-block finalize) over a batch
 // of runs: device arrays of StageArgs, one per run; tile_begin / block_begin
 // hold nruns + 1 prefix offsets.  A single run is a batch of one.  With fin
-// (a single run only), the launch also finalizes the earlier stage whose
-// arguments fin points to, over its nblk blocks, in workgroups ahead of the
-// tiles.  ev0 / ev1 (either may be null) take the launch's start and end.
+// (the earlier stage's per-run arguments), the launch also finalizes that
+// stage over the nblk blocks that block_begin divides among the runs, in
+// workgroups ahead of the tiles.  ev0 / ev1 (either may be null) take the
+// launch's start and end.
 hipError_t launch_stage_batch(int nv, const TileShape &ts, const StageArgs *args, const int32_t *tile_begin,
-                              int nruns, int ntiles, const StageArgs *fin, int nblk, hipEvent_t ev0, hipEvent_t ev1,
-                              hipStream_t s);
+                              int nruns, int ntiles, const StageArgs *fin, const int32_t *block_begin, int nblk,
+                              hipEvent_t ev0, hipEvent_t ev1, hipStream_t s);
 hipError_t launch_finalize_batch(int nv, const StageArgs *args, const int32_t *block_begin, int nruns,
                                  int nblocks, hipStream_t s);
 hipError_t launch_batch_reset(const StageArgs *args, int nruns, hipStream_t s);

@@ -708,10 +708,10 @@ constexpr int kFinLds = 7 * 4 + 7 * 4 + 1;   // doubles of per-wave scratch (nv 
 
 template <int NV>
 __device__ __forceinline__ void finalize_block(const StageArgs &A, const int b, const int tpb, const int lane,
-                                               const bool active, double *s_ds, double *s_de, int *s_perr) {
+                                               bool active, double *s_ds, double *s_de, int *s_perr) {
     constexpr int NS = SumMap<NV>::n, NR = NS + 1;
     static_assert(NS + 7 * NV <= 64, "one wave per block");
-    const int ef = *(volatile const int32_t *)A.err_flag;
+    const int ef = active ? *(volatile const int32_t *)A.err_flag : 0;
     const size_t oi = (size_t)b * 7 * NV, oe = (size_t)b * 7 * 4;
     double ist_c = 0.0, ist_b = 0.0, oec_c = 0.0, oec_b = 0.0, tq_c = 0.0, tq_b = 0.0;
     if (active) {
@@ -749,10 +749,10 @@ __device__ __forceinline__ void finalize_block(const StageArgs &A, const int b, 
         }
     }
     if (lane == 0) *s_perr = 0;
-    // a stage that already failed (err_flag holds an earlier stage id) writes
-    // nothing; every lane of the workgroup reads the same flag, so the return
-    // is uniform
-    if (ef != 0 && ef < A.stage_id) return;
+    // a run whose earlier stage already failed (err_flag holds an earlier
+    // stage id) writes nothing; the wave stays for the barriers (the waves
+    // of one workgroup may belong to different runs)
+    if (ef != 0 && ef < A.stage_id) active = false;
     __syncthreads();
     if (active) {
         if (lane < 4) {
@@ -820,24 +820,28 @@ finalize_kernel_batch(const StageArgs *__restrict__ args, const int32_t *__restr
 // global tiles begin[r] .. begin[r + 1] - 1, and each run reads its own
 // StageArgs.  One-cell-per-thread tiles fit four 256-thread workgroups per
 // CU in LDS; ask the register allocator for the matching four waves per SIMD.
-// With fin (a single run), the first nfin workgroups finalize the earlier
-// stage whose arguments fin holds, one block per wave, ahead of the tiles;
-// nfin is a multiple of 8, so the tiles keep their XCD mapping.  The two
-// stages use different tile-sum buffers and the tiles read nothing the
-// finalize writes.
+// With fin, the first nfin workgroups finalize the earlier stage whose
+// per-run arguments fin holds (blocks of run r: fin_begin[r] ..
+// fin_begin[r + 1] - 1), one block per wave, ahead of the tiles; nfin is a
+// multiple of 8, so the tiles keep their XCD mapping.  The two stages use
+// different tile-sum buffers and the tiles read nothing the finalize writes.
 template <int NV, int TX, int TY, int NT>
 __global__ void __launch_bounds__(NT)
 __attribute__((amdgpu_waves_per_eu((TX * TY == NT) ? 4 : 2)))
 stage_kernel_batch(const StageArgs *__restrict__ args, const int32_t *__restrict__ begin, int nruns,
-                   const StageArgs *__restrict__ fin, int nfin, int fin_blocks) {
+                   const StageArgs *__restrict__ fin, const int32_t *__restrict__ fin_begin, int fin_nruns,
+                   int nfin) {
     static_assert((NT / 64) * kFinLds <= TileLds<NV, TX, TY, NT>::size, "finalize scratch fits the tile LDS");
     __shared__ double lds[TileLds<NV, TX, TY, NT>::size];
     __shared__ int s_key;
     if ((int)blockIdx.x < nfin) {
         const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const int b = blockIdx.x * (NT / 64) + w;
+        const bool active = b < fin_begin[fin_nruns];
+        const int r = active ? run_of(fin_begin, fin_nruns, b) : 0;
         double *sc = lds + w * kFinLds;
-        finalize_block<NV>(*fin, b, fin->tpb, lane, b < fin_blocks, sc, sc + 7 * 4, (int *)(sc + 7 * 4 + 7 * 4));
+        finalize_block<NV>(fin[r], active ? b - fin_begin[r] : 0, fin[r].tpb, lane, active, sc, sc + 7 * 4,
+                           (int *)(sc + 7 * 4 + 7 * 4));
         return;
     }
     const int t = xcd_tile((int)blockIdx.x - nfin, begin[nruns]);
@@ -1006,25 +1010,26 @@ TileShape tile_shape_for(int bs, int nv) {
 // (hipExtLaunchKernelGGL): no marker packets between dependent launches.
 template <int NV, int TX, int TY, int NT>
 static hipError_t launch_batch_t(const StageArgs *args, const int32_t *begin, int nruns, int ntiles,
-                                 const StageArgs *fin, int nblk, hipEvent_t ev0, hipEvent_t ev1, hipStream_t s) {
+                                 const StageArgs *fin, const int32_t *fin_begin, int fin_nruns, int nblk,
+                                 hipEvent_t ev0, hipEvent_t ev1, hipStream_t s) {
     constexpr int WPG = NT / 64;   // blocks finalized per workgroup
     const int nfin = fin ? ((nblk + WPG - 1) / WPG + 7) / 8 * 8 : 0;
     if (ev0 || ev1)
         hipExtLaunchKernelGGL((stage_kernel_batch<NV, TX, TY, NT>), dim3(ntiles + nfin), dim3(NT), 0, s, ev0, ev1, 0,
-                              args, begin, nruns, fin, nfin, nblk);
+                              args, begin, nruns, fin, fin_begin, fin_nruns, nfin);
     else
         hipLaunchKernelGGL((stage_kernel_batch<NV, TX, TY, NT>), dim3(ntiles + nfin), dim3(NT), 0, s, args, begin,
-                           nruns, fin, nfin, nblk);
+                           nruns, fin, fin_begin, fin_nruns, nfin);
     return hipGetLastError();
 }
 
 template <int NV>
 static hipError_t launch_batch_nv(const TileShape &ts, const StageArgs *args, const int32_t *begin, int nruns,
-                                  int ntiles, const StageArgs *fin, int nblk, hipEvent_t e0, hipEvent_t e1,
-                                  hipStream_t s) {
+                                  int ntiles, const StageArgs *fin, const int32_t *fin_begin, int fin_nruns,
+                                  int nblk, hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
 #define CDC_SHAPE(X, Y, T)                                                                        \
     if (ts.tx == X && ts.ty == Y && ts.nt == T)                                                   \
-        return launch_batch_t<NV, X, Y, T>(args, begin, nruns, ntiles, fin, nblk, e0, e1, s);
+        return launch_batch_t<NV, X, Y, T>(args, begin, nruns, ntiles, fin, fin_begin, fin_nruns, nblk, e0, e1, s);
     CDC_SHAPE(16, 32, 256)
     CDC_SHAPE(32, 16, 256)
     CDC_SHAPE(16, 16, 256)
@@ -1038,11 +1043,10 @@ static hipError_t launch_batch_nv(const TileShape &ts, const StageArgs *args, co
 }
 
 hipError_t launch_stage_batch(int nv, const TileShape &ts, const StageArgs *args, const int32_t *tile_begin,
-                              int nruns, int ntiles, const StageArgs *fin, int nblk, hipEvent_t ev0, hipEvent_t ev1,
-                              hipStream_t s) {
-    if (fin && nruns != 1) return hipErrorInvalidValue;
-    return nv == 3 ? launch_batch_nv<3>(ts, args, tile_begin, nruns, ntiles, fin, nblk, ev0, ev1, s)
-                   : launch_batch_nv<4>(ts, args, tile_begin, nruns, ntiles, fin, nblk, ev0, ev1, s);
+                              int nruns, int ntiles, const StageArgs *fin, const int32_t *block_begin, int nblk,
+                              hipEvent_t ev0, hipEvent_t ev1, hipStream_t s) {
+    return nv == 3 ? launch_batch_nv<3>(ts, args, tile_begin, nruns, ntiles, fin, block_begin, nruns, nblk, ev0, ev1, s)
+                   : launch_batch_nv<4>(ts, args, tile_begin, nruns, ntiles, fin, block_begin, nruns, nblk, ev0, ev1, s);
 }
 
 hipError_t launch_finalize_batch(int nv, const StageArgs *args, const int32_t *block_begin, int nruns,
