@@ -459,7 +459,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile, d
     double kval = 0.0;
     if (tid == 0) s_key = INT_MAX;
     // all of this thread's halo-tile loads first, then the update phase's
-    // operands, so that their latencies overlap each other and phases 1-3
+    // operands, so that their latencies overlap each other and phases 1-3;
+    // issued at raised wave priority, so that a starting tile's requests go
+    // out ahead of the arithmetic of the other workgroups on the CU
+    __builtin_amdgcn_s_setprio(3);
     constexpr int KE = (EXY + NT - 1) / NT;
     double ue[KE][NV];
 #pragma unroll
@@ -481,9 +484,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile, d
         const int c = tid + q * NT;
         const int ci = c / TY, cj = c - (c / TY) * TY;
         const size_t g = (size_t)(gi0 + ci) * N + (gj0 + cj);
-#pragma unroll
         // read once per stage: streamed, so that L2 keeps the halo lines the
         // neighbouring tiles re-read
+#pragma unroll
         for (int v = 0; v < NV; ++v) {
             pu0[q][v] = __builtin_nontemporal_load(&gu0[v * NN + g]);
             pub[q][v] = A.read_base ? __builtin_nontemporal_load(&gub[v * NN + g]) : 0.0;
@@ -491,6 +494,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile, d
         pbr[q] = __builtin_nontemporal_load(&gaux[g]);
         pom[q] = A.k.cooling_model == CDC_COOLING_BETA ? __builtin_nontemporal_load(&gaux[NN + g]) : 0.0;
     }
+    __builtin_amdgcn_s_setprio(0);
 #pragma unroll
     for (int q = 0; q < KE; ++q) {
         const int e = tid + q * NT;
