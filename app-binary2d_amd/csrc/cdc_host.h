@@ -88,6 +88,12 @@ struct cdc_solver {
     size_t cap_args = 0;
     int32_t *h_begin = nullptr, *d_begin = nullptr;   // pinned staging and device copy
     int32_t *h_flag = nullptr;   // pinned read-back of the error flag
+    // max signal speed of an advance's result, folded by its last finalize
+    // into d_amax_adv and read back into pinned h_amax; amax_host holds it
+    // for the committed state of version amax_host_version
+    double *d_amax_adv = nullptr, *h_amax = nullptr;
+    double amax_host = 0.0;
+    uint64_t amax_host_version = UINT64_MAX;
 
     // the device work of one untimed advance as a replayable graph, per
     // stage count and tile shape (the arguments are re-read from the pinned

@@ -66,6 +66,9 @@ struct StageArgs {
     const double *tq_c, *tq_b;
     double *tq_o;
     int32_t *perr;                 // [nblk] perturbation error of the block
+    // when set (the last stage of an advance), the finalize also folds the
+    // blocks' max signal speeds into *amax_out (zeroed beforehand)
+    double *amax_out;
     int32_t N, nb, bs, ntiles, tpb;  // tpb: tiles per block
     int32_t stage_id, read_base, want_amax, reuse_prim;
     double R, L;           // domain radius, block length
@@ -106,6 +109,9 @@ hipError_t launch_max_speed(int nv, const double *u, int64_t ncell, double gm, d
 hipError_t launch_aos_to_soa(int nv, int nb, int bs, const double *aos, double *soa, hipStream_t s);
 hipError_t launch_soa_to_aos(int nv, int nb, int bs, const double *soa, double *aos, hipStream_t s);
 hipError_t launch_set_int(int32_t *p, int32_t v, hipStream_t s);
+// Zeroes an advance's error flag and max-speed slot (a kernel, so that a
+// captured graph holds no memset node).
+hipError_t launch_reset_advance(int32_t *flag, double *amax, hipStream_t s);
 // r in 1..16 (a power of two), down = 1 for averaging, 0 for injection
 hipError_t launch_resample(int nv, const double *src, int64_t Ns, double *dst, int64_t Nd, int r, int down,
                            hipStream_t s);

@@ -714,7 +714,7 @@ template <int NV>
 __device__ __forceinline__ void finalize_block(const StageArgs &A, const int b, const int tpb, const int lane,
                                                bool active, double *s_ds, double *s_de, int *s_perr) {
     constexpr int NS = SumMap<NV>::n, NR = NS + 1;
-    static_assert(NS + 7 * NV <= 64, "one wave per block");
+    static_assert(NS + 1 + 7 * NV <= 64, "one wave per block");
     const int ef = active ? *(volatile const int32_t *)A.err_flag : 0;
     const size_t oi = (size_t)b * 7 * NV, oe = (size_t)b * 7 * 4;
     double ist_c = 0.0, ist_b = 0.0, oec_c = 0.0, oec_b = 0.0, tq_c = 0.0, tq_b = 0.0;
@@ -730,23 +730,30 @@ __device__ __forceinline__ void finalize_block(const StageArgs &A, const int b, 
             tq_c = A.tq_c[(size_t)b * 2 + (lane - 32)];
             if (A.read_base) tq_b = A.tq_b[(size_t)b * 2 + (lane - 32)];
         }
-        if (lane < NS) {
+        if (lane < NS || (lane == NS && A.amax_out)) {
             double t = 0.0;
             // tiles in order; unrolled so that the loads are in flight together
             const auto *ts = dev::gptr(A.tile_sums) + (size_t)b * tpb * NR + lane;
             int j = 0;
+            const bool mx = lane == NS;   // the max-speed column: maxima, not sums
             for (; j + 8 <= tpb; j += 8) {
                 double v[8];
 #pragma unroll
                 for (int q = 0; q < 8; ++q) v[q] = ts[(size_t)(j + q) * NR];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) t += v[q];
+                for (int q = 0; q < 8; ++q) t = mx ? fmax(t, v[q]) : t + v[q];
             }
-            for (; j < tpb; ++j) t += ts[(size_t)j * NR];
-            s_ds[SumMap<NV>::item(lane) * NV + SumMap<NV>::comp(lane)] = t * A.area;
-        } else if (lane < NS + 7 * NV) {
+            for (; j < tpb; ++j) t = mx ? fmax(t, ts[(size_t)j * NR]) : t + ts[(size_t)j * NR];
+            if (!mx) {
+                s_ds[SumMap<NV>::item(lane) * NV + SumMap<NV>::comp(lane)] = t * A.area;
+            } else if (t > 0.0) {
+                // speeds are >= 0, whose bit patterns order like their values:
+                // an integer maximum, exact and independent of block order
+                atomicMax((unsigned long long *)A.amax_out, (unsigned long long)__double_as_longlong(t));
+            }
+        } else if (lane > NS && lane < NS + 1 + 7 * NV) {
             // structurally zero components
-            const int z = lane - NS;
+            const int z = lane - NS - 1;
             bool nonzero = false;
             for (int s = 0; s < NS; ++s) nonzero |= (SumMap<NV>::item(s) * NV + SumMap<NV>::comp(s)) == z;
             if (!nonzero) s_ds[z] = 0.0;
@@ -977,6 +984,11 @@ __global__ void __launch_bounds__(256) resample_kernel(int nv, const double *src
 
 __global__ void set_int_kernel(int32_t *p, int32_t v) { *p = v; }
 
+__global__ void reset_advance_kernel(int32_t *flag, double *amax) {
+    *flag = 0;
+    *amax = 0.0;
+}
+
 // max over a strided column (the per-tile max signal speed slot), one block
 __global__ void __launch_bounds__(256) column_max_kernel(const double *x, int n, int stride, double *out) {
     __shared__ double red[256];
@@ -1114,6 +1126,11 @@ hipError_t launch_resample(int nv, const double *src, int64_t Ns, double *dst, i
 
 hipError_t launch_set_int(int32_t *p, int32_t v, hipStream_t s) {
     hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, p, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_advance(int32_t *flag, double *amax, hipStream_t s) {
+    hipLaunchKernelGGL(reset_advance_kernel, dim3(1), dim3(1), 0, s, flag, amax);
     return hipGetLastError();
 }
 

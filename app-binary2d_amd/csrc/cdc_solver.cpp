@@ -168,6 +168,8 @@ void free_all(cdc_solver *s) {
     if (s->h_args) (void)hipHostFree(s->h_args);
     if (s->h_begin) (void)hipHostFree(s->h_begin);
     if (s->h_flag) (void)hipHostFree(s->h_flag);
+    if (s->h_amax) (void)hipHostFree(s->h_amax);
+    (void)hipFree(s->d_amax_adv);
     for (auto e : s->ev) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
@@ -305,11 +307,14 @@ static int enqueue_advance(cdc_solver *s, size_t n, int ntiles, hipEvent_t *ev) 
             }
         HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         int rc = CDC_OK;
-        if (cdc::launch_set_int(s->d_err_flag, 0, s->stream) != hipSuccess) rc = set_msg(CDC_ERR_HIP, "set_int");
+        if (cdc::launch_reset_advance(s->d_err_flag, s->d_amax_adv, s->stream) != hipSuccess)
+            rc = set_msg(CDC_ERR_HIP, "reset");
         if (!rc) rc = enqueue_stages(s, n, ntiles, s->ts, nullptr, true, true);
-        if (!rc && hipMemcpyAsync(s->h_flag, s->d_err_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream) !=
-                       hipSuccess)
-            rc = set_msg(CDC_ERR_HIP, "flag read-back");
+        if (!rc && (hipMemcpyAsync(s->h_flag, s->d_err_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream) !=
+                        hipSuccess ||
+                    hipMemcpyAsync(s->h_amax, s->d_amax_adv, sizeof(double), hipMemcpyDeviceToHost, s->stream) !=
+                        hipSuccess))
+            rc = set_msg(CDC_ERR_HIP, "read-back");
         hipGraph_t graph = nullptr;
         const hipError_t ec = hipStreamEndCapture(s->stream, &graph);
         hipGraphExec_t exec = nullptr;
@@ -329,14 +334,20 @@ static int enqueue_advance(cdc_solver *s, size_t n, int ntiles, hipEvent_t *ev) 
         (void)hipGetLastError();
         s->use_graph = false;
     }
-    HIP_TRY(cdc::launch_set_int(s->d_err_flag, 0, s->stream));
+    HIP_TRY(cdc::launch_reset_advance(s->d_err_flag, s->d_amax_adv, s->stream));
     int rc = enqueue_stages(s, n, ntiles, s->ts, ev, true, true);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(s->h_flag, s->d_err_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->h_amax, s->d_amax_adv, sizeof(double), hipMemcpyDeviceToHost, s->stream));
     return CDC_OK;
 }
 
 int compute_amax(cdc_solver *s, double *out) {
+    if (s->amax_host_version == s->version) {
+        // left by the advance that produced the committed state
+        *out = s->amax_host;
+        return CDC_OK;
+    }
     std::vector<double> h;
     if (s->amax_valid) {
         // max signal speed of the committed state, left by the last stage
@@ -654,8 +665,10 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
     s->namax = 1024;
     ALLOC(s->d_amax, (size_t)s->namax * sizeof(double));
     ALLOC(s->d_begin, 4 * sizeof(int32_t));
+    ALLOC(s->d_amax_adv, sizeof(double));
     if (hipHostMalloc((void **)&s->h_begin, 4 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&s->h_flag, sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&s->h_flag, sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&s->h_amax, sizeof(double), hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         return fail(CDC_ERR_OUT_OF_MEMORY, "pinned staging");
     }
@@ -1006,6 +1019,9 @@ static int advance_at(cdc_solver_t *s, int32_t fold, double dt, cdc_error_t *err
     StepPlan plan;
     cdc::host::plan_steps(s, fold, dt, plan);
     const size_t nst = plan.args.size();
+    // the last stage's finalize leaves the result's max signal speed (the
+    // next call's dt) in d_amax_adv
+    plan.args.back().amax_out = s->d_amax_adv;
     if (s->timing) {
         while (s->ev.size() < 2 * nst) {
             hipEvent_t e;
@@ -1024,11 +1040,13 @@ static int advance_at(cdc_solver_t *s, int32_t fold, double dt, cdc_error_t *err
         // repeat the call unfused from the untouched committed state, where
         // a failed stage skips the rest, so that the flag and the records
         // name the first error as the reference reports it.
-        HIP_TRY(cdc::launch_set_int(s->d_err_flag, 0, s->stream));
+        HIP_TRY(cdc::launch_reset_advance(s->d_err_flag, s->d_amax_adv, s->stream));
         rc = cdc::host::launch_stages(s, plan.args.data(), nst, s->ts, nullptr, true, false);
         if (rc) return rc;
-        HIP_TRY(hipMemcpyAsync(&flag, s->d_err_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->h_flag, s->d_err_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->h_amax, s->d_amax_adv, sizeof(double), hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
+        flag = *s->h_flag;
     }
     if (s->timing) {
         for (size_t i = 0; i < nst; ++i) {
@@ -1040,6 +1058,8 @@ static int advance_at(cdc_solver_t *s, int32_t fold, double dt, cdc_error_t *err
     }
     if (flag == 0) {
         cdc::host::commit(s, plan);
+        s->amax_host = *s->h_amax;
+        s->amax_host_version = s->version;
         return CDC_OK;
     }
     return cdc::host::resolve_error(s, flag, plan, err);

@@ -262,6 +262,25 @@ def test_unbound_orbit_error():
     check_states(g, o)
 
 
+@pytest.mark.parametrize("kind", ["iso", "euler"])
+def test_max_speed_left_by_an_advance(kind):
+    """The last finalize of an advance folds the blocks' max signal speeds
+    into one value (the next dt's); it must equal a fresh reduction over the
+    whole state, and the oracle's."""
+    mesh, hydro, ph = H.small(kind, nb=2, bs=32, R=4.0, rk_order="RK2")
+    u0 = H.initial(mesh, hydro, ph)
+    g, o = pair(mesh, hydro, ph, u0)
+    for _ in range(3):
+        g.advance(2)
+        o.advance(2)
+        folded = g.max_signal_speed()
+        st = g.get_state()
+        fresh = Solver(mesh, hydro, ph, st["conserved"])
+        assert folded == fresh.max_signal_speed()
+        assert folded == pytest.approx(o.max_signal_speed(), rel=1e-12)
+        fresh.close()
+
+
 @pytest.mark.parametrize("kind,kw", [
     ("euler", dict(rk_order="RK3", viscosity={"alpha": 0.1}, binary_eccentricity=0.5, binary_mass_ratio=0.5)),
     ("iso", dict(rk_order="RK2", sink_model="torque_free")),
