@@ -21,8 +21,11 @@
 // (src/scheme.rs:307-315, src/state.rs:195-197).
 //
 // FP64 reciprocals and square roots use the hardware estimates (v_rcp_f64,
-// v_rsq_f64) refined by Newton-Raphson steps to within an ulp or so of the
-// IEEE results, at a third of the instructions of the IEEE expansions.
+// v_rsq_f64, about 5.6e-8 relative) refined by one Newton-Raphson step:
+// measured over 524,288 log-uniform inputs in 1e-18 .. 1e18, 1/x within
+// 10.8 ulp, 1/sqrt(x) within 19 ulp, sqrt(x) within 0.5 ulp of the exact
+// values, at a fraction of the instructions of the IEEE expansions.  The
+// max signal speed, which sets dt, uses IEEE division and sqrt.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <limits.h>
@@ -74,36 +77,30 @@ __device__ __forceinline__ gdouble *gptr(double *p) { return (gdouble *)p; }
 
 // ---- refined hardware estimates -------------------------------------------
 
+// 1 / x: one Newton step from the estimate
 __device__ __forceinline__ double frcp(double x) {
-    double y = __builtin_amdgcn_rcp(x);
-    double e = __builtin_fma(-x, y, 1.0);
-    y = __builtin_fma(y, e, y);
-    e = __builtin_fma(-x, y, 1.0);
+    const double y = __builtin_amdgcn_rcp(x);
+    const double e = __builtin_fma(-x, y, 1.0);
     return __builtin_fma(y, e, y);
 }
 
-// 1 / sqrt(x), x > 0: coupled Newton iteration on (g ~ sqrt x, h ~ 1/(2 sqrt x))
+// 1 / sqrt(x), x > 0: one step of the coupled Newton iteration on
+// (g ~ sqrt x, h ~ 1/(2 sqrt x))
 __device__ __forceinline__ double frsq(double x) {
     const double y = __builtin_amdgcn_rsq(x);
-    double h = 0.5 * y, g = x * y;
-    double r = __builtin_fma(-h, g, 0.5);
-    h = __builtin_fma(h, r, h);
-    g = __builtin_fma(g, r, g);
-    r = __builtin_fma(-h, g, 0.5);
-    h = __builtin_fma(h, r, h);
-    return h + h;
+    const double h = 0.5 * y, g = x * y;
+    const double r = __builtin_fma(-h, g, 0.5);
+    return 2.0 * __builtin_fma(h, r, h);
 }
 
-// sqrt(x), x >= 0
+// sqrt(x), x >= 0: one coupled step, then one residual correction of g
 __device__ __forceinline__ double fsqrt(double x) {
     const double y = __builtin_amdgcn_rsq(x);
     double h = 0.5 * y, g = x * y;
-    double r = __builtin_fma(-h, g, 0.5);
+    const double r = __builtin_fma(-h, g, 0.5);
     h = __builtin_fma(h, r, h);
     g = __builtin_fma(g, r, g);
-    double d = __builtin_fma(-g, g, x);
-    g = __builtin_fma(d, h, g);
-    d = __builtin_fma(-g, g, x);
+    const double d = __builtin_fma(-g, g, x);
     g = __builtin_fma(d, h, g);
     return x > 0.0 ? g : x;
 }
