@@ -139,6 +139,7 @@ SIGNATURES = {
     "cdc_abi_version": (c_int32, []),
     "cdc_struct_size": (c_int32, [c_int32]),
     "cdc_device_count": (c_int32, []),
+    "cdc_math_probe": (c_int32, [c_void_p, c_void_p, c_int32]),
     "cdc_error_string": (c_char_p, [c_int32]),
     "cdc_last_error_message": (c_char_p, []),
     "cdc_solver_create": (c_int32, [POINTER(CMesh), POINTER(CHydro), POINTER(CPhysics),

@@ -112,6 +112,8 @@ hipError_t launch_set_int(int32_t *p, int32_t v, hipStream_t s);
 // Zeroes an advance's error flag and max-speed slot (a kernel, so that a
 // captured graph holds no memset node).
 hipError_t launch_reset_advance(int32_t *flag, double *amax, hipStream_t s);
+// out[3 i .. 3 i + 2] = the stage kernel's 1/x, 1/sqrt(x), sqrt(x) of x[i]
+hipError_t launch_math_probe(const double *x, double *out, int n, hipStream_t s);
 // r in 1..16 (a power of two), down = 1 for averaging, 0 for injection
 hipError_t launch_resample(int nv, const double *src, int64_t Ns, double *dst, int64_t Nd, int r, int down,
                            hipStream_t s);

@@ -981,6 +981,15 @@ __global__ void __launch_bounds__(256) resample_kernel(int nv, const double *src
 
 __global__ void set_int_kernel(int32_t *p, int32_t v) { *p = v; }
 
+// the refined estimates, for measuring their error
+__global__ void __launch_bounds__(256) math_probe_kernel(const double *x, double *out, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    out[3 * (size_t)i + 0] = dev::frcp(x[i]);
+    out[3 * (size_t)i + 1] = dev::frsq(x[i]);
+    out[3 * (size_t)i + 2] = dev::fsqrt(x[i]);
+}
+
 __global__ void reset_advance_kernel(int32_t *flag, double *amax) {
     *flag = 0;
     *amax = 0.0;
@@ -1123,6 +1132,11 @@ hipError_t launch_resample(int nv, const double *src, int64_t Ns, double *dst, i
 
 hipError_t launch_set_int(int32_t *p, int32_t v, hipStream_t s) {
     hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, p, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_math_probe(const double *x, double *out, int n, hipStream_t s) {
+    hipLaunchKernelGGL(math_probe_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, out, n);
     return hipGetLastError();
 }
 

@@ -563,6 +563,25 @@ int32_t cdc_struct_size(int32_t which) {
     }
 }
 
+int32_t cdc_math_probe(const double *x, double *out, int32_t n) {
+    if (!x || !out || n < 0) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null pointer or negative count");
+    if (n == 0) return CDC_OK;
+    double *dx = nullptr, *dout = nullptr;
+    if (hipMalloc((void **)&dx, (size_t)n * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&dout, (size_t)n * 3 * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(dx);
+        return set_msg(CDC_ERR_OUT_OF_MEMORY, "math probe buffers");
+    }
+    hipError_t e = hipMemcpy(dx, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = cdc::launch_math_probe(dx, dout, n, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out, dout, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(dx);
+    (void)hipFree(dout);
+    if (e != hipSuccess) return set_msg(CDC_ERR_HIP, std::string("math probe: ") + hipGetErrorString(e));
+    return CDC_OK;
+}
+
 int32_t cdc_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) {

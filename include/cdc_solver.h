@@ -164,6 +164,12 @@ int32_t cdc_struct_size(int32_t which);  /* 0 mesh, 1 hydro, 2 physics, 3 error,
 /* Number of visible HIP devices (0 when none). */
 int32_t cdc_device_count(void);
 
+/* The stage kernel's FP64 1/x, 1/sqrt(x) and sqrt(x) (hardware estimates
+ * refined by one Newton step) of n host values on the current device:
+ * out[3 i + 0 .. 2] for x[i].  No reference counterpart: it exposes the
+ * device arithmetic so that its error bounds can be measured. */
+int32_t cdc_math_probe(const double *x, double *out, int32_t n);
+
 /* Static text for a return code. */
 const char *cdc_error_string(int32_t code);
 
