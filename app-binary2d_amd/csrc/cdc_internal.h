@@ -71,6 +71,7 @@ struct StageArgs {
     double *amax_out;
     int32_t N, nb, bs, ntiles, tpb;  // tpb: tiles per block
     int32_t stage_id, read_base, want_amax, reuse_prim;
+    int32_t cool_factor;   // beta cooling reads exp(-dt Omega / beta) from aux plane 2
     double R, L;           // domain radius, block length
     double time, dt, c1, c0, area;
     double el[4];          // Physics::orbital_elements
@@ -112,6 +113,8 @@ hipError_t launch_set_int(int32_t *p, int32_t v, hipStream_t s);
 // Zeroes an advance's error flag and max-speed slot (a kernel, so that a
 // captured graph holds no memset node).
 hipError_t launch_reset_advance(int32_t *flag, double *amax, hipStream_t s);
+// aux plane 2 = exp(-dt Omega / beta) from plane 1 and a->dt (nn cells)
+hipError_t launch_cooling_factor(const StageArgs *a, double *aux, int64_t nn, hipStream_t s);
 // out[3 i .. 3 i + 2] = the stage kernel's 1/x, 1/sqrt(x), sqrt(x) of x[i]
 hipError_t launch_math_probe(const double *x, double *out, int n, hipStream_t s);
 // r in 1..16 (a power of two), down = 1 for averaging, 0 for injection

@@ -298,11 +298,12 @@ __device__ __forceinline__ void face_flux(const StageArgs &A, int axis, const do
 // Isothermal / Euler source_terms (src/physics.rs:528-582, 683-774) with
 // Physics::source_terms (src/physics.rs:423-457).  `sd, vx, vy, id, p0` is
 // the primitive recovery of u without floors; `brate` the static buffer
-// rate, `omega` the static beta-cooling frequency.  it[item][v].
+// rate; `cool` the beta-cooling frequency Omega, or, when A.cool_factor, the
+// relaxation factor exp(-dt Omega / beta) itself.  it[item][v].
 template <int NV>
 __device__ __forceinline__ void cell_sources(const StageArgs &A, const double *u, const double *u0, double x,
                                              double y, double vx, double vy, double id, double p0,
-                                             double brate, double omega, double it[7][NV]) {
+                                             double brate, double cool, double it[7][NV]) {
     const StageConsts &k = A.k;
     const Orbit &o = A.orb;
     const double dt = A.dt;
@@ -371,7 +372,7 @@ __device__ __forceinline__ void cell_sources(const StageArgs &A, const double *u
             const double bvx = u0[1] * ib0, bvy = u0[2] * ib0;
             const double bp = (u0[3] - 0.5 * u0[0] * (bvx * bvx + bvy * bvy)) * k.gm1;
             const double et = bp * (ib0 * k.inv_gm1);
-            ec = et + (e0 - et) * exp(-dt * omega * k.cooling_inv_beta);
+            ec = et + (e0 - et) * (A.cool_factor ? cool : exp(-dt * cool * k.cooling_inv_beta));
         } else {
             const double sdi = k.density_index == 1 ? sd : (k.density_index == 2 ? sd * sd : pow(sd, (double)k.density_index));
             const double a = k.cooling_coefficient / sdi * k.cooling_f;
@@ -489,7 +490,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile, d
             pub[q][v] = A.read_base ? __builtin_nontemporal_load(&gub[v * NN + g]) : 0.0;
         }
         pbr[q] = __builtin_nontemporal_load(&gaux[g]);
-        pom[q] = A.k.cooling_model == CDC_COOLING_BETA ? __builtin_nontemporal_load(&gaux[NN + g]) : 0.0;
+        pom[q] = A.k.cooling_model == CDC_COOLING_BETA
+                     ? __builtin_nontemporal_load(&gaux[(A.cool_factor ? 2 : 1) * NN + g])
+                     : 0.0;
     }
     __builtin_amdgcn_s_setprio(0);
 #pragma unroll
@@ -601,9 +604,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile, d
         }
         if constexpr (NV == 4) p0 = (uc[3] - 0.5 * uc[0] * (vx * vx + vy * vy)) * A.k.gm1;
         const double brate = pbr[q];
-        const double omega = pom[q];
+        const double cool = pom[q];
         double it[7][NV];
-        dev::cell_sources<NV>(A, uc, u0, axx.c(li0 + ci), axy.c(lj0 + cj), vx, vy, id, p0, brate, omega, it);
+        dev::cell_sources<NV>(A, uc, u0, axx.c(li0 + ci), axy.c(lj0 + cj), vx, vy, id, p0, brate, cool, it);
         double out[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -981,6 +984,17 @@ __global__ void __launch_bounds__(256) resample_kernel(int nv, const double *src
 
 __global__ void set_int_kernel(int32_t *p, int32_t v) { *p = v; }
 
+// The beta-cooling relaxation factor exp(-dt Omega / beta) of every cell for
+// an advance (one dt for all its stages): aux plane 2 from plane 1 (Omega),
+// the same expression the stage kernel evaluates when it is not given.
+__global__ void __launch_bounds__(256) cooling_factor_kernel(const StageArgs *__restrict__ a, double *aux,
+                                                              int64_t nn) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= nn) return;
+    const double dt = a->dt, cool = aux[nn + c];
+    aux[2 * nn + c] = exp(-dt * cool * a->k.cooling_inv_beta);
+}
+
 // the refined estimates, for measuring their error
 __global__ void __launch_bounds__(256) math_probe_kernel(const double *x, double *out, int n) {
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -1132,6 +1146,11 @@ hipError_t launch_resample(int nv, const double *src, int64_t Ns, double *dst, i
 
 hipError_t launch_set_int(int32_t *p, int32_t v, hipStream_t s) {
     hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, p, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_cooling_factor(const StageArgs *a, double *aux, int64_t nn, hipStream_t s) {
+    hipLaunchKernelGGL(cooling_factor_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, a, aux, nn);
     return hipGetLastError();
 }
 

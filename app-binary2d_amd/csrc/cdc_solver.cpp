@@ -269,6 +269,9 @@ static int enqueue_stages(cdc_solver *s, size_t n, int ntiles, const cdc::TileSh
                           bool finalize, bool fuse) {
     HIP_TRY(hipMemcpyAsync(s->d_args, s->h_args, n * sizeof(StageArgs), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_begin, s->h_begin, 4 * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+    // every stage of an advance has the first one's dt
+    if (s->h_args[0].cool_factor)
+        HIP_TRY(cdc::launch_cooling_factor(s->d_args, s->d_aux, (int64_t)s->ncell(), s->stream));
     fuse = fuse && finalize;
     for (size_t i = 0; i < n; ++i) {
         // fused: this launch also finalizes stage i - 1 (its tile sums are in
@@ -679,7 +682,9 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
     ALLOC(s->d_tile_err, (size_t)s->max_tiles * sizeof(cdc::TileErr));
     ALLOC(s->d_perr, (size_t)s->nblk * sizeof(int32_t));
     ALLOC(s->d_err_flag, sizeof(int32_t));
-    s->naux = (hydro->kind == CDC_EULER && hydro->cooling_model == CDC_COOLING_BETA) ? 2 : 1;
+    // planes: buffer rate; with beta cooling also Omega and the per-advance
+    // relaxation factor exp(-dt Omega / beta)
+    s->naux = (hydro->kind == CDC_EULER && hydro->cooling_model == CDC_COOLING_BETA) ? 3 : 1;
     ALLOC(s->d_aux, (size_t)s->naux * s->ncell() * sizeof(double));
     s->namax = 1024;
     ALLOC(s->d_amax, (size_t)s->namax * sizeof(double));
@@ -1039,8 +1044,10 @@ static int advance_at(cdc_solver_t *s, int32_t fold, double dt, cdc_error_t *err
     cdc::host::plan_steps(s, fold, dt, plan);
     const size_t nst = plan.args.size();
     // the last stage's finalize leaves the result's max signal speed (the
-    // next call's dt) in d_amax_adv
+    // next call's dt) in d_amax_adv; with beta cooling the stages read the
+    // advance's relaxation factor from aux plane 2
     plan.args.back().amax_out = s->d_amax_adv;
+    for (auto &a : plan.args) a.cool_factor = s->naux > 2 ? 1 : 0;
     if (s->timing) {
         while (s->ev.size() < 2 * nst) {
             hipEvent_t e;
