@@ -308,8 +308,12 @@ static int enqueue_advance(cdc_solver *s, size_t n, int ntiles, hipEvent_t *ev) 
                 HIP_TRY(hipGraphLaunch(g.exec, s->stream));
                 return CDC_OK;
             }
-        HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         int rc = CDC_OK;
+        if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            s->use_graph = false;
+            return enqueue_advance(s, n, ntiles, ev);
+        }
         if (cdc::launch_reset_advance(s->d_err_flag, s->d_amax_adv, s->stream) != hipSuccess)
             rc = set_msg(CDC_ERR_HIP, "reset");
         if (!rc) rc = enqueue_stages(s, n, ntiles, s->ts, nullptr, true, true);
