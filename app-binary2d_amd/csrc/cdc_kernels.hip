@@ -718,6 +718,7 @@ __device__ __forceinline__ void finalize_block(const StageArgs &A, const int b, 
     const int ef = active ? *(volatile const int32_t *)A.err_flag : 0;
     const size_t oi = (size_t)b * 7 * NV, oe = (size_t)b * 7 * 4;
     double ist_c = 0.0, ist_b = 0.0, oec_c = 0.0, oec_b = 0.0, tq_c = 0.0, tq_b = 0.0;
+    double bmax = 0.0;   // lane NS: the block's max signal speed (when A.amax_out)
     if (active) {
         if (lane < 7 * 4) {
             if (lane < 7 * NV) {
@@ -744,13 +745,8 @@ __device__ __forceinline__ void finalize_block(const StageArgs &A, const int b, 
                 for (int q = 0; q < 8; ++q) t = mx ? fmax(t, v[q]) : t + v[q];
             }
             for (; j < tpb; ++j) t = mx ? fmax(t, ts[(size_t)j * NR]) : t + ts[(size_t)j * NR];
-            if (!mx) {
-                s_ds[SumMap<NV>::item(lane) * NV + SumMap<NV>::comp(lane)] = t * A.area;
-            } else if (t > 0.0) {
-                // speeds are >= 0, whose bit patterns order like their values:
-                // an integer maximum, exact and independent of block order
-                atomicMax((unsigned long long *)A.amax_out, (unsigned long long)__double_as_longlong(t));
-            }
+            if (!mx) s_ds[SumMap<NV>::item(lane) * NV + SumMap<NV>::comp(lane)] = t * A.area;
+            else bmax = t;
         } else if (lane > NS && lane < NS + 1 + 7 * NV) {
             // structurally zero components
             const int z = lane - NS - 1;
@@ -781,6 +777,12 @@ __device__ __forceinline__ void finalize_block(const StageArgs &A, const int b, 
     }
     __syncthreads();
     if (!active) return;
+    if (lane == NS && bmax > 0.0) {
+        // after the last barrier, so that no wait covers this atomic.  Speeds
+        // are >= 0, whose bit patterns order like their values: an integer
+        // maximum, exact and independent of the blocks' order
+        atomicMax((unsigned long long *)A.amax_out, (unsigned long long)__double_as_longlong(bmax));
+    }
     if (*s_perr) {
         if (lane == 0) {
             A.perr[b] = CDC_ERR_UNBOUND_ORBIT;
