@@ -144,9 +144,14 @@ int32_t cdc_ensemble_create(cdc_solver_t *const *members, int32_t n, cdc_ensembl
         while (b) { int t = a % b; a = b; b = t; }
         bs_gcd = a;
     }
-    // one tile shape for every member: the first supported shape that
-    // divides every block size (block sizes are multiples of 8)
-    const cdc::TileShape ts = cdc::tile_shape_for(bs_gcd, m0->nv);
+    // one tile shape for every member: the shape the members already share
+    // when it divides every block size, else the first supported shape that
+    // does (block sizes are multiples of 8)
+    cdc::TileShape ts = m0->ts;
+    bool shared = bs_gcd % ts.tx == 0 && bs_gcd % ts.ty == 0;
+    for (int i = 1; i < n && shared; ++i)
+        shared = members[i]->ts.tx == ts.tx && members[i]->ts.ty == ts.ty && members[i]->ts.nt == ts.nt;
+    if (!shared) ts = cdc::tile_shape_for(bs_gcd, m0->nv);
     for (int i = 0; i < n; ++i) {
         const int rc = cdc_solver_set_kernel(members[i], ts.tx, ts.ty, ts.nt);
         if (rc) return rc;

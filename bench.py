@@ -97,6 +97,18 @@ def run_ensemble(args):
     members = workloads.ensemble_members()
     inits = [workloads.initial_state(m) for m in members]
     solvers = [Solver(m["mesh"], m["hydro"], m["physics"], u) for m, u in zip(members, inits)]
+    # one stage kernel for all members: the given tile, or the autotuner's
+    # pick on the largest member (most of the work)
+    if args.tile:
+        shape = tuple(int(v) for v in args.tile.lower().split("x"))
+        for g in solvers:
+            g.set_tile(*shape)
+    elif not args.no_autotune:
+        big = max(solvers, key=lambda g: g.mesh.total_zones())
+        big.autotune(reps=3)
+        kernel = big.get_kernel()
+        for g in solvers:
+            g.set_kernel(*kernel)
     ens = Ensemble(solvers)
     for _ in range(args.warmup):
         ens.advance()

@@ -304,9 +304,10 @@ int32_t cdc_solver_get_timing(cdc_solver_t *solver, int64_t *launches,
  * cdc_solver_advance(member, 1, ...) would have.  A member that fails keeps
  * its pre-call state and reports the error cdc_solver_advance would have
  * (codes_out[i], errs_out[i]); the others commit.  Creating the ensemble sets
- * every member's tile shape to one shared shape (results of a member depend
- * on its tile shape only through the summation order of the integrated
- * source terms).  Members stay usable on their own; the ensemble does not
+ * every member's tile shape to one shared shape: the one the members already
+ * share when it divides every block size, else the default for the greatest
+ * common block size (results of a member depend on its tile shape only
+ * through the summation order of the integrated source terms).  Members stay usable on their own; the ensemble does not
  * own them and must be destroyed before them.
  *   active:    n flags (NULL = all), dt_out: n (written for active members),
  *   codes_out: n, errs_out: n (either may be NULL).

@@ -137,3 +137,25 @@ def test_ensemble_rejects_mixed_members():
     b = members_euler()[0]
     with pytest.raises(Exception):
         Ensemble([Solver(*a), Solver(*b)])
+
+
+def test_shared_member_tile_is_kept():
+    specs = members_euler()
+    batch = [Solver(*s) for s in specs]
+    for g in batch:
+        g.set_tile(8, 8)
+    ens = Ensemble(batch)
+    assert ens.get_tile() == (8, 8)
+    solo = [Solver(*s) for s in specs]
+    for g in solo:
+        g.set_tile(8, 8)
+    dt, codes, _ = ens.advance()
+    assert np.all(codes == 0)
+    for i, g in enumerate(solo):
+        assert g.advance(1) == dt[i]
+        same_state(g, batch[i])
+    # a shape that does not divide every block size falls back to the default
+    other = [Solver(*s) for s in specs]
+    other[0].set_tile(32, 16)
+    other[1].set_tile(16, 16)
+    assert Ensemble(other).get_tile() != (32, 16)
