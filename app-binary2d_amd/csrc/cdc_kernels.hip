@@ -412,7 +412,11 @@ struct TileLds {
     static constexpr int CHUNK_FIT = (A - NR * CH) / NT;
     static constexpr int CHUNK = CHUNK_FIT >= NR ? NR : (CHUNK_FIT < 1 ? 1 : CHUNK_FIT);
     static constexpr int R = NR * CH + CHUNK * NT;
-    static constexpr int size = A > R ? A : R;
+    // coordinate table of the tile, kept apart from the reused space: x
+    // vertices [XV, XV + TX], x centres, y vertices, y centres
+    static constexpr int XV = 0, XC = TX + 1, YV = 2 * TX + 1, YC = 2 * TX + TY + 2, NGEO = 2 * (TX + TY) + 2;
+    static constexpr int geo = A > R ? A : R;
+    static constexpr int size = geo + NGEO;
 };
 
 // One RK stage on one TX x TY tile of one block (the P, F and U phases of
@@ -447,10 +451,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile, d
     const dev::Axis axy = dev::block_axis(A.R, A.L, bj, bs);
     const bool alpha = A.k.viscosity_model == CDC_VISC_ALPHA;
 
+    using TL = TileLds<NV, TX, TY, NT>;
+    static_assert(TL::NGEO <= NT, "one thread per coordinate table entry");
     double *P = lds;
     double *C = lds + NV * EXY;
     double *GX = C + NC;
     double *GY = GX + NV * RXY;
+    double *G = lds + TL::geo;
 
     // ---- phase 1: primitives on the tile + 2-zone halo --------------------
     int key = INT_MAX, kcode = 0;
@@ -495,6 +502,16 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile, d
                      : 0.0;
     }
     __builtin_amdgcn_s_setprio(0);
+    // face and cell coordinates of the tile, once (bit-identical to the
+    // per-face evaluation of the block's linspace)
+    if (tid < TL::NGEO) {
+        double x;
+        if (tid < TL::XC) x = axx.v(li0 + tid);
+        else if (tid < TL::YV) x = axx.c(li0 + tid - TL::XC);
+        else if (tid < TL::YC) x = axy.v(lj0 + tid - TL::YV);
+        else x = axy.c(lj0 + tid - TL::YC);
+        G[tid] = x;
+    }
 #pragma unroll
     for (int q = 0; q < KE; ++q) {
         const int e = tid + q * NT;
@@ -549,8 +566,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile, d
             const int eL = isy ? (fi + 2) * EY + fj + 1 : (fi + 1) * EY + fj + 2;
             const int rL = isy ? (fi + 1) * RY + fj : fi * RY + fj + 1;
             const int de = isy ? 1 : EY, dr = isy ? 1 : RY;
-            const double fxp = isy ? axx.c(li0 + fi) : axx.v(li0 + fi);
-            const double fyp = isy ? axy.v(lj0 + fj) : axy.c(lj0 + fj);
+            const double fxp = G[(isy ? TL::XC : TL::XV) + fi];
+            const double fyp = G[(isy ? TL::YV : TL::YC) + fj];
             dev::face_flux<NV, EXY, RXY>(A, isy, P, C, GX, GY, eL, eL + de, rL, rL + dr, fxp, fyp, fl[q]);
         }
     }
@@ -606,7 +623,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile, d
         const double brate = pbr[q];
         const double cool = pom[q];
         double it[7][NV];
-        dev::cell_sources<NV>(A, uc, u0, axx.c(li0 + ci), axy.c(lj0 + cj), vx, vy, id, p0, brate, cool, it);
+        dev::cell_sources<NV>(A, uc, u0, G[TL::XC + ci], G[TL::YC + cj], vx, vy, id, p0, brate, cool, it);
         double out[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
