@@ -131,39 +131,82 @@ class Stalled(RuntimeError):
     """The time step collapsed (a run that would never reach its end)."""
 
 
-def run_to(step, solver, t_end, log=None, every=2000, min_dt_ratio=1e-2):
-    """Advance with `step(solver)` (which returns the time advanced) until
-    the solver's time reaches t_end.  Raises Stalled when a step advances
-    less than min_dt_ratio of the first one.  Returns (steps, wall seconds)."""
+def run_to(step, solver, t_end, log=None, every=2000, min_dt_ratio=1e-2, fold=1):
+    """Advance with `step(solver)` (which returns the time advanced, at most
+    `fold` CFL steps) until the solver's time is exactly t_end: once the next
+    call could pass t_end, the rest is covered by equal plain steps no longer
+    than the CFL step (advance_dt).  Raises Stalled when a step advances less
+    than min_dt_ratio of the first one.  Returns (calls, wall seconds)."""
     n = 0
     t0 = time.perf_counter()
-    t = solver.time()
     first = None
-    while t < t_end:
+    while True:
+        t = solver.time()
+        cfl = solver.advance(0)        # the CFL step of the committed state; nothing advanced
+        if t + fold * cfl >= t_end:
+            break
         d = step(solver)
         first = d if first is None else first
         if d < min_dt_ratio * first:
             raise Stalled(f"time step collapsed to {d:.3e} (first {first:.3e}) at t = {t / ORBIT:.4f} orbits")
-        t += d
         n += 1
-        if n % 64 == 0:
-            t = solver.time()      # the exact committed time
         if log and n % every == 0:
             log(f"  t = {t / ORBIT:.3f} orbits after {n} steps")
+    rem = t_end - solver.time()
+    if rem > 0.0:
+        k = max(1, math.ceil(rem / cfl))
+        solver.advance_dt(rem / k, k)
+        n += 1
     return n, time.perf_counter() - t0
 
 
-def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, sample_every=100,
-                    max_samples=200, train_steps=4000, lr=1e-3, seed=5, limiter=0.1, train_mask=0.05,
-                    pair_steps=1, variants=((0.1, 1.0, 0.05),), log=print):
-    """Train a closure for the coarse grid on pairs from the fine workload
-    (the loss over cells with density >= train_mask; each pair compares
-    pair_steps coarse steps with the fine run over the same time),
-    then run plain fine, plain coarse and learned coarse runs over the
-    held-out window [train_orbits, train_orbits + eval_orbits] from the same
-    fine state; one learned run per (limiter kappa, damping, mask density)
-    variant, the damping scaling the trained output.  Returns a dict of wall times per
-    orbit and accretion rates."""
+def heldout_one_step(fine, net, coarse_mesh, hydro, physics, cu0, pairs=16, gap_steps=40):
+    """The trained closure against the plain coarse step on states it was not
+    trained on: from `pairs` down-sampled states of the fine run, gap_steps
+    fine steps apart, one plain and one corrected coarse step (both start from
+    the same state, so both take the same CFL dt), each compared with the fine
+    run advanced over that dt and down-sampled.  Returns the L1 errors per
+    conserved field (Sigma, Sigma vx, Sigma vy) summed over the pairs, plain
+    and learned, and their ratio (below 1: the closure gains)."""
+    from .solver import Solver
+    plain, corr, truth = (Solver(coarse_mesh, hydro, physics, cu0) for _ in range(3))
+    e_plain, e_learned = np.zeros(3), np.zeros(3)
+    try:
+        for _ in range(pairs):
+            fine.advance(gap_steps)
+            plain.resample_from(fine)
+            corr.resample_from(fine)
+            dt = plain.advance(1)
+            dtl = net.advance(corr)
+            if dtl != dt:
+                raise RuntimeError(f"corrected step took dt {dtl!r}, plain step {dt!r}")
+            m = max(1, math.ceil(dt / fine.advance(0)))
+            fine.advance_dt(dt / m, m)
+            truth.resample_from(fine)
+            ut = truth.get_state()["conserved"]
+            up = plain.get_state()["conserved"]
+            ul = corr.get_state()["conserved"]
+            for v in range(3):
+                e_plain[v] += np.abs(up[..., v] - ut[..., v]).sum()
+                e_learned[v] += np.abs(ul[..., v] - ut[..., v]).sum()
+    finally:
+        for s in (plain, corr, truth):
+            s.close()
+    return {"pairs": pairs, "gap_fine_steps": gap_steps, "l1_plain": e_plain.tolist(),
+            "l1_learned": e_learned.tolist(), "ratio": (e_learned / e_plain).tolist()}
+
+
+def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, max_samples=200,
+                  train_steps=4000, lr=1e-3, seed=5, limiter=0.1, train_mask=0.05, pair_steps=1, log=print):
+    """Train a closure for the coarse grid (coarse_blocks^2 blocks, 4x coarser
+    than the fine workload) on pairs taken every sample_every fine steps along
+    the fine run up to train_orbits: each pair compares pair_steps coarse steps
+    with the fine run over the same time, from the same down-sampled state, and
+    the loss runs over cells with density >= train_mask.  Returns a dict with
+    the fine solver at the end of the training window, the trained closure
+    (limiter kappa and mask as given), the coarse mesh, hydro, physics, the
+    coarse initial state, the trained parameters and output scale, and the
+    training figures."""
     from .config import Mesh
     from .solver import Solver, resample_state
     from . import workloads
@@ -183,11 +226,9 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
     t_train = train_orbits * ORBIT
     log(f"training pairs: fine {fm.num_blocks * fm.block_size}^2 to {train_orbits} orbits, coarse {N}^2")
     tp0 = time.perf_counter()
-    nfine = 0
     while fine.time() < t_train:
         for _ in range(max(1, sample_every // 10)):
             fine.advance(10)
-        nfine += max(1, sample_every // 10) * 10
         if net.samples()[0] >= max_samples:
             continue
         base.resample_from(fine)
@@ -197,22 +238,62 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
         dtc = min(plain.advance(0), 4.0 * fine.advance(0))
         plain.advance_dt(dtc, pair_steps)
         fine.advance_dt(dtc / 4.0, 4 * pair_steps)
-        nfine += 4 * pair_steps
         truth.resample_from(fine)
         net.add_sample(base, plain, truth, dtc * pair_steps)
         k = net.samples()[0]
         if k % 20 == 0:
             log(f"  {k} samples, t = {fine.time() / ORBIT:.3f} orbits")
+    for sv in (base, plain, truth):
+        sv.close()
     tpairs = time.perf_counter() - tp0
     nsamp, ms = net.samples()
-    net.set_scales(output_scale=np.sqrt(ms) * dxc, loss_weight=1.0 / ms)
+    scale = np.sqrt(ms) * dxc
+    net.set_scales(output_scale=scale, loss_weight=1.0 / ms)
     log(f"{nsamp} samples in {tpairs:.1f} s; target rms {np.sqrt(ms)}")
 
     tt0 = time.perf_counter()
     hist = net.train(train_steps, lr=lr, seed=seed)
     ttrain = time.perf_counter() - tt0
-    params = net.params()
     log(f"trained {train_steps} steps in {ttrain:.1f} s: loss {hist[:50].mean():.4f} -> {hist[-200:].mean():.4f}")
+    return {"fine": fine, "net": net, "coarse_mesh": cm, "hydro": hydro, "physics": ph, "cu0": cu0,
+            "params": net.params(), "scale": scale,
+            "info": {"fine_grid": fm.num_blocks * fm.block_size, "coarse_grid": N, "samples": nsamp,
+                     "train_steps": train_steps, "loss_first50": float(hist[:50].mean()),
+                     "loss_last200": float(hist[-200:].mean()), "pairs_s": tpairs, "train_s": ttrain}}
+
+
+def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, sample_every=100,
+                    max_samples=200, train_steps=4000, lr=1e-3, seed=5, limiter=0.1, train_mask=0.05,
+                    pair_steps=1, variants=((0.1, 1.0, 0.05),), heldout_pairs=16, log=print):
+    """Train a closure on the fine workload (train_on_fine), measure it on the
+    fine states that follow the training window (heldout_one_step, from a
+    device snapshot of the fine run), then run plain fine, plain coarse and
+    learned coarse runs over the held-out window
+    [train_orbits, train_orbits + eval_orbits] from the same fine state; one
+    learned run per (limiter kappa, damping, mask density) variant, the
+    damping scaling the trained output.  Every run ends exactly at the
+    window's end.  Reported per run: wall time per orbit, the time-averaged
+    accretion rate and torque on the binary over the window, and for the
+    coarse runs the L1 error of Sigma at the window's end against the
+    down-sampled fine state, with the relative errors of the rates against
+    the fine run."""
+    from .solver import Snapshot, Solver, resample_state
+
+    t = train_on_fine(fine_w, coarse_blocks=coarse_blocks, train_orbits=train_orbits, sample_every=sample_every,
+                      max_samples=max_samples, train_steps=train_steps, lr=lr, seed=seed, limiter=limiter,
+                      train_mask=train_mask, pair_steps=pair_steps, log=log)
+    fine, net, cm, hydro, ph, cu0 = t["fine"], t["net"], t["coarse_mesh"], t["hydro"], t["physics"], t["cu0"]
+    fm = fine_w["mesh"]
+    N = cm.num_blocks * cm.block_size
+    params, trained_scale = t["params"], t["scale"]
+    out = dict(t["info"])
+    if heldout_pairs > 0:
+        snap = Snapshot(fine)
+        fine.save(snap)
+        out["heldout_one_step"] = heldout_one_step(fine, net, cm, hydro, ph, cu0, pairs=heldout_pairs)
+        fine.restore(snap)
+        snap.close()
+        log(f"held-out one step: L1 ratio learned / plain {out['heldout_one_step']['ratio']}")
 
     # held-out window from the fine state at the end of training
     t_start = fine.time()
@@ -221,11 +302,7 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
     start.resample_from(fine)
     coarse = Solver(cm, hydro, ph, cu0)
     coarse.resample_from(fine)
-    trained_scale = np.sqrt(ms) * dxc
-    out = {"fine_grid": fm.num_blocks * fm.block_size, "coarse_grid": N, "samples": nsamp,
-           "train_steps": train_steps, "loss_first50": float(hist[:50].mean()),
-           "loss_last200": float(hist[-200:].mean()), "pairs_s": tpairs, "train_s": ttrain,
-           "window_orbits": [t_start / ORBIT, t_end / ORBIT]}
+    out["window_orbits"] = [t_start / ORBIT, t_end / ORBIT]
     # plain runs step as the reference's driver does (fold = 10 steps per
     # advance at one dt); the learned run corrects every step
     runs = [("fine", fine, lambda s: 10 * s.advance(10), None),
@@ -234,6 +311,7 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
         kappa, damping, mask = (tuple(v) + (0.0,))[:3]
         runs.append((f"learned_k{kappa:g}_d{damping:g}_m{mask:g}", None, lambda s: net.advance(s),
                      (kappa, damping, mask)))
+    sig_fine = None
     for name, solver, step, var in runs:
         if var is not None:
             net.close()
@@ -242,23 +320,31 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
             net.set_scales(output_scale=trained_scale * var[1])
             solver = Solver(cm, hydro, ph, cu0)
             solver.resample_from(start)
-        m0, ta = accreted(solver), solver.time()
+        m0, ta, tq0 = accreted(solver), solver.time(), solver.torque().sum()
         try:
-            n, wall = run_to(step, solver, t_end, log=lambda m: log(f"{name}:{m}"), every=1000)
+            n, wall = run_to(step, solver, t_end, log=lambda m: log(f"{name}:{m}"), every=1000,
+                             fold=10 if var is None else 1)
         except (HydroError, Stalled) as e:
             log(f"{name}: failed: {e}")
             out[name] = {"failed": str(e)}
             continue
         tb = solver.time()
         mdot = (accreted(solver) - m0) / (tb - ta)
+        torque = (solver.torque().sum() - tq0) / (tb - ta)
+        u = solver.get_state()["conserved"]
         out[name] = {"advance_calls": n, "wall_s": wall, "orbits": (tb - ta) / ORBIT,
-                     "wall_per_orbit_s": wall / ((tb - ta) / ORBIT), "mdot": mdot}
-        log(f"{name}: {n} steps, {wall:.2f} s, mdot {mdot:.6e}")
+                     "wall_per_orbit_s": wall / ((tb - ta) / ORBIT), "mdot": mdot, "torque": torque}
+        if name == "fine":
+            sig_fine = resample_state(fm, cm, u)[0][..., 0]
+        elif sig_fine is not None:
+            out[name]["sigma_l1_rel"] = float(np.abs(u[..., 0] - sig_fine).sum() / np.abs(sig_fine).sum())
+        log(f"{name}: {n} calls, {wall:.2f} s, mdot {mdot:.6e}, torque {torque:.6e}")
     if "mdot" in out.get("fine", {}):
-        ref = out["fine"]["mdot"]
+        ref, reft = out["fine"]["mdot"], out["fine"]["torque"]
         for name in [r[0] for r in runs[1:]]:
             if "mdot" in out.get(name, {}):
                 out[name]["mdot_rel_err"] = abs(out[name]["mdot"] - ref) / abs(ref)
+                out[name]["torque_rel_err"] = abs(out[name]["torque"] - reft) / abs(reft)
                 out[name]["speedup_vs_fine"] = out["fine"]["wall_per_orbit_s"] / out[name]["wall_per_orbit_s"]
     net.close()
     return out

@@ -5,6 +5,8 @@
   euler2048 gamma-law, 32x32 blocks of 64^2 (2048^2), RK3, alpha viscosity,
             eccentric (e = 0.5) unequal (q = 0.5) binary, beta cooling, pressure
             floor 1e-10 (as the reference's cooling-cbd setup floors pressure), 100 steps
+iso2048   the isothermal physics at 32x32 blocks of 64^2 (2048^2): the fine
+            run of the learned-closure study (coarse grid 512^2)
 ensemble64 (ensemble_members) 64 isothermal RK2 runs of 256^2 to 1024^2 with
             drawn binary and disk parameters, half an orbit each
 Domain [-8a, 8a]^2, a = M = G = 1, softening = sink radius = 0.05, sink rate
@@ -40,6 +42,8 @@ WORKLOADS = {
                    seed=1, steps=None, orbits=1.0),
     "iso1024": dict(mesh=Mesh(16, 64, 8.0), hydro=Isothermal(mach_number=10.0), physics=base_physics(),
                     seed=1, steps=200, orbits=None),
+    "iso2048": dict(mesh=Mesh(32, 64, 8.0), hydro=Isothermal(mach_number=10.0), physics=base_physics(),
+                    seed=5, steps=None, orbits=None),
     "euler2048": dict(mesh=Mesh(32, 64, 8.0),
                       hydro=Euler(gamma_law_index=5.0 / 3.0, cooling_coefficient=0.0, cooling_beta=10.0,
                                   pressure_floor=1e-10),

@@ -159,17 +159,12 @@ def run_ensemble(args):
 
 
 def run_learned(args):
-    """Learned 512^2 closure vs plain 2048^2 (and plain 512^2): wall time
-    per orbit and accretion rate over a held-out orbit."""
+    """Learned 512^2 closure vs plain 2048^2 (and plain 512^2), isothermal:
+    wall time per orbit, accretion rate, torque and Sigma error over a
+    held-out window, and the held-out one-step gain of the closure."""
     from cdc_amd import workloads
     from cdc_amd.closure import learned_vs_fine
-    w = workloads.get("euler2048")
-    # runs of several orbits need the floor the 100-step benchmark does not,
-    # and constant nu: alpha viscosity in the hot rarefied cavity outruns its
-    # explicit limit within a tenth of an orbit (the CFL step of
-    # src/scheme.rs:336-340 has no viscous bound)
-    w["hydro"].density_floor = 1e-6
-    w["physics"].viscosity = {"constant_nu": 1e-3}
+    w = workloads.get("iso2048")
     log = lambda m: print(m, file=sys.stderr, flush=True)
     variants = [tuple(float(x) for x in v.split(":")) for v in args.variants.split(",")]
     r = learned_vs_fine(w, train_orbits=args.train_orbits, eval_orbits=args.eval_orbits,
@@ -186,7 +181,7 @@ def run_learned(args):
         "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 (net: bf16 operands, fp32 accumulation)",
         "data": "synthetic (seeded cavity disk, seed %d); training pairs from the same run" % w["seed"],
-        "config": {"workload": "learned512", "fine": "euler2048 with constant nu 1e-3 and density floor 1e-6",
+        "config": {"workload": "learned512", "fine": "iso2048 (isothermal, 32x32 blocks of 64^2, RK2)",
                    **r},
     }
     print(json.dumps(line), flush=True)
