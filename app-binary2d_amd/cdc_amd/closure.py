@@ -40,6 +40,7 @@ class Closure:
         _check(self.lib, self.lib.cdc_closure_create(byref(c), byref(h)))
         self.handle = h
         self.grid = int(grid)
+        self.capacity = int(max_samples)
 
     def close(self):
         if getattr(self, "handle", None):
@@ -160,14 +161,17 @@ def run_to(step, solver, t_end, log=None, every=2000, min_dt_ratio=1e-2, fold=1)
     return n, time.perf_counter() - t0
 
 
-def heldout_one_step(fine, net, coarse_mesh, hydro, physics, cu0, pairs=16, gap_steps=40):
+def heldout_one_step(fine, net, coarse_mesh, hydro, physics, cu0, pairs=16, gap_steps=40, burn_in=0):
     """The trained closure against the plain coarse step on states it was not
     trained on: from `pairs` down-sampled states of the fine run, gap_steps
     fine steps apart, one plain and one corrected coarse step (both start from
     the same state, so both take the same CFL dt), each compared with the fine
     run advanced over that dt and down-sampled.  Returns the L1 errors per
     conserved field (Sigma, Sigma vx, Sigma vy) summed over the pairs, plain
-    and learned, and their ratio (below 1: the closure gains)."""
+    and learned, and their ratio (below 1: the closure gains).  With
+    burn_in > 0 both steps start from a coarse run stepped burn_in times
+    beside the fine run, and the errors are those of the increments (the
+    step's change against the down-sampled fine run's change)."""
     from .solver import Solver
     plain, corr, truth = (Solver(coarse_mesh, hydro, physics, cu0) for _ in range(3))
     e_plain, e_learned = np.zeros(3), np.zeros(3)
@@ -175,15 +179,21 @@ def heldout_one_step(fine, net, coarse_mesh, hydro, physics, cu0, pairs=16, gap_
         for _ in range(pairs):
             fine.advance(gap_steps)
             plain.resample_from(fine)
-            corr.resample_from(fine)
+            for _ in range(burn_in):
+                step_beside(plain, fine)
+            truth.resample_from(fine)
+            t0 = truth.get_state()["conserved"]
+            ub = plain.get_state()["conserved"]
+            corr.resample_from(plain)
             dt = plain.advance(1)
             dtl = net.advance(corr)
-            if dtl != dt:
+            if abs(dtl - dt) > 1e-12 * dt:
                 raise RuntimeError(f"corrected step took dt {dtl!r}, plain step {dt!r}")
             m = max(1, math.ceil(dt / fine.advance(0)))
             fine.advance_dt(dt / m, m)
             truth.resample_from(fine)
-            ut = truth.get_state()["conserved"]
+            # the state the fine run's increment carries the base to
+            ut = ub + (truth.get_state()["conserved"] - t0)
             up = plain.get_state()["conserved"]
             ul = corr.get_state()["conserved"]
             for v in range(3):
@@ -192,23 +202,82 @@ def heldout_one_step(fine, net, coarse_mesh, hydro, physics, cu0, pairs=16, gap_
     finally:
         for s in (plain, corr, truth):
             s.close()
-    return {"pairs": pairs, "gap_fine_steps": gap_steps, "l1_plain": e_plain.tolist(),
+    return {"pairs": pairs, "gap_fine_steps": gap_steps, "burn_in": burn_in, "l1_plain": e_plain.tolist(),
             "l1_learned": e_learned.tolist(), "ratio": (e_learned / e_plain).tolist()}
 
 
+def step_beside(coarse, fine):
+    """One plain CFL step of the coarse solver, and the fine solver advanced
+    over the same time in equal sub-steps no longer than its own CFL step.
+    Returns the coarse step's dt."""
+    dt = coarse.advance(1)
+    m = max(1, math.ceil(dt / fine.advance(0)))
+    fine.advance_dt(dt / m, m)
+    return dt
+
+
+def loop_pairs(net, fl, snap, coarse_mesh, hydro, physics, cu0, steps=200, every=10):
+    """Model-in-the-loop training pairs: the fine solver fl restarts from the
+    device snapshot snap, the coarse state starts as its down-sampling and is
+    stepped `steps` times with the closure (net.advance) while fl advances
+    over the same time steps (each in equal sub-steps no longer than its CFL
+    step); every `every` steps the pair is (the corrected run's state, its
+    plain coarse step, fl down-sampled at the step's end).  The targets thus
+    hold the drift of the corrected run from the fine trajectory as well as
+    the one-step error.  Returns (pairs added, 1 if the corrected run failed
+    before the end else 0)."""
+    from .solver import Solver
+    c, base, plain, truth = (Solver(coarse_mesh, hydro, physics, cu0) for _ in range(4))
+    added, failed = 0, 0
+    try:
+        fl.restore(snap)
+        c.resample_from(fl)
+        for k in range(steps):
+            take = k % every == 0 and net.samples()[0] < net.capacity
+            if take:
+                base.resample_from(c)
+                plain.resample_from(c)
+                dt = plain.advance(1)
+            try:
+                dtc = net.advance(c)
+            except HydroError:
+                failed = 1
+                break
+            if take and abs(dtc - dt) > 1e-12 * dt:
+                raise RuntimeError(f"corrected step took dt {dtc!r}, plain step {dt!r}")
+            m = max(1, math.ceil(dtc / fl.advance(0)))
+            fl.advance_dt(dtc / m, m)
+            if take:
+                truth.resample_from(fl)
+                net.add_sample(base, plain, truth, dtc)
+                added += 1
+    finally:
+        for s in (c, base, plain, truth):
+            s.close()
+    return added, failed
+
+
 def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, max_samples=200,
-                  train_steps=4000, lr=1e-3, seed=5, limiter=0.1, train_mask=0.05, pair_steps=1, log=print):
+                  train_steps=4000, lr=1e-3, seed=5, limiter=0.1, train_mask=0.05, pair_steps=1,
+                  loop_rounds=0, loop_starts=8, loop_steps=200, loop_every=10, loop_train_steps=2000, burn_in=0,
+                  log=print):
     """Train a closure for the coarse grid (coarse_blocks^2 blocks, 4x coarser
     than the fine workload) on pairs taken every sample_every fine steps along
     the fine run up to train_orbits: each pair compares pair_steps coarse steps
     with the fine run over the same time, from the same down-sampled state, and
-    the loss runs over cells with density >= train_mask.  Returns a dict with
+    the loss runs over cells with density >= train_mask.  With burn_in > 0 a
+    pair instead starts from a coarse run stepped burn_in times beside the
+    fine run, and its target is the difference of the two runs' increments
+    over the next coarse step (their tendencies at that state).  Then loop_rounds
+    rounds of model-in-the-loop pairs (loop_pairs) from loop_starts fine
+    states spread over the training window, each round followed by
+    loop_train_steps more training steps.  Returns a dict with
     the fine solver at the end of the training window, the trained closure
     (limiter kappa and mask as given), the coarse mesh, hydro, physics, the
     coarse initial state, the trained parameters and output scale, and the
     training figures."""
     from .config import Mesh
-    from .solver import Solver, resample_state
+    from .solver import Snapshot, Solver, resample_state
     from . import workloads
 
     fm, hydro, ph = fine_w["mesh"], fine_w["hydro"], fine_w["physics"]
@@ -219,8 +288,11 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
     base, plain, truth = (Solver(cm, hydro, ph, cu0) for _ in range(3))
     N = cm.num_blocks * cm.block_size
     dxc = 2.0 * cm.domain_radius / N
-    net = Closure(N, cm.domain_radius, max_samples=max_samples, seed=seed, init_output_gain=0.1,
-                  limiter=limiter, mask_density=train_mask)
+    per_round = loop_starts * (-(-loop_steps // loop_every))
+    net = Closure(N, cm.domain_radius, max_samples=max_samples + loop_rounds * per_round, seed=seed,
+                  init_output_gain=0.1, limiter=limiter, mask_density=train_mask)
+    snaps = []
+    snap_dt = t_snap = train_orbits * ORBIT / loop_starts if loop_rounds > 0 else None
 
     # training pairs along the fine run
     t_train = train_orbits * ORBIT
@@ -229,17 +301,36 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
     while fine.time() < t_train:
         for _ in range(max(1, sample_every // 10)):
             fine.advance(10)
+        if snap_dt is not None and len(snaps) < loop_starts and fine.time() >= t_snap - snap_dt:
+            snaps.append(Snapshot(fine))
+            fine.save(snaps[-1])
+            t_snap += snap_dt
         if net.samples()[0] >= max_samples:
             continue
         base.resample_from(fine)
-        plain.resample_from(fine)
-        # pair_steps coarse steps of dtc against 4 pair_steps fine steps of dtc / 4:
-        # the target is the mean rate of the difference over that window
-        dtc = min(plain.advance(0), 4.0 * fine.advance(0))
-        plain.advance_dt(dtc, pair_steps)
-        fine.advance_dt(dtc / 4.0, 4 * pair_steps)
-        truth.resample_from(fine)
-        net.add_sample(base, plain, truth, dtc * pair_steps)
+        if burn_in > 0:
+            # tendency target: the base state is a coarse run stepped burn_in
+            # times beside the fine run, and the target is the difference of
+            # the two runs' increments over the next step
+            for _ in range(burn_in):
+                step_beside(base, fine)
+            truth.resample_from(fine)
+            t0 = truth.get_state()["conserved"]
+            plain.resample_from(base)
+            dtc = step_beside(plain, fine)
+            truth.resample_from(fine)
+            t1 = truth.get_state()["conserved"]
+            truth.set_state(base.get_state()["conserved"] + (t1 - t0))
+            net.add_sample(base, plain, truth, dtc)
+        else:
+            plain.resample_from(fine)
+            # pair_steps coarse steps of dtc against 4 pair_steps fine steps of dtc / 4:
+            # the target is the mean rate of the difference over that window
+            dtc = min(plain.advance(0), 4.0 * fine.advance(0))
+            plain.advance_dt(dtc, pair_steps)
+            fine.advance_dt(dtc / 4.0, 4 * pair_steps)
+            truth.resample_from(fine)
+            net.add_sample(base, plain, truth, dtc * pair_steps)
         k = net.samples()[0]
         if k % 20 == 0:
             log(f"  {k} samples, t = {fine.time() / ORBIT:.3f} orbits")
@@ -255,16 +346,35 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
     hist = net.train(train_steps, lr=lr, seed=seed)
     ttrain = time.perf_counter() - tt0
     log(f"trained {train_steps} steps in {ttrain:.1f} s: loss {hist[:50].mean():.4f} -> {hist[-200:].mean():.4f}")
+    loop_info = []
+    if snaps:
+        fl = Solver(fm, hydro, ph, u0)
+        for rnd in range(loop_rounds):
+            tl0 = time.perf_counter()
+            added, failed = 0, 0
+            for sn in snaps:
+                a, f = loop_pairs(net, fl, sn, cm, hydro, ph, cu0, steps=loop_steps, every=loop_every)
+                added += a
+                failed += f
+            h2 = net.train(loop_train_steps, lr=lr, seed=seed + 1 + rnd)
+            loop_info.append({"pairs": added, "failed_runs": failed, "loss_last200": float(h2[-200:].mean()),
+                              "s": time.perf_counter() - tl0})
+            log(f"loop round {rnd}: {added} pairs ({failed} runs failed), loss {h2[-200:].mean():.4f}")
+        fl.close()
+        for sn in snaps:
+            sn.close()
     return {"fine": fine, "net": net, "coarse_mesh": cm, "hydro": hydro, "physics": ph, "cu0": cu0,
             "params": net.params(), "scale": scale,
             "info": {"fine_grid": fm.num_blocks * fm.block_size, "coarse_grid": N, "samples": nsamp,
                      "train_steps": train_steps, "loss_first50": float(hist[:50].mean()),
-                     "loss_last200": float(hist[-200:].mean()), "pairs_s": tpairs, "train_s": ttrain}}
+                     "loss_last200": float(hist[-200:].mean()), "pairs_s": tpairs, "train_s": ttrain,
+                     "loop_rounds": loop_info}}
 
 
 def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, sample_every=100,
                     max_samples=200, train_steps=4000, lr=1e-3, seed=5, limiter=0.1, train_mask=0.05,
-                    pair_steps=1, variants=((0.1, 1.0, 0.05),), heldout_pairs=16, log=print):
+                    pair_steps=1, variants=((0.1, 1.0, 0.05),), heldout_pairs=16, loop_rounds=0, loop_starts=8,
+                    loop_steps=200, loop_every=10, loop_train_steps=2000, burn_in=0, log=print):
     """Train a closure on the fine workload (train_on_fine), measure it on the
     fine states that follow the training window (heldout_one_step, from a
     device snapshot of the fine run), then run plain fine, plain coarse and
@@ -281,7 +391,9 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
 
     t = train_on_fine(fine_w, coarse_blocks=coarse_blocks, train_orbits=train_orbits, sample_every=sample_every,
                       max_samples=max_samples, train_steps=train_steps, lr=lr, seed=seed, limiter=limiter,
-                      train_mask=train_mask, pair_steps=pair_steps, log=log)
+                      train_mask=train_mask, pair_steps=pair_steps, loop_rounds=loop_rounds,
+                      loop_starts=loop_starts, loop_steps=loop_steps, loop_every=loop_every,
+                      loop_train_steps=loop_train_steps, burn_in=burn_in, log=log)
     fine, net, cm, hydro, ph, cu0 = t["fine"], t["net"], t["coarse_mesh"], t["hydro"], t["physics"], t["cu0"]
     fm = fine_w["mesh"]
     N = cm.num_blocks * cm.block_size
@@ -290,7 +402,8 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
     if heldout_pairs > 0:
         snap = Snapshot(fine)
         fine.save(snap)
-        out["heldout_one_step"] = heldout_one_step(fine, net, cm, hydro, ph, cu0, pairs=heldout_pairs)
+        out["heldout_one_step"] = heldout_one_step(fine, net, cm, hydro, ph, cu0, pairs=heldout_pairs,
+                                                   burn_in=burn_in)
         fine.restore(snap)
         snap.close()
         log(f"held-out one step: L1 ratio learned / plain {out['heldout_one_step']['ratio']}")

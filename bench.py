@@ -168,7 +168,9 @@ def run_learned(args):
     log = lambda m: print(m, file=sys.stderr, flush=True)
     variants = [tuple(float(x) for x in v.split(":")) for v in args.variants.split(",")]
     r = learned_vs_fine(w, train_orbits=args.train_orbits, eval_orbits=args.eval_orbits,
-                        train_steps=args.train_steps, pair_steps=args.pair_steps, variants=variants, log=log)
+                        train_steps=args.train_steps, pair_steps=args.pair_steps, variants=variants,
+                        loop_rounds=args.loop_rounds, loop_steps=args.loop_steps, loop_every=args.loop_every,
+                        burn_in=args.burn_in, log=log)
     # the headline: the fastest learned variant within 2% accretion-rate error
     ok = [k for k, v in r.items() if k.startswith("learned") and isinstance(v, dict)
           and v.get("mdot_rel_err", 1.0) <= 0.02]
@@ -205,6 +207,11 @@ def main():
     ap.add_argument("--eval-orbits", type=float, default=2.0)
     ap.add_argument("--train-steps", type=int, default=4000)
     ap.add_argument("--pair-steps", type=int, default=1, help="coarse steps per training pair")
+    ap.add_argument("--loop-rounds", type=int, default=0, help="rounds of model-in-the-loop training pairs")
+    ap.add_argument("--loop-steps", type=int, default=200, help="corrected coarse steps per loop run")
+    ap.add_argument("--loop-every", type=int, default=10, help="coarse steps between loop training pairs")
+    ap.add_argument("--burn-in", type=int, default=0,
+                    help="coarse steps beside the fine run before each pair (tendency targets)")
     ap.add_argument("--variants", default="0.1:1:0.05,0.05:1:0.05,0.02:0.5:0.05",
                     help="learned runs as kappa:damping:mask_density,...")
     args = ap.parse_args()
