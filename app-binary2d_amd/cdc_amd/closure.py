@@ -166,7 +166,8 @@ def heldout_one_step(fine, net, coarse_mesh, hydro, physics, cu0, pairs=16, gap_
     trained on: from `pairs` down-sampled states of the fine run, gap_steps
     fine steps apart, one plain and one corrected coarse step (both start from
     the same state, so both take the same CFL dt), each compared with the fine
-    run advanced over that dt and down-sampled.  Returns the L1 errors per
+    run advanced over that dt and down-sampled (a state from which either
+    coarse step fails is skipped and counted).  Returns the L1 errors per
     conserved field (Sigma, Sigma vx, Sigma vy) summed over the pairs, plain
     and learned, and their ratio (below 1: the closure gains).  With
     burn_in > 0 both steps start from a coarse run stepped burn_in times
@@ -175,6 +176,7 @@ def heldout_one_step(fine, net, coarse_mesh, hydro, physics, cu0, pairs=16, gap_
     from .solver import Solver
     plain, corr, truth = (Solver(coarse_mesh, hydro, physics, cu0) for _ in range(3))
     e_plain, e_learned = np.zeros(3), np.zeros(3)
+    skipped = 0
     try:
         for _ in range(pairs):
             fine.advance(gap_steps)
@@ -185,8 +187,12 @@ def heldout_one_step(fine, net, coarse_mesh, hydro, physics, cu0, pairs=16, gap_
             t0 = truth.get_state()["conserved"]
             ub = plain.get_state()["conserved"]
             corr.resample_from(plain)
-            dt = plain.advance(1)
-            dtl = net.advance(corr)
+            try:
+                dt = plain.advance(1)
+                dtl = net.advance(corr)
+            except HydroError:
+                skipped += 1       # a coarse step failed from this state: no comparison
+                continue
             if abs(dtl - dt) > 1e-12 * dt:
                 raise RuntimeError(f"corrected step took dt {dtl!r}, plain step {dt!r}")
             m = max(1, math.ceil(dt / fine.advance(0)))
@@ -202,7 +208,8 @@ def heldout_one_step(fine, net, coarse_mesh, hydro, physics, cu0, pairs=16, gap_
     finally:
         for s in (plain, corr, truth):
             s.close()
-    return {"pairs": pairs, "gap_fine_steps": gap_steps, "burn_in": burn_in, "l1_plain": e_plain.tolist(),
+    return {"pairs": pairs - skipped, "skipped": skipped, "gap_fine_steps": gap_steps, "burn_in": burn_in,
+            "l1_plain": e_plain.tolist(),
             "l1_learned": e_learned.tolist(), "ratio": (e_learned / e_plain).tolist()}
 
 
@@ -294,19 +301,8 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
     snaps = []
     snap_dt = t_snap = train_orbits * ORBIT / loop_starts if loop_rounds > 0 else None
 
-    # training pairs along the fine run
-    t_train = train_orbits * ORBIT
-    log(f"training pairs: fine {fm.num_blocks * fm.block_size}^2 to {train_orbits} orbits, coarse {N}^2")
-    tp0 = time.perf_counter()
-    while fine.time() < t_train:
-        for _ in range(max(1, sample_every // 10)):
-            fine.advance(10)
-        if snap_dt is not None and len(snaps) < loop_starts and fine.time() >= t_snap - snap_dt:
-            snaps.append(Snapshot(fine))
-            fine.save(snaps[-1])
-            t_snap += snap_dt
-        if net.samples()[0] >= max_samples:
-            continue
+    def pair(base, plain, truth, fine):
+        """Fill base, plain and truth for one training pair; returns its dt."""
         base.resample_from(fine)
         if burn_in > 0:
             # tendency target: the base state is a coarse run stepped burn_in
@@ -321,16 +317,39 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
             truth.resample_from(fine)
             t1 = truth.get_state()["conserved"]
             truth.set_state(base.get_state()["conserved"] + (t1 - t0))
-            net.add_sample(base, plain, truth, dtc)
-        else:
-            plain.resample_from(fine)
-            # pair_steps coarse steps of dtc against 4 pair_steps fine steps of dtc / 4:
-            # the target is the mean rate of the difference over that window
-            dtc = min(plain.advance(0), 4.0 * fine.advance(0))
-            plain.advance_dt(dtc, pair_steps)
-            fine.advance_dt(dtc / 4.0, 4 * pair_steps)
-            truth.resample_from(fine)
-            net.add_sample(base, plain, truth, dtc * pair_steps)
+            return dtc
+        plain.resample_from(fine)
+        # pair_steps coarse steps of dtc against 4 pair_steps fine steps of dtc / 4:
+        # the target is the mean rate of the difference over that window
+        dtc = min(plain.advance(0), 4.0 * fine.advance(0))
+        plain.advance_dt(dtc, pair_steps)
+        fine.advance_dt(dtc / 4.0, 4 * pair_steps)
+        truth.resample_from(fine)
+        return dtc * pair_steps
+
+    # training pairs along the fine run
+    skipped = 0
+    t_train = train_orbits * ORBIT
+    log(f"training pairs: fine {fm.num_blocks * fm.block_size}^2 to {train_orbits} orbits, coarse {N}^2")
+    tp0 = time.perf_counter()
+    while fine.time() < t_train:
+        for _ in range(max(1, sample_every // 10)):
+            fine.advance(10)
+        if snap_dt is not None and len(snaps) < loop_starts and fine.time() >= t_snap - snap_dt:
+            snaps.append(Snapshot(fine))
+            fine.save(snaps[-1])
+            t_snap += snap_dt
+        if net.samples()[0] >= max_samples:
+            continue
+        try:
+            dtc = pair(base, plain, truth, fine)
+        except HydroError as e:
+            # the coarse scheme fails from this down-sampled state (or the
+            # fine run's sub-steps do, leaving it unchanged): no pair
+            skipped += 1
+            log(f"  no pair at t = {fine.time() / ORBIT:.3f} orbits: {e}")
+            continue
+        net.add_sample(base, plain, truth, dtc)
         k = net.samples()[0]
         if k % 20 == 0:
             log(f"  {k} samples, t = {fine.time() / ORBIT:.3f} orbits")
@@ -368,7 +387,7 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
             "info": {"fine_grid": fm.num_blocks * fm.block_size, "coarse_grid": N, "samples": nsamp,
                      "train_steps": train_steps, "loss_first50": float(hist[:50].mean()),
                      "loss_last200": float(hist[-200:].mean()), "pairs_s": tpairs, "train_s": ttrain,
-                     "loop_rounds": loop_info}}
+                     "skipped_pairs": skipped, "loop_rounds": loop_info}}
 
 
 def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, sample_every=100,

@@ -173,16 +173,19 @@ __device__ __forceinline__ int try_to_primitive(const StageConsts &k, const doub
 
 // Hydrodynamics::max_signal_speed of a conserved state: iso
 // max(|vx|, |vy|) (src/physics.rs:521-526), Euler max(|vx|, |vy|) + c_s.
-// IEEE division and sqrt: this value sets dt.
+// Isothermal: max(|Sx|, |Sy|) / |Sigma|, one IEEE division, equal to the max
+// of the two correctly rounded quotients (rounding is monotonic).  Euler:
+// the refined reciprocal and square root (within 11 and 0.5 ulp).
 template <int NV>
 __device__ __forceinline__ double max_signal_speed(const double *u, double gm) {
     if constexpr (NV == 3) {
-        return fmax(fabs(u[1] / u[0]), fabs(u[2] / u[0]));
+        return fmax(fabs(u[1]), fabs(u[2])) / fabs(u[0]);
     } else {
         const double d = u[0];
-        const double vx = u[1] / d, vy = u[2] / d;
+        const double id = frcp(d);
+        const double vx = u[1] * id, vy = u[2] * id;
         const double p = (u[3] - 0.5 * d * (vx * vx + vy * vy)) * (gm - 1.0);
-        return fmax(fabs(vx), fabs(vy)) + sqrt(gm * p / d);
+        return fmax(fabs(vx), fabs(vy)) + fsqrt(gm * p * id);
     }
 }
 

@@ -168,6 +168,7 @@ def run_learned(args):
     log = lambda m: print(m, file=sys.stderr, flush=True)
     variants = [tuple(float(x) for x in v.split(":")) for v in args.variants.split(",")]
     r = learned_vs_fine(w, train_orbits=args.train_orbits, eval_orbits=args.eval_orbits,
+                        sample_every=args.sample_every, max_samples=args.max_samples,
                         train_steps=args.train_steps, pair_steps=args.pair_steps, variants=variants,
                         loop_rounds=args.loop_rounds, loop_steps=args.loop_steps, loop_every=args.loop_every,
                         burn_in=args.burn_in, log=log)
@@ -207,6 +208,8 @@ def main():
     ap.add_argument("--eval-orbits", type=float, default=2.0)
     ap.add_argument("--train-steps", type=int, default=4000)
     ap.add_argument("--pair-steps", type=int, default=1, help="coarse steps per training pair")
+    ap.add_argument("--sample-every", type=int, default=100, help="fine steps between training pairs")
+    ap.add_argument("--max-samples", type=int, default=200, help="training pairs kept")
     ap.add_argument("--loop-rounds", type=int, default=0, help="rounds of model-in-the-loop training pairs")
     ap.add_argument("--loop-steps", type=int, default=200, help="corrected coarse steps per loop run")
     ap.add_argument("--loop-every", type=int, default=10, help="coarse steps between loop training pairs")
