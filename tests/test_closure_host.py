@@ -1,0 +1,67 @@
+"""Host logic of the learned-closure study, with a stand-in solver (no GPU):
+run_to ends a run exactly at the window's end, covering the remainder with
+equal plain steps no longer than the CFL step, and reports a collapsing
+time step; step_beside advances the fine run over the coarse step's dt in
+sub-steps within the fine run's own CFL step."""
+import math
+
+import pytest
+
+from cdc_amd.closure import Stalled, run_to, step_beside
+
+
+class Stand:
+    """time(), advance(fold) at a CFL step (fold 0: report it only),
+    advance_dt(dt, fold) at a given step; the CFL step may shrink per call."""
+
+    def __init__(self, cfl, shrink=1.0):
+        self.t, self.cfl, self.shrink = 0.0, cfl, shrink
+        self.plain = []
+
+    def time(self):
+        return self.t
+
+    def advance(self, fold=1):
+        dt = self.cfl
+        if fold:
+            self.t += fold * dt
+            self.cfl *= self.shrink
+        return dt
+
+    def advance_dt(self, dt, fold=1):
+        assert dt > 0.0
+        self.plain.append((dt, fold))
+        for _ in range(fold):
+            self.t += dt
+
+
+@pytest.mark.parametrize("cfl,fold,t_end", [(0.013, 10, 1.0), (0.013, 1, 1.0), (0.25, 10, 0.1),
+                                            (1e-3, 10, 2 * math.pi)])
+def test_run_to_lands_on_the_window_end(cfl, fold, t_end):
+    s = Stand(cfl)
+    n, _ = run_to(lambda v: fold * v.advance(fold), s, t_end, fold=fold)
+    assert s.t == pytest.approx(t_end, rel=1e-13, abs=0.0)
+    # the remainder: one call of equal plain steps, none longer than the CFL step
+    assert len(s.plain) <= 1
+    for dt, k in s.plain:
+        assert dt <= cfl * (1 + 1e-15) and k <= fold
+    # every regular call fits before t_end
+    regular = n - len(s.plain)
+    assert regular * fold * cfl < t_end + 1e-12
+
+
+def test_run_to_reports_a_collapsing_step():
+    s = Stand(0.01, shrink=0.5)
+    with pytest.raises(Stalled):
+        run_to(lambda v: v.advance(1), s, 10.0)
+
+
+@pytest.mark.parametrize("dtc,dtf", [(0.04, 0.01), (0.04, 0.0099), (0.01, 0.02), (0.03, 0.0071)])
+def test_step_beside_sub_steps_within_the_fine_cfl(dtc, dtf):
+    c, f = Stand(dtc), Stand(dtf)
+    dt = step_beside(c, f)
+    assert dt == dtc
+    (sub, m), = f.plain
+    assert m == max(1, math.ceil(dtc / dtf))
+    assert sub <= dtf * (1 + 1e-15)
+    assert f.t == pytest.approx(c.t, rel=1e-14)
