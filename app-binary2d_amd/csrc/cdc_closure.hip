@@ -122,6 +122,100 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(const ConvArgs a) {
     }
 }
 
+// The same convolution for grids whose size is a multiple of 16: a workgroup
+// takes a 16 x 16 pixel tile of one frame, staged with its periodic 1-pixel
+// halo in LDS (80 B per pixel, so that the 16 B operand reads of 8
+// consecutive lanes hit distinct banks); each wave computes 4 rows x 16
+// columns as two 32-pixel x 32-channel tiles with v_mfma_f32_32x32x16_bf16
+// (lane l supplies row l % 32 of A and column l % 32 of B over
+// k = 8 (l / 32) .. + 7; D as for 32x32x8), B (the layer's weights) held in
+// registers for the whole tile.  K runs over (tap, 16-channel half) blocks.
+// FIRST: the first layer, whose inputs have 4 channels: a block of 16 k
+// covers 4 taps x 4 channels, so the 9 taps take 3 blocks instead of 18.
+constexpr int TT = 16, TH = TT + 2, TPS = 40;   // tile side, halo side, LDS pixel stride (bf16)
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4b __attribute__((ext_vector_type(4)));
+
+__device__ inline bf16x8 cat4(bf16x4b lo, bf16x4b hi) {
+    return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
+template <bool FIRST>
+__global__ void __launch_bounds__(256) conv3x3_tile_kernel(const ConvArgs a) {
+    __shared__ __attribute__((aligned(16))) uint16_t tile[TH * TH * TPS];
+    constexpr int NB = FIRST ? 3 : 18;   // k blocks of 16
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 31, h = lane >> 5;
+    const int n = a.n, nn = n * n, tpr = n / TT, tpf = tpr * tpr;
+    const int f = blockIdx.x / tpf, t = blockIdx.x - f * tpf;
+    const int i0 = (t / tpr) * TT, j0 = (t - (t / tpr) * tpr) * TT;
+    const size_t fb = (size_t)f * nn;
+
+    // weights: column m of B, k = 8 h .. + 7 of every block
+    bf16x8 wb[NB];
+    const uint16_t *wrow = a.w + (size_t)m * K;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        if constexpr (FIRST) {
+            const bf16x4b z = {};
+            const int t0 = 4 * b + 2 * h;
+            const bf16x4b lo = t0 < 9 ? *(const bf16x4b *)(wrow + t0 * C) : z;
+            const bf16x4b hi = t0 + 1 < 9 ? *(const bf16x4b *)(wrow + (t0 + 1) * C) : z;
+            wb[b] = cat4(lo, hi);
+        } else {
+            wb[b] = *(const bf16x8 *)(wrow + (b >> 1) * C + (b & 1) * 16 + 8 * h);
+        }
+    }
+    // the halo tile, 16 B per thread and step (FIRST: the 4 input channels only)
+    constexpr int PARTS = FIRST ? 1 : 4;
+    for (int q = tid; q < TH * TH * PARTS; q += 256) {
+        const int hp = FIRST ? q : q >> 2, part = FIRST ? 0 : q & 3;
+        const int hi = hp / TH, hj = hp - hi * TH;
+        const int gi = wrap(i0 - 1 + hi, n), gj = wrap(j0 - 1 + hj, n);
+        const uint16_t *src = a.x + (fb + (size_t)gi * n + gj) * C + part * 8;
+        if constexpr (FIRST) *(uint2 *)(tile + hp * TPS) = *(const uint2 *)src;
+        else *(uint4 *)(tile + hp * TPS + part * 8) = *(const uint4 *)src;
+    }
+    __syncthreads();
+
+    const int ch = lane & 31;
+    const float bias = a.bias ? a.bias[ch] : 0.0f;
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        // A rows: pixel m of rows 4 wave + 2 half .. + 1
+        const int pr = 4 * wave + 2 * half + (m >> 4), pc = m & 15;
+        f32x16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            bf16x8 av;
+            if constexpr (FIRST) {
+                const bf16x4b z = {};
+                const int t0 = 4 * b + 2 * h, t1 = t0 + 1;
+                const bf16x4b lo = t0 < 9 ? *(const bf16x4b *)(tile + ((pr + t0 / 3) * TH + pc + t0 % 3) * TPS) : z;
+                const bf16x4b hi = t1 < 9 ? *(const bf16x4b *)(tile + ((pr + t1 / 3) * TH + pc + t1 % 3) * TPS) : z;
+                av = cat4(lo, hi);
+            } else {
+                const int tap = b >> 1;
+                av = *(const bf16x8 *)(tile + ((pr + tap / 3) * TH + pc + tap % 3) * TPS + (b & 1) * 16 + 8 * h);
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, wb[b], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int mr = 8 * (r >> 2) + 4 * h + (r & 3);
+            const int gi = i0 + 4 * wave + 2 * half + (mr >> 4), gj = j0 + (mr & 15);
+            const size_t pg = fb + (size_t)gi * n + gj;
+            float v = acc[r] + bias;
+            if (a.relu) v = fmaxf(v, 0.0f);
+            if (a.mask && !(bf2f(a.mask[pg * C + ch]) > 0.0f)) v = 0.0f;
+            if (a.out_bf) a.out_bf[pg * C + ch] = f2bf(v);
+            else if (ch < a.out_ch) a.out_f[pg * YS + ch] = v;
+        }
+    }
+}
+
 struct WgradArgs {
     const uint16_t *g;   // [P][32] output gradient
     const uint16_t *x;   // [P][32] layer input
@@ -478,7 +572,13 @@ int conv(cdc_closure *c, const uint16_t *x, int layer, bool backward, const uint
     a.P = c->P;
     a.relu = (!backward && layer < NL - 1) ? 1 : 0;
     a.out_ch = 6;
-    hipLaunchKernelGGL(conv3x3_kernel, dim3((c->P + 127) / 128), dim3(256), 0, s, a);
+    if (c->n % TT == 0) {
+        const dim3 grid((unsigned)(c->P / (TT * TT)));
+        if (layer == 0 && !backward) hipLaunchKernelGGL(conv3x3_tile_kernel<true>, grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(conv3x3_tile_kernel<false>, grid, dim3(256), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(conv3x3_kernel, dim3((c->P + 127) / 128), dim3(256), 0, s, a);
+    }
     HIP_TRY(hipGetLastError());
     return CDC_OK;
 }
