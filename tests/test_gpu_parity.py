@@ -162,26 +162,55 @@ def test_iso256_one_orbit():
     assert step > 100
 
 
-def test_iso1024_steps():
+def lockstep(w, steps, check_every):
+    """The workload stepped `steps` times in lockstep with the oracle from the
+    same seeded state: dt within 1e-12 and the full state check
+    (check_states) for the first 10 steps, then dt within 1e-9 and the
+    conserved state within 1e-9 every check_every steps and at the end.
+    Returns both solvers and the largest state error seen."""
+    u0 = workloads.initial_state(w)
+    g, o = pair(w["mesh"], w["hydro"], w["physics"], u0)
+    worst = 0.0
+    for k in range(steps):
+        dg, do = g.advance(1), o.advance(1)
+        if k < 10:
+            assert dg == pytest.approx(do, rel=1e-12)
+            worst = max(worst, check_states(g, o))
+        else:
+            assert dg == pytest.approx(do, rel=1e-9)
+            if (k + 1) % check_every == 0 or k == steps - 1:
+                e = H.field_err(g.get_state()["conserved"], o.get_state()["conserved"])
+                assert e <= 1e-9, f"state error {e:.3e} after {k + 1} steps"
+                worst = max(worst, e)
+    return g, o, worst
+
+
+def test_iso1024_200_steps():
+    """The 1024^2 isothermal workload (the drop-in path) for its 200 RK2
+    steps in lockstep with the oracle: the state within 1e-12 at every check
+    (measured 2.0e-14); the accretion items at the end within 1e-9 of the
+    oracle's."""
     w = workloads.get("iso1024")
-    u0 = workloads.initial_state(w)
-    g, o = pair(w["mesh"], w["hydro"], w["physics"], u0)
-    for _ in range(8):
-        assert g.advance(1) == pytest.approx(o.advance(1), rel=1e-12)
-        check_states(g, o)
+    g, o, worst = lockstep(w, 200, 20)
+    print(f"iso1024: largest field-relative state error over 200 steps {worst:.2e}")
+    assert worst <= 1e-12
+    sg, so = g.get_state(), o.get_state()
+    assert sg["iteration"] == (200, 1)
+    mg, mo = np.array(accretion_series(sg)), np.array(accretion_series(so))
+    assert np.max(np.abs(mg - mo)) <= 1e-9 * np.max(np.abs(mo))
 
 
-def test_euler2048_steps_and_mass_budget():
+def test_euler2048_100_steps_and_mass_budget():
+    """The 2048^2 gamma-law workload (alpha viscosity, e = 0.5, q = 0.5, beta
+    cooling, RK3) for its 100 steps in lockstep with the oracle: the state
+    within 1e-12 at every check (measured 3.7e-13); the GPU's mass changes
+    only through the sources."""
     w = workloads.get("euler2048")
-    u0 = workloads.initial_state(w)
-    g, o = pair(w["mesh"], w["hydro"], w["physics"], u0)
-    for _ in range(2):
-        assert g.advance(1) == pytest.approx(o.advance(1), rel=1e-12)
-        check_states(g, o)
+    g, o, worst = lockstep(w, 100, 10)
+    print(f"euler2048: largest field-relative state error over 100 steps {worst:.2e}")
+    assert worst <= 1e-12
     o.close()
-    # full 100-step run on the GPU alone: mass changes only through the sources
-    for _ in range(98):
-        g.advance(1)
+    u0 = workloads.initial_state(w)
     s = g.get_state()
     dA = w["mesh"].cell_spacing() ** 2
     dm = (s["conserved"][..., 0].sum() - u0[..., 0].sum()) * dA
