@@ -79,7 +79,7 @@ struct StageArgs {
     StageConsts k;
 };
 
-// Tile shape chosen for a block size; see DESIGN.md for the LDS budget.
+// Tile shape chosen for a block size (the largest shape that divides it).
 struct TileShape {
     int tx, ty, nt;   // tile of tx x ty cells, nt threads
 };

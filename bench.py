@@ -219,8 +219,9 @@ def main():
                     help="learned runs as kappa:damping:mask_density,...")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              f"(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print(f"bench.py: runs on one GPU only; --gpus {args.gpus} is not supported (the mesh would "
+              f"shard as 2-D sub-domains with one halo exchange per RK stage, not implemented)",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "ensemble64":

@@ -9,8 +9,7 @@
  *                                                       (src/scheme.rs:323-349)
  *
  * and the data it moves (State / BlockState, src/state.rs:27-62).  A Rust
- * caller binds them with an `extern "C"` block; the binding is spelled out in
- * INTEGRATION.md.
+ * caller binds them with an `extern "C"` block over these declarations.
  *
  * Data layout at the boundary (identical to the reference's in-memory arrays,
  * so a checkpoint converts with a memcpy):

@@ -3,7 +3,7 @@
  * update (clemson-cal/app-binary2d: src/scheme.rs, src/physics.rs,
  * src/state.rs, src/mesh.rs).
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP path
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity reference for the HIP path
  * and the timed CPU baseline.  Nothing in the product links, loads or calls
  * it; only the tests, the smoke check and the benchmark's CPU-baseline leg do.
  *

@@ -1,7 +1,7 @@
 """CPU restatement of the learned flux closure (test infrastructure only).
 
-Only tests/ may import this module; it is the checker of the HIP closure
-kernels, never part of the product path.  The closure has no counterpart in
+Only tests/ may import this module; it is the reference the HIP closure
+kernels are compared with, never part of the product path.  The closure has no counterpart in
 the reference: it corrects a coarse run of the per-block RK update
 (src/scheme.rs:323-349) by a flux divergence predicted from the state at the
 start of each step.  Everything here is FP64 numpy.

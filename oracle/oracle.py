@@ -1,6 +1,6 @@
 """ctypes wrapper of the CPU oracle (liboracle: cdc_oracle.c).
 
-TEST INFRASTRUCTURE ONLY: the parity checker and the timed CPU baseline.
+TEST INFRASTRUCTURE ONLY: the parity reference and the timed CPU baseline.
 Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import
 it.  It takes the same configuration structs as the product's C ABI
 (include/cdc_solver.h), built by the caller.

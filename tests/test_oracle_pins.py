@@ -1,5 +1,5 @@
 """Pins of the CPU oracle (oracle/cdc_oracle.c) before it is trusted as the
-parity checker.  The reference has no tests, fixtures or golden vectors, and
+parity reference.  The reference has no tests, fixtures or golden vectors, and
 its arithmetic crates are absent, so each building block is pinned by what
 the mathematics fixes (closed forms, consistency, symmetry, conservation) or
 by the reference's own independent restatement in
