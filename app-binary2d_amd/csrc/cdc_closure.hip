@@ -216,6 +216,137 @@ __global__ void __launch_bounds__(256) conv3x3_tile_kernel(const ConvArgs a) {
     }
 }
 
+// The whole forward pass over one 16 x 16 output tile (inference: no
+// activations kept), for grids whose size is a multiple of 16.  The tile's
+// features with a 4-pixel periodic halo (24 x 24 pixels, 4 channels) go to
+// LDS; layer 1 computes 22 x 22 pixels, layer 2 20 x 20, layer 3 18 x 18 and
+// layer 4 the tile itself, each layer reading the previous one's output from
+// LDS (80 B per pixel), so that no intermediate activation leaves the CU.
+// Operands and layouts as in conv3x3_tile_kernel; 32-pixel MFMA row tiles
+// run over each layer's region in raster order, dealt round-robin to the
+// four waves; the last tile of a region is partly padding (computed, not
+// stored).  Layer 4's 6 outputs are written as fp32 [pixel][YS].
+struct FusedArgs {
+    const uint16_t *x;     // features [P][32] (channels 0..3 used)
+    const uint16_t *w;     // bf16 forward weights [layer][32][K]
+    const float *par;      // fp32 parameters (biases at layer * LP + LW)
+    float *y;              // [P][YS]
+    int n;
+};
+
+constexpr int FR0 = TT + 8, FR1 = TT + 6, FR2 = TT + 4, FR3 = TT + 2;   // region sides
+constexpr int FUSED_LDS = FR0 * FR0 * 4 + FR1 * FR1 * TPS + FR2 * FR2 * TPS;  // bf16 elements
+
+// Column m of B, k = 8 h .. + 7 of every k block, for one layer's weights.
+template <bool FIRST>
+__device__ __forceinline__ void load_weights(const FusedArgs &a, int layer, int lane, bf16x8 *wb) {
+    constexpr int NB = FIRST ? 3 : 18;
+    const int m = lane & 31, h = lane >> 5;
+    const uint16_t *wrow = a.w + (size_t)layer * C * K + (size_t)m * K;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        if constexpr (FIRST) {
+            const bf16x4b z = {};
+            const int t0 = 4 * b + 2 * h;
+            const bf16x4b lo = t0 < 9 ? *(const bf16x4b *)(wrow + t0 * C) : z;
+            const bf16x4b hi = t0 + 1 < 9 ? *(const bf16x4b *)(wrow + (t0 + 1) * C) : z;
+            wb[b] = cat4(lo, hi);
+        } else {
+            wb[b] = *(const bf16x8 *)(wrow + (b >> 1) * C + (b & 1) * 16 + 8 * h);
+        }
+    }
+}
+
+// One layer over its output region: each wave takes row tiles mt and mt + 4
+// together (two independent accumulator chains), mt = wave, wave + 8, ...
+template <bool FIRST, bool LAST, int rin, int rout>
+__device__ __forceinline__ void fused_layer(const FusedArgs &a, int layer, const bf16x8 *wb, const uint16_t *in,
+                                            uint16_t *out, int i0, int j0, int wave, int lane) {
+    constexpr int NB = FIRST ? 3 : 18;
+    constexpr int IPS = FIRST ? 4 : TPS;   // input pixel stride (bf16)
+    const int m = lane & 31, h = lane >> 5, ch = lane & 31;
+    const float bias = a.par[(size_t)layer * LP + LW + ch];
+    constexpr int npx = rout * rout, ntile = (npx + 31) / 32;
+    for (int mt = wave; mt < ntile; mt += 8) {
+        int base[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            // A rows: pixel (mt + 4 u) * 32 + m of the output region (padding clamped to 0)
+            int p = (mt + 4 * u) * 32 + m;
+            p = p < npx ? p : 0;
+            const int pr = p / rout, pc = p - (p / rout) * rout;
+            base[u] = pr * rin + pc;
+        }
+        f32x16 acc[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[u][r] = 0.0f;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                bf16x8 av;
+                if constexpr (FIRST) {
+                    const bf16x4b z = {};
+                    const int t0 = 4 * b + 2 * h, t1 = t0 + 1;
+                    const bf16x4b lo = t0 < 9 ? *(const bf16x4b *)(in + (base[u] + (t0 / 3) * rin + t0 % 3) * IPS) : z;
+                    const bf16x4b hi = t1 < 9 ? *(const bf16x4b *)(in + (base[u] + (t1 / 3) * rin + t1 % 3) * IPS) : z;
+                    av = cat4(lo, hi);
+                } else {
+                    const int tap = b >> 1;
+                    av = *(const bf16x8 *)(in + (base[u] + (tap / 3) * rin + tap % 3) * IPS + (b & 1) * 16 + 8 * h);
+                }
+                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, wb[b], acc[u], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int q = (mt + 4 * u) * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+                if (q >= npx) continue;
+                const int qr = q / rout, qc = q - (q / rout) * rout;
+                const float v = acc[u][r] + bias;
+                if constexpr (LAST) {
+                    if (ch < 6) a.y[((size_t)(i0 + qr) * a.n + (j0 + qc)) * YS + ch] = v;
+                } else {
+                    out[(qr * rout + qc) * TPS + ch] = f2bf(fmaxf(v, 0.0f));
+                }
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) net_forward_fused_kernel(const FusedArgs a) {
+    __shared__ __attribute__((aligned(16))) uint16_t lds[FUSED_LDS];
+    uint16_t *F = lds;                           // features, 24 x 24 x 4
+    uint16_t *A1 = F + FR0 * FR0 * 4;            // layer 1 (22 x 22), then layer 3 (18 x 18)
+    uint16_t *A2 = A1 + FR1 * FR1 * TPS;         // layer 2 (20 x 20)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n = a.n, tpr = n / TT;
+    const int i0 = (blockIdx.x / tpr) * TT, j0 = (blockIdx.x - (blockIdx.x / tpr) * tpr) * TT;
+    // two register sets of weights: the next layer's load while a layer computes
+    bf16x8 wa[18], wb[18];
+    load_weights<true>(a, 0, lane, wa);
+    load_weights<false>(a, 1, lane, wb);
+    for (int q = tid; q < FR0 * FR0; q += 256) {
+        const int fi = q / FR0, fj = q - fi * FR0;
+        const int gi = wrap(i0 - 4 + fi, n), gj = wrap(j0 - 4 + fj, n);
+        *(uint2 *)(F + q * 4) = *(const uint2 *)(a.x + ((size_t)gi * n + gj) * C);
+    }
+    __syncthreads();
+    fused_layer<true, false, FR0, FR1>(a, 0, wa, F, A1, i0, j0, wave, lane);
+    load_weights<false>(a, 2, lane, wa);
+    __syncthreads();
+    fused_layer<false, false, FR1, FR2>(a, 1, wb, A1, A2, i0, j0, wave, lane);
+    load_weights<false>(a, 3, lane, wb);
+    __syncthreads();
+    fused_layer<false, false, FR2, FR3>(a, 2, wa, A2, A1, i0, j0, wave, lane);
+    __syncthreads();
+    fused_layer<false, true, FR3, TT>(a, 3, wb, A1, nullptr, i0, j0, wave, lane);
+}
+
 struct WgradArgs {
     const uint16_t *g;   // [P][32] output gradient
     const uint16_t *x;   // [P][32] layer input
@@ -583,11 +714,27 @@ int conv(cdc_closure *c, const uint16_t *x, int layer, bool backward, const uint
     return CDC_OK;
 }
 
-int forward(cdc_closure *c, hipStream_t s) {
+// the layer-by-layer forward pass (training: backprop reads the activations)
+int forward_layers(cdc_closure *c, hipStream_t s) {
     int rc;
     for (int l = 0; l < NL - 1; ++l)
         if ((rc = conv(c, c->d_act[l], l, false, nullptr, c->d_act[l + 1], nullptr, s))) return rc;
     return conv(c, c->d_act[NL - 1], NL - 1, false, nullptr, nullptr, c->d_y, s);
+}
+
+// the net's outputs only (inference): fused over 16 x 16 tiles where the
+// grid allows, layer by layer otherwise
+int forward(cdc_closure *c, hipStream_t s) {
+    if (c->n % TT != 0 || c->P != c->n * c->n) return forward_layers(c, s);
+    FusedArgs a{};
+    a.x = c->d_act[0];
+    a.w = c->d_wf;
+    a.par = c->d_par;
+    a.y = c->d_y;
+    a.n = c->n;
+    hipLaunchKernelGGL(net_forward_fused_kernel, dim3((unsigned)(c->P / (TT * TT))), dim3(256), 0, s, a);
+    HIP_TRY(hipGetLastError());
+    return CDC_OK;
 }
 
 int wgrad(cdc_closure *c, const uint16_t *g, const uint16_t *x, int layer, hipStream_t s) {
@@ -638,7 +785,7 @@ int backprop(cdc_closure *c, int k, int step, hipStream_t s) {
     HIP_TRY(hipMemcpyAsync(c->d_act[0], c->d_feat + (size_t)k * c->P * C, (size_t)c->P * C * sizeof(uint16_t),
                            hipMemcpyDeviceToDevice, s));
     int rc;
-    if ((rc = forward(c, s))) return rc;
+    if ((rc = forward_layers(c, s))) return rc;
     if ((rc = loss(c, c->d_tgt + (size_t)k * c->P * 4, step, s))) return rc;
     int cur = 0;
     for (int l = NL - 1; l >= 0; --l) {
