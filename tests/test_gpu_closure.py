@@ -56,8 +56,12 @@ def test_features_match_oracle():
     np.testing.assert_allclose(got, want, rtol=2.0 ** -8, atol=1e-30)
 
 
-def test_forward_matches_oracle():
-    mesh, hydro, ph, u0 = setup()
+@pytest.mark.parametrize("nb,bs", [(2, 16), (3, 16), (1, 64), (5, 8)],
+                         ids=["32-fused", "48-fused", "64-fused", "40-layers"])
+def test_forward_matches_oracle(nb, bs):
+    """Grids whose size is a multiple of 16 take the fused four-layer kernel
+    (2x2, 3x3 and 4x4 tiles of 16^2 here); others the layer-by-layer one."""
+    mesh, hydro, ph, u0 = setup(nb=nb, bs=bs)
     g = Solver(mesh, hydro, ph, u0)
     N = mesh.num_blocks * mesh.block_size
     net = Closure(N, mesh.domain_radius)
@@ -103,9 +107,12 @@ def make_sample(net, mesh, hydro, ph, u0, seed=9):
     return base, to_global(T, mesh.num_blocks, mesh.block_size), dt
 
 
+@pytest.mark.parametrize("nb,bs", [(2, 16), (5, 8)], ids=["32-tiled", "40-simple"])
 @pytest.mark.parametrize("mask", [0.0, 1e-3], ids=["all-cells", "masked"])
-def test_sample_gradient_matches_oracle(mask):
-    mesh, hydro, ph, u0 = setup()
+def test_sample_gradient_matches_oracle(mask, nb, bs):
+    """Grids whose size is a multiple of 16 run the tiled layer kernel in the
+    forward and input-gradient passes, others the simple one."""
+    mesh, hydro, ph, u0 = setup(nb=nb, bs=bs)
     N = mesh.num_blocks * mesh.block_size
     net = Closure(N, mesh.domain_radius, max_samples=2, mask_density=mask)
     base, T, dt = make_sample(net, mesh, hydro, ph, u0)
