@@ -476,3 +476,155 @@ def test_torque_is_minus_the_sum_of_cell_torques():
                     want[k] -= xc[i] * fy - yc[j] * fx
                     mag[k] += abs(xc[i] * fy) + abs(yc[j] * fx)
         np.testing.assert_allclose(tq[b], want, rtol=0, atol=1e-12 * mag.max())
+
+
+# ---- max signal speed, perturb's four outputs, RK weights: closed forms ----------
+
+def _uniform_oracle(kind, prim, nb=1, bs=8, R=4.0, **phys):
+    """An oracle whose whole mesh holds one primitive state."""
+    mesh, hydro, ph = H.small(kind, nb=nb, bs=bs, R=R, **phys)
+    nv = hydro.nv
+    d, vx, vy = prim[0], prim[1], prim[2]
+    u = [d, d * vx, d * vy]
+    if nv == 4:
+        u.append(prim[3] / (hydro.gamma_law_index - 1.0) + 0.5 * d * (vx * vx + vy * vy))
+    u0 = np.broadcast_to(np.array(u), (nb * nb, bs, bs, nv)).copy()
+    return H.oracle_for(mesh, hydro, ph, u0), hydro
+
+
+@pytest.mark.parametrize("v", [(0.0, 0.0), (0.7, 0.0), (0.0, -0.7), (0.5, -0.5), (-0.3, 1.1)])
+def test_euler_max_signal_speed_closed_form(v):
+    """Euler Primitive::max_signal_speed (src/physics.rs:679-681, hydro-euler):
+    the larger |v| component plus c_s = sqrt(gamma p / Sigma).  Dropping gamma,
+    using |v| instead of the larger component, or leaving out c_s all fail."""
+    d, p = 2.0, 0.3
+    o, hydro = _uniform_oracle("euler", (d, v[0], v[1], p))
+    gm = hydro.gamma_law_index
+    expect = max(abs(v[0]), abs(v[1])) + math.sqrt(gm * p / d)
+    assert o.max_signal_speed() == pytest.approx(expect, rel=1e-14)
+    assert abs(expect - (max(abs(v[0]), abs(v[1])) + math.sqrt(p / d))) > 1e-3      # gamma matters
+    if v[0] and v[1]:
+        assert abs(expect - (math.hypot(*v) + math.sqrt(gm * p / d))) > 1e-3       # not |v|
+
+
+@pytest.mark.parametrize("v", [(0.0, 0.0), (0.7, 0.0), (0.5, -0.9)])
+def test_iso_max_signal_speed_has_no_sound_speed(v):
+    """Isothermal max_signal_speed (src/physics.rs:521-526): max(|vx|, |vy|)
+    with no sound speed, as written in the reference."""
+    o, _ = _uniform_oracle("iso", (1.5, v[0], v[1]))
+    assert o.max_signal_speed() == pytest.approx(max(abs(v[0]), abs(v[1])), rel=1e-15)
+
+
+def _pert(st, el, dm=(0.0, 0.0), dp1=(0.0, 0.0), dp2=(0.0, 0.0)):
+    """oracle perturb with delta = (dm1, dm2, dpx1, dpx2, dpy1, dpy2)."""
+    return O.perturb(st, el, [dm[0], dm[1], dp1[0], dp2[0], dp1[1], dp2[1]])
+
+
+def _kick_to_relative(st, dv):
+    """Momentum kick on body 2 that changes the relative velocity by dv."""
+    return st[1, 0] * np.asarray(dv)
+
+
+def test_perturb_tangential_kick_of_circular_binary():
+    """A circular binary (r = a = 1, M = 1) whose relative speed becomes v' along
+    the orbit: r is an apsis, so e' = |r v'^2 / M - 1|, a' = 1 / (2/r - v'^2/M)
+    (vis-viva), and M, q are unchanged."""
+    p = phys_c(binary_eccentricity=0.0, binary_mass_ratio=0.5)
+    el = O.orbital_elements(p)
+    st = O.orbital_state(p, 0.9)
+    vrel = st[1, 3:5] - st[0, 3:5]
+    vc = np.hypot(*vrel)
+    for f in (1.05, 0.9, 1.2):
+        vp = f * vc
+        rc, de = _pert(st, el, dp2=_kick_to_relative(st, (f - 1.0) * vrel))
+        assert rc == 0
+        assert de[3] == pytest.approx(abs(vp * vp - 1.0), rel=1e-10)
+        assert de[0] == pytest.approx(1.0 / (2.0 - vp * vp) - 1.0, rel=1e-10)
+        assert abs(de[1]) < 1e-15 and abs(de[2]) < 1e-15
+
+
+def test_perturb_radial_kick_of_circular_binary():
+    """A radial velocity v_r added to a circular binary: h is unchanged and
+    e' = |v_r| / v_c exactly; a' = 1 / (2/r - (v_c^2 + v_r^2)/M)."""
+    p = phys_c(binary_eccentricity=0.0, binary_mass_ratio=1.0)
+    el = O.orbital_elements(p)
+    st = O.orbital_state(p, 2.1)
+    r = st[1, 1:3] - st[0, 1:3]
+    rhat = r / np.hypot(*r)
+    vc = np.hypot(*(st[1, 3:5] - st[0, 3:5]))
+    for vr in (0.01, -0.05, 0.2):
+        rc, de = _pert(st, el, dp2=_kick_to_relative(st, vr * rhat))
+        assert rc == 0
+        assert de[3] == pytest.approx(abs(vr) / vc, rel=1e-10)
+        assert de[0] == pytest.approx(1.0 / (2.0 - (vc * vc + vr * vr)) - 1.0, rel=1e-10)
+
+
+def test_perturb_pericentre_kick_of_eccentric_binary():
+    """An e = 0.5 binary at pericentre (t = 0, r_p = a (1 - e) = 0.5) whose
+    relative speed is raised to v' along the orbit: it stays the pericentre,
+    so e' = r_p v'^2 / M - 1; a mass dm added at the body's velocity raises M
+    and m1 only, so e' = r_p v_p^2 / (M + dm) - 1 and q' = m2 / (m1 + dm)."""
+    e, q = 0.5, 0.5
+    p = phys_c(binary_eccentricity=e, binary_mass_ratio=q)
+    el = O.orbital_elements(p)
+    st = O.orbital_state(p, 0.0)
+    rp = np.hypot(*(st[1, 1:3] - st[0, 1:3]))
+    assert rp == pytest.approx(1.0 - e, rel=1e-14)
+    vrel = st[1, 3:5] - st[0, 3:5]
+    vpn = np.hypot(*vrel)
+    assert rp * vpn ** 2 - 1.0 == pytest.approx(e, rel=1e-12)
+    for f in (1.02, 1.1):
+        rc, de = _pert(st, el, dp2=_kick_to_relative(st, (f - 1.0) * vrel))
+        assert rc == 0
+        assert de[3] + e == pytest.approx(rp * (f * vpn) ** 2 - 1.0, rel=1e-11)
+        assert de[0] + 1.0 == pytest.approx(1.0 / (2.0 / rp - (f * vpn) ** 2), rel=1e-11)
+    dm = 2e-3
+    m1, m2 = st[0, 0], st[1, 0]
+    rc, de = _pert(st, el, dm=(dm, 0.0), dp1=dm * st[0, 3:5])
+    assert rc == 0
+    assert de[1] == pytest.approx(dm, rel=1e-12)
+    assert de[2] + q == pytest.approx(m2 / (m1 + dm), rel=1e-13)
+    assert de[3] + e == pytest.approx(rp * vpn ** 2 / (1.0 + dm) - 1.0, rel=1e-11)
+    assert de[0] + 1.0 == pytest.approx(1.0 / (2.0 / rp - vpn ** 2 / (1.0 + dm)) * (1.0 + dm) / (1.0 + dm),
+                                        rel=1e-11)
+
+
+@pytest.mark.parametrize("rk,poly", [
+    ("RK1", lambda z: 1 - z),
+    ("RK2", lambda z: 1 - z + z * z / 2),
+    ("RK3", lambda z: 1 - z + z * z / 2 - z ** 3 / 6),
+])
+def test_rk_weights_reproduce_taylor_polynomials(rk, poly):
+    """SSP Runge-Kutta weights (godunov-core RungeKuttaOrder via
+    WeightedAverage, src/scheme.rs:346, src/state.rs:183-200).  Gas at rest
+    with uniform density, pressure and buffer rate (buffer_scale so large
+    that tanh((r - R)/scale) = 0), no sinks or fake mass, and a softening
+    length so large that the bodies' pull is below 1e-17: the state stays
+    uniform and at rest, and the density obeys dSigma/dt = -k (Sigma -
+    Sigma_0), so one step scales Sigma - Sigma_0 by the order-k Taylor
+    polynomial of exp(-z), z = k dt.  Any other stage weights (b = 1/2 for
+    RK2; 3/4, 1/3 for RK3) fail."""
+    mesh, hydro, ph = H.small("euler", nb=1, bs=8, R=4.0, rk_order=rk, sink_rate=0.0, fake_mass_rate=0.0,
+                              buffer_rate=40.0, buffer_scale=1e300, binary_mass_ratio=1.0,
+                              binary_eccentricity=0.0, viscosity={"constant_nu": 0.0},
+                              softening_length=1e6)
+    hydro.cooling_coefficient = 0.0
+    # a light disk, so that its pull on the binary stays far from unbinding it
+    d0, p0 = 1e-6, 1e-6
+    u_init = np.zeros((1, 8, 8, 4))
+    u_init[..., 0] = d0
+    u_init[..., 3] = p0 / (hydro.gamma_law_index - 1.0)
+    o = H.oracle_for(mesh, hydro, ph, u_init)
+    u = u_init.copy()
+    d = 3e-6
+    u[..., 0] = d
+    u[..., 3] = p0 / (hydro.gamma_law_index - 1.0)
+    o.set_state(u)
+    dt = o.advance(1)
+    k = 0.5 * ph.buffer_rate * math.sqrt(1.0 / mesh.domain_radius ** 3)
+    z = k * dt
+    assert 0.05 < z < 1.0
+    got = (o.get_state()["conserved"][..., 0] - d0) / (d - d0)
+    np.testing.assert_allclose(got, poly(z), rtol=1e-12)
+    others = [1 - z, 1 - z + z * z / 2, 1 - z + z * z / 2 - z ** 3 / 6]
+    assert sum(abs(poly(z) - w) > 1e-6 for w in others) == 2
