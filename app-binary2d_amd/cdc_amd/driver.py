@@ -111,7 +111,7 @@ class Runner:
         st = self.solver.get_state()
         ckpt.write(path, st, self.solver.mesh, self.solver.hydro, self.physics, model=self.model,
                    control=vars(self.control), tasks={k: v.to_dict() for k, v in self.tasks.items()},
-                   time_series=self.time_series)
+                   time_series=self.time_series, torque=self.solver.torque())
         self.checkpoints.append(path)
 
     # ---- the loop (src/main.rs:99-210) -----------------------------------

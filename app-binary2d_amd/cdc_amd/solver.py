@@ -123,7 +123,10 @@ class Solver:
         self._cp = physics.to_c()
         _check(self.lib, self.lib.cdc_solver_set_physics(self.handle, byref(self._cp)))
 
-    def set_state(self, conserved, time=0.0, iteration=(0, 1), ist=None, oec=None):
+    def set_state(self, conserved, time=0.0, iteration=(0, 1), ist=None, oec=None, torque=None):
+        """Upload a State; `torque` ([nblk][2] per-block torque impulses, as
+        torque() returns them) restores the running torque totals, else they
+        start from zero."""
         u = np.ascontiguousarray(conserved, dtype=np.float64).reshape(self.shape)
         i = None if ist is None else np.ascontiguousarray(ist, dtype=np.float64).reshape(self.nblk, 7, self.nv)
         e = None if oec is None else np.ascontiguousarray(oec, dtype=np.float64).reshape(self.nblk, 7, 4)
@@ -131,6 +134,13 @@ class Solver:
                                                       int(iteration[1]), _ptr(u),
                                                       None if i is None else _ptr(i),
                                                       None if e is None else _ptr(e)))
+        if torque is not None:
+            self.set_torque(torque)
+
+    def set_torque(self, torque):
+        """Restore the per-block torque impulses [nblk][2] of the current state."""
+        t = np.ascontiguousarray(torque, dtype=np.float64).reshape(self.nblk, 2)
+        _check(self.lib, self.lib.cdc_solver_set_torque(self.handle, _ptr(t)))
 
     def get_state(self):
         """Returns dict(time, iteration=(num, den), conserved, ist, oec)."""

@@ -1002,6 +1002,15 @@ int32_t cdc_solver_get_torque(cdc_solver_t *s, double *torque) {
     return CDC_OK;
 }
 
+int32_t cdc_solver_set_torque(cdc_solver_t *s, const double *torque) {
+    if (!s || !torque) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null pointer");
+    HIP_TRY(hipMemcpyAsync(s->d_tq[s->cur], torque, (size_t)s->nblk * 2 * sizeof(double), hipMemcpyHostToDevice,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->version += 1;
+    return CDC_OK;
+}
+
 int32_t cdc_solver_time_series_sample(cdc_solver_t *s, double *time, double *ist_total, double *oec_total,
                                       double *torque_total) {
     if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");

@@ -58,6 +58,43 @@ def valu_issue(prof, avg_launch_ms):
     return {"floor_ms": floor_ms, "frac": floor_ms / avg_launch_ms, "source": prof.get("pmc_source")}
 
 
+def host_cpus():
+    """(CPUs of the host, CPUs this process may run on, physical cores of the
+    host from /proc/cpuinfo or None)."""
+    total = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = total
+    phys = set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            pid = cid = None
+            for line in f:
+                if line.startswith("physical id"):
+                    pid = line.split(":", 1)[1].strip()
+                elif line.startswith("core id"):
+                    cid = line.split(":", 1)[1].strip()
+                elif not line.strip():
+                    if pid is not None and cid is not None:
+                        phys.add((pid, cid))
+                    pid = cid = None
+    except OSError:
+        pass
+    return total, allowed, (len(phys) or None)
+
+
+def default_cpu_threads():
+    """Every CPU this process may run on, capped by OMP_NUM_THREADS when the
+    environment sets it (a shared host's CPU share for this GPU)."""
+    _, allowed, _ = host_cpus()
+    try:
+        cap = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        cap = 0
+    return min(allowed, cap) if cap > 0 else allowed
+
+
 def cpu_baseline(w, u0, seconds, threads):
     from oracle import oracle as O
     O.set_threads(threads)
@@ -83,9 +120,14 @@ def cpu_baseline(w, u0, seconds, threads):
                     break
     except OSError:
         pass
-    return {"value": n * zones / el, "unit": "zone-updates/s", "cores": O.get_threads(), "kind": "port",
-            "sample": f"{n} RK steps of the same workload ({el:.1f} s), C oracle, OpenMP over blocks, {cpu}",
-            "mzps": n * zones / el * 1e-6}
+    total, allowed, phys = host_cpus()
+    used = O.get_threads()
+    return {"value": n * zones / el, "unit": "zone-updates/s", "cores": used, "kind": "port",
+            "sample": f"{n} RK steps of the same workload ({el:.1f} s), C oracle, OpenMP over blocks, "
+                      f"{used} threads on {cpu}",
+            "mzps": n * zones / el * 1e-6,
+            "host": {"cpu": cpu, "logical_cpus": total, "physical_cores": phys, "cpus_allowed": allowed,
+                     "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}}
 
 
 def run_ensemble(args):
@@ -200,7 +242,8 @@ def main():
                     help="steps of the second pass that times each stage launch with HIP events")
     ap.add_argument("--workload", default="euler2048")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="oracle threads (default: every CPU this process may run on, capped by OMP_NUM_THREADS)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tile", default=None, help="TXxTY stage-kernel tile (skips the autotuner)")
@@ -308,7 +351,7 @@ def main():
     }
     g.close()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(w, u0, args.cpu_seconds, args.cpu_threads)
+        line["cpu_baseline"] = cpu_baseline(w, u0, args.cpu_seconds, args.cpu_threads or default_cpu_threads())
     print(json.dumps(line), flush=True)
 
 

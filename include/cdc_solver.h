@@ -243,6 +243,12 @@ int32_t cdc_solver_resample_from(cdc_solver_t *dst, const cdc_solver_t *src);
  */
 int32_t cdc_solver_get_torque(cdc_solver_t *solver, double *torque);
 
+/* Restore per-block torque impulses ([block][2], as cdc_solver_get_torque
+ * returns them) into the current state, after cdc_solver_set_state, so that
+ * a run restarted from a checkpoint carries its running totals on.  No
+ * reference counterpart (the reference keeps no torque accumulator). */
+int32_t cdc_solver_set_torque(cdc_solver_t *solver, const double *torque);
+
 /*
  * State::time_series_sample (src/state.rs:135-146): the time and the sums
  * over blocks (in block order) of integrated_source_terms [7][nv] and

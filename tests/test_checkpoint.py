@@ -76,7 +76,93 @@ def test_checkpoint_tree_round_trip(tmp_path, hydro):
     assert blk["conserved"]["v"] == 1 and blk["conserved"]["dim"] == [8, 8]
     np.testing.assert_array_equal(blk["conserved"]["data"], st["conserved"][5].reshape(64, nv))
     assert list(blk["integrated_source_terms"]) == list(ckpt.ITEMS)
-    assert tree["config"]["hydro"] == {ckpt.hydro_tag(hydro): ckpt._plain(hydro)}
+    assert r["model"] == {"seeded_cavity": {"seed": 1}}
+
+
+# The reference's serde field names, restated here from the reference's
+# structs (not taken from the module under test): every one of these structs
+# except Control rejects unknown fields.
+REF_MESH = {"num_blocks", "block_size", "domain_radius"}                         # src/mesh.rs:19-37
+REF_ISO = {"mach_number", "density_floor"}                                       # src/physics.rs:255-265
+REF_EULER = {"gamma_law_index", "cooling_coefficient", "pressure_floor",         # src/physics.rs:270-303
+             "density_floor", "density_index", "cooling_slow_start"}
+REF_PHYSICS = {"buffer_rate", "buffer_scale", "binary_eccentricity",             # src/physics.rs:121-217
+               "binary_mass_ratio", "lambda_multiplier", "viscosity", "sink_model", "cfl", "plm",
+               "plm_density_trigger", "fake_mass_threshold", "fake_mass_rate", "one_body", "mach_ceiling",
+               "rk_order", "sink_radius", "sink_rate", "softening_length", "dt_min", "safe_mode_duration",
+               "safe_cfl", "safe_plm", "safe_mach_ceiling", "safe_rk_order", "lambda", "nu", "alpha"}
+REF_PHYSICS_REQUIRED = {"buffer_rate", "buffer_scale", "binary_eccentricity", "binary_mass_ratio", "cfl", "plm",
+                        "rk_order", "sink_radius", "sink_rate"}
+REF_CONTROL_REQUIRED = {"num_orbits", "checkpoint_interval", "fold", "output_directory"}   # src/app.rs:94-131
+REF_MODELS = {"infinite_disk": {"mach_number", "surface_density"},                # src/app.rs:41-48,
+              "infinite_alpha_disk": {"mach_number", "surface_density"},          # src/disks.rs:11-93
+              "finite_disk": {"disk_mass", "disk_width", "disk_radius", "softening_length", "mach_number"},
+              "residual_test": set()}
+
+
+def _check_reference_config(cfg, hydro_tag):
+    assert set(cfg) == {"hydro", "model", "mesh", "physics", "control"}            # src/app.rs:169-177
+    assert set(cfg["hydro"]) == {hydro_tag}
+    h = cfg["hydro"][hydro_tag]
+    assert set(h) <= (REF_ISO if hydro_tag == "isothermal" else REF_EULER)
+    assert {"gamma_law_index", "cooling_coefficient"} <= set(h) if hydro_tag == "euler" else "mach_number" in h
+    assert set(cfg["mesh"]) == REF_MESH
+    assert set(cfg["physics"]) <= REF_PHYSICS and REF_PHYSICS_REQUIRED <= set(cfg["physics"])
+    (vk, vv), = cfg["physics"]["viscosity"].items()
+    assert vk in ("constant_nu", "alpha") and isinstance(vv, float)
+    assert cfg["physics"]["sink_model"] in ("impartial", "torque_free")
+    assert cfg["physics"]["rk_order"] in ("RK1", "RK2", "RK3")
+    assert REF_CONTROL_REQUIRED <= set(cfg["control"])
+    (mk, mv), = cfg["model"].items()
+    assert mk in REF_MODELS and set(mv) <= REF_MODELS[mk]
+    # exactly one Mach number between hydro and model (InfiniteDiskModel::validate)
+    if mk.startswith("infinite"):
+        assert (mv.get("mach_number") is not None) != (h.get("mach_number") is not None)
+
+
+@pytest.mark.parametrize("name", ["iso256", "euler2048"])
+def test_checkpoint_config_has_only_reference_fields(tmp_path, name):
+    """The written config is readable by the reference's deny_unknown_fields
+    structs: every key is one of the reference's serde fields (so `lambda`,
+    not `lambda_`, and no `cooling_beta` in euler), `model` is an AnyModel
+    variant, and the extensions (beta cooling, the seeded cavity model,
+    per-block torques) sit under the App-level extension key and come back."""
+    w = workloads.get(name)
+    mesh, hydro, ph = Mesh(2, 8, 8.0), w["hydro"], w["physics"]
+    ph.lambda_ = 0.25
+    nv = hydro.nv
+    rng = np.random.default_rng(3)
+    st = {"time": 0.5, "iteration": (3, 1), "conserved": rng.normal(size=(4, 8, 8, nv)),
+          "ist": rng.normal(size=(4, 7, nv)), "oec": rng.normal(size=(4, 7, 4))}
+    torque = rng.normal(size=(4, 2))
+    model = {"seeded_cavity": {"seed": 3, "cavity_radius": 2.5}}
+    p = tmp_path / "chkpt.0000.cbor"
+    ckpt.write(str(p), st, mesh, hydro, ph, model=model, control={"num_orbits": 2.0, "fold": 10},
+               torque=torque)
+    tree = cbor.loads(p.read_bytes())
+    tag = ckpt.hydro_tag(hydro)
+    _check_reference_config(tree["config"], tag)
+    assert tree["config"]["physics"]["lambda"] == 0.25
+    assert set(tree) == {"state", "tasks", "time_series", "config", "version", "cdc_amd"}
+    r = ckpt.read(str(p))
+    assert r["model"] == model
+    np.testing.assert_array_equal(r["state"]["torque"], torque)
+    # the config reads back into this package's types with the extensions folded in
+    from cdc_amd.config import Configuration
+    cfg = Configuration.from_dict({**r["config"], "model": r["model"]})
+    assert cfg.physics.lambda_ == 0.25
+    assert getattr(cfg.hydro, "cooling_beta", None) == getattr(hydro, "cooling_beta", None)
+
+
+def test_checkpoint_keeps_a_reference_model():
+    mesh, hydro = Mesh(1, 4, 4.0), Isothermal(mach_number=10.0)
+    st = {"time": 0.0, "iteration": (0, 1), "conserved": np.ones((1, 4, 4, 3)), "ist": np.zeros((1, 7, 3)),
+          "oec": np.zeros((1, 7, 4))}
+    model = {"finite_disk": {"disk_mass": 0.1, "disk_width": 1.5, "disk_radius": 3.0, "softening_length": 0.05,
+                             "mach_number": None}}
+    tree = ckpt.pack(st, mesh, hydro, workloads.base_physics(), model=model)
+    _check_reference_config(tree["config"], "isothermal")
+    assert tree["config"]["model"] == model and "cdc_amd" not in tree
 
 
 def test_time_series_rates_and_checkpoint_round_trip(tmp_path):

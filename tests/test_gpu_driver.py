@@ -72,21 +72,34 @@ def test_checkpoints_and_restart_continue_bitwise(tmp_path):
     # restart from the middle checkpoint and finish
     r = ckpt.read(full.checkpoints[1])
     g = Solver(mesh, hydro, ph, u0)
-    st = r["state"]
-    g.set_state(st["conserved"], time=st["time"], iteration=st["iteration"], ist=st["ist"], oec=st["oec"])
+    ckpt.restore(g, r)
     tasks = new_tasks({k: v for k, v in r["tasks"].items()})
-    series = [{"time": s["time"], "ist": s["ist"], "oec": s["oec"]} for s in r["time_series"]]
+    series = list(r["time_series"])
+    assert all("torque" in s for s in series)
     ctl2 = Control(**{**vars(ctl), "output_directory": str(tmp_path / "b")})
     rest = Runner(g, ph, ctl2, tasks=tasks, time_series=series, log=logs.append).run()
     a, b = full.solver.get_state(), rest.solver.get_state()
     np.testing.assert_array_equal(a["conserved"], b["conserved"])
     np.testing.assert_array_equal(a["ist"], b["ist"])
     assert a["time"] == b["time"] and a["iteration"] == b["iteration"]
+    # the per-block torque totals carry on through the restart
+    np.testing.assert_array_equal(full.solver.torque(), rest.solver.torque())
     # as in the reference, the restarted loop's first side effects may take a
     # second sample at the checkpoint's time when the task's next_time lags
     # behind (RecurringTask::advance adds the interval to next_time)
     assert sorted({s["time"] for s in rest.time_series}) == [s["time"] for s in full.time_series]
     assert rest.tasks["write_checkpoint"].count == full.tasks["write_checkpoint"].count
+    # ... and the torque and accretion series (and the torque rates taken from
+    # them) equal the uninterrupted run's
+    from cdc_amd import timeseries as TS
+    uniq = {}
+    for s in rest.time_series:
+        uniq.setdefault(s["time"], s)
+    rs = TS.extract([uniq[t] for t in sorted(uniq)])
+    fs = TS.extract(full.time_series)
+    np.testing.assert_array_equal(rs["torque"], fs["torque"])
+    np.testing.assert_array_equal(rs["ist"], fs["ist"])
+    np.testing.assert_array_equal(TS.rates(rs)[2], TS.rates(fs)[2])
 
 
 def test_fallback_rewinds_and_proceeds_in_safe_mode(tmp_path):
