@@ -7,6 +7,10 @@
             floor 1e-10 (as the reference's cooling-cbd setup floors pressure), 100 steps
 iso2048   the isothermal physics at 32x32 blocks of 64^2 (2048^2): the fine
             run of the learned-closure study (coarse grid 512^2)
+euler256  the euler2048 gas and binary at 4x4 blocks of 64^2 (256^2) with
+            constant nu = 1e-3 in place of alpha viscosity: the gamma-law
+            configuration that runs a whole orbit (1063 RK3 steps); alpha
+            viscosity outruns its explicit limit in the hot cavity gas
 ensemble64 (ensemble_members) 64 isothermal RK2 runs of 256^2 to 1024^2 with
             drawn binary and disk parameters, half an orbit each
 Domain [-8a, 8a]^2, a = M = G = 1, softening = sink radius = 0.05, sink rate
@@ -50,6 +54,12 @@ WORKLOADS = {
                       physics=base_physics(binary_eccentricity=0.5, binary_mass_ratio=0.5, rk_order="RK3",
                                            viscosity={"alpha": 0.1}),
                       seed=3, steps=100, orbits=None),
+    "euler256": dict(mesh=Mesh(4, 64, 8.0),
+                     hydro=Euler(gamma_law_index=5.0 / 3.0, cooling_coefficient=0.0, cooling_beta=10.0,
+                                 pressure_floor=1e-10),
+                     physics=base_physics(binary_eccentricity=0.5, binary_mass_ratio=0.5, rk_order="RK3",
+                                          viscosity={"constant_nu": 1e-3}),
+                     seed=3, steps=None, orbits=1.0),
 }
 
 

@@ -134,45 +134,91 @@ def accretion_series(solver_state):
     return solver_state["ist"][:, 0, 0].sum(), solver_state["ist"][:, 1, 0].sum()
 
 
-def test_iso256_one_orbit():
-    """The isothermal 256^2 configuration stepped for one orbit in lockstep
-    with the oracle: per-step state <= 1e-12 for the first steps, <= 1e-9 at
-    the end, and the accretion and torque time series <= 1e-9."""
-    w = workloads.get("iso256")
+def seed_from_oracle(h, o):
+    """Put the oracle's whole current state (conserved, ist, oec, torque, time,
+    iteration) into the solver h."""
+    so = o.get_state()
+    h.set_state(so["conserved"], time=so["time"], iteration=so["iteration"], ist=so["ist"], oec=so["oec"],
+                torque=o.get_torque())
+
+
+def one_orbit(name, reseed_every=10):
+    """A workload stepped for one orbit in lockstep with the oracle.  The
+    per-step bar holds along the whole orbit: every reseed_every steps a
+    second solver takes the oracle's state and one step from it, within
+    1e-12 of the oracle's step (check_states); the free-running solver ends
+    within 1e-9 of the oracle, and its accretion and torque time series
+    (every step) within 1e-9.  Returns the step count and the worst
+    per-step error."""
+    w = workloads.get(name)
     u0 = workloads.initial_state(w)
     g, o = pair(w["mesh"], w["hydro"], w["physics"], u0)
+    h = Solver(w["mesh"], w["hydro"], w["physics"], u0)
     t_end = 2 * math.pi * w["orbits"]
-    step = 0
+    step, worst, checks = 0, 0.0, 0
     mg, mo, tg, to_ = [], [], [], []
-    while g.get_state()["time"] < t_end:
+    while g.time() < t_end:
+        reseed = step >= 10 and step % reseed_every == 0
+        if reseed:
+            seed_from_oracle(h, o)
         dg, do = g.advance(1), o.advance(1)
         assert dg == pytest.approx(do, rel=1e-9)
+        if reseed:
+            assert h.advance(1) == pytest.approx(do, rel=1e-12)
+            worst = max(worst, check_states(h, o))
+            checks += 1
         sg, so = g.get_state(), o.get_state()
         if step < 10:
-            check_states(g, o)
+            worst = max(worst, check_states(g, o))
+            checks += 1
         mg.append(accretion_series(sg)); mo.append(accretion_series(so))
-        tg.append(sg["ist"][:, 2:4, 1:3].sum(axis=0)); to_.append(so["ist"][:, 2:4, 1:3].sum(axis=0))
+        tg.append(g.time_series_sample()["torque"]); to_.append(o.get_torque().sum(axis=0))
         step += 1
     sg, so = g.get_state(), o.get_state()
-    assert H.field_err(sg["conserved"], so["conserved"]) <= 1e-9
+    end = H.field_err(sg["conserved"], so["conserved"])
+    assert end <= 1e-9, f"state error after one orbit {end:.3e}"
     mg, mo = np.array(mg), np.array(mo)
     assert np.max(np.abs(mg - mo)) <= 1e-9 * np.max(np.abs(mo))
     tg, to_ = np.array(tg), np.array(to_)
     assert np.max(np.abs(tg - to_)) <= 1e-9 * np.max(np.abs(to_))
+    assert checks >= step // reseed_every
+    print(f"{name}: {step} steps, {checks} one-step checks (worst {worst:.2e}), error after one orbit {end:.2e}")
+    return step, worst
+
+
+def test_iso256_one_orbit():
+    """The isothermal 256^2 configuration for one orbit (see one_orbit)."""
+    step, _ = one_orbit("iso256")
     assert step > 100
+
+
+def test_euler256_one_orbit():
+    """The gamma-law gas of the 2048^2 workload (e = 0.5, q = 0.5, beta
+    cooling, RK3) at 256^2 with constant nu, for one orbit (see one_orbit)."""
+    step, _ = one_orbit("euler256")
+    assert step > 500
 
 
 def lockstep(w, steps, check_every):
     """The workload stepped `steps` times in lockstep with the oracle from the
     same seeded state: dt within 1e-12 and the full state check
     (check_states) for the first 10 steps, then dt within 1e-9 and the
-    conserved state within 1e-9 every check_every steps and at the end.
+    conserved state within 1e-9 every check_every steps and at the end; at
+    those checks a second solver also takes the oracle's state and one step
+    from it, within 1e-12 of the oracle's (the per-step bar along the run).
     Returns both solvers and the largest state error seen."""
     u0 = workloads.initial_state(w)
     g, o = pair(w["mesh"], w["hydro"], w["physics"], u0)
+    h = Solver(w["mesh"], w["hydro"], w["physics"], u0)
     worst = 0.0
     for k in range(steps):
+        reseed = k >= 10 and (k + 1) % check_every == 0
+        if reseed:
+            seed_from_oracle(h, o)
         dg, do = g.advance(1), o.advance(1)
+        if reseed:
+            assert h.advance(1) == pytest.approx(do, rel=1e-12)
+            worst = max(worst, check_states(h, o))
         if k < 10:
             assert dg == pytest.approx(do, rel=1e-12)
             worst = max(worst, check_states(g, o))
@@ -182,6 +228,7 @@ def lockstep(w, steps, check_every):
                 e = H.field_err(g.get_state()["conserved"], o.get_state()["conserved"])
                 assert e <= 1e-9, f"state error {e:.3e} after {k + 1} steps"
                 worst = max(worst, e)
+    h.close()
     return g, o, worst
 
 
