@@ -157,6 +157,7 @@ SIGNATURES = {
     "cdc_solver_max_signal_speed": (c_int32, [c_void_p, POINTER(c_double)]),
     "cdc_solver_get_torque": (c_int32, [c_void_p, c_void_p]),
     "cdc_solver_set_torque": (c_int32, [c_void_p, c_void_p]),
+    "cdc_max_signal_speed_noalloc": (c_int32, [c_int32, c_double, c_void_p, c_int64, c_void_p, c_void_p]),
     "cdc_snapshot_create": (c_int32, [c_void_p, POINTER(c_void_p)]),
     "cdc_snapshot_destroy": (None, [c_void_p]),
     "cdc_snapshot_time": (c_int32, [c_void_p, POINTER(c_double)]),

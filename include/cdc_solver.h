@@ -275,6 +275,23 @@ int32_t cdc_solver_restore(cdc_solver_t *solver, const cdc_snapshot_t *snapshot)
 /* State::max_signal_speed (src/state.rs:121-133) of the current state. */
 int32_t cdc_solver_max_signal_speed(cdc_solver_t *solver, double *out);
 
+/*
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the variant of cdc_solver_max_signal_speed that
+ * allocates nothing.  `conserved` is a state in device memory, nv doubles per
+ * cell ([cell][v], the boundary layout; any cell order), nv = 3 for
+ * CDC_ISOTHERMAL, 4 for CDC_EULER; the max signal speed is written to the
+ * device double *out.  Work is enqueued on `stream` (a hipStream_t; NULL is
+ * the null stream) and the call returns without waiting.  Partial results go
+ * to a fixed device array of 1024 doubles (8 KiB) inside the library, one per
+ * GPU, kept until the process ends: concurrent calls on one GPU race.
+ * ncell must lie in [1, 2^40]; larger or null arguments are rejected.
+ */
+int32_t cdc_max_signal_speed_noalloc(int32_t kind, double gamma_law_index,
+                                     const double *conserved, int64_t ncell,
+                                     double *out, void *stream);
+
 /* Tile shape (tx rows along x by ty columns along y, inside one block) of
  * the fused stage kernel.  Supported: 16x32, 32x16, 16x16, 8x32, 32x8 (256
  * threads), 8x16, 16x8 (128 threads), 8x8 (64 threads), each dividing
