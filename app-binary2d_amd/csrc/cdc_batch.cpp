@@ -29,6 +29,7 @@ using cdc::host::StepPlan;
 struct cdc_ensemble {
     std::vector<cdc_solver *> members;
     int nv = 0, rk = 0;
+    int cool = cdc::kCoolAny;   // the members' shared cooling law (StageCooling)
     cdc::TileShape ts{};
     hipStream_t stream = nullptr;
     bool use_graph = true;
@@ -110,7 +111,7 @@ int enqueue_step(cdc_ensemble *e, int nrun, int total_tiles, int total_blocks) {
     for (int k = 0; k < e->rk; ++k) {
         const cdc::StageArgs *a = e->d_args + (size_t)k * nrun;
         HIP_TRY(cdc::launch_stage_batch(e->nv, e->ts, a, e->d_tile_begin, nrun, total_tiles, nullptr, nullptr, 0,
-                                        nullptr, nullptr, e->stream));
+                                        e->cool, k > 0 ? 1 : 0, nullptr, nullptr, e->stream));
         HIP_TRY(cdc::launch_finalize_batch(e->nv, a, e->d_block_begin, nrun, total_blocks, e->stream));
     }
     HIP_TRY(cdc::launch_batch_tail(e->nv, e->d_args + (size_t)(e->rk - 1) * nrun, nrun, e->d_amax, e->d_flags,
@@ -160,6 +161,12 @@ int32_t cdc_ensemble_create(cdc_solver_t *const *members, int32_t n, cdc_ensembl
     e->members.assign(members, members + n);
     e->nv = m0->nv;
     e->rk = m0->phys.rk_order;
+    if (e->nv == 4) {
+        // the ensemble's stages evaluate exp(-dt Omega / beta) themselves
+        bool same = true;
+        for (int i = 1; i < n; ++i) same = same && members[i]->hydro.cooling_model == m0->hydro.cooling_model;
+        if (same) e->cool = m0->hydro.cooling_model == CDC_COOLING_BETA ? cdc::kCoolBetaExp : cdc::kCoolT4;
+    }
     e->ts = ts;
     e->amax.assign(n, 0.0);
     e->amax_of.assign(n, UINT64_MAX);

@@ -279,7 +279,9 @@ __device__ __forceinline__ void face_flux(const StageArgs &A, int axis, const do
 // the primitive recovery of u without floors; `brate` the static buffer
 // rate; `cool` the beta-cooling frequency Omega, or, when A.cool_factor, the
 // relaxation factor exp(-dt Omega / beta) itself.  it[item][v].
-template <int NV>
+// `cool`: the cooling law when it is fixed at compile time (StageCooling);
+// kCoolAny selects it at run time from the stage arguments.
+template <int NV, int COOL = kCoolAny>
 __device__ __forceinline__ void cell_sources(const StageArgs &A, const double *u, const double *u0, double x,
                                              double y, double vx, double vy, double id, double p0,
                                              double brate, double cool, double it[7][NV]) {
@@ -346,16 +348,25 @@ __device__ __forceinline__ void cell_sources(const StageArgs &A, const double *u
         it[CDC_ITEM_GRAV2][3] = (fx2 * vx + fy2 * vy) * dt;
         const double e0 = p0 * (id * k.inv_gm1);
         double ec;
-        if (k.cooling_model == CDC_COOLING_BETA) {
-            const double ib0 = frcp(u0[0]);
-            const double bvx = u0[1] * ib0, bvy = u0[2] * ib0;
-            const double bp = (u0[3] - 0.5 * u0[0] * (bvx * bvx + bvy * bvy)) * k.gm1;
-            const double et = bp * (ib0 * k.inv_gm1);
-            ec = et + (e0 - et) * (A.cool_factor ? cool : exp(-dt * cool * k.cooling_inv_beta));
+        const bool beta = COOL == kCoolAny ? k.cooling_model == CDC_COOLING_BETA : COOL != kCoolT4;
+        if (beta) {
+            if constexpr (COOL != kCoolT4) {
+                const double ib0 = frcp(u0[0]);
+                const double bvx = u0[1] * ib0, bvy = u0[2] * ib0;
+                const double bp = (u0[3] - 0.5 * u0[0] * (bvx * bvx + bvy * bvy)) * k.gm1;
+                const double et = bp * (ib0 * k.inv_gm1);
+                double f;
+                if constexpr (COOL == kCoolBetaFactor) f = cool;
+                else if constexpr (COOL == kCoolBetaExp) f = exp(-dt * cool * k.cooling_inv_beta);
+                else f = A.cool_factor ? cool : exp(-dt * cool * k.cooling_inv_beta);
+                ec = et + (e0 - et) * f;
+            }
         } else {
-            const double sdi = k.density_index == 1 ? sd : (k.density_index == 2 ? sd * sd : pow(sd, (double)k.density_index));
-            const double a = k.cooling_coefficient / sdi * k.cooling_f;
-            ec = e0 * pow(1.0 + 3.0 * a * (e0 * e0 * e0) * dt, -1.0 / 3.0);
+            if constexpr (COOL == kCoolAny || COOL == kCoolT4) {
+                const double sdi = k.density_index == 1 ? sd : (k.density_index == 2 ? sd * sd : pow(sd, (double)k.density_index));
+                const double a = k.cooling_coefficient / sdi * k.cooling_f;
+                ec = e0 * pow(1.0 + 3.0 * a * (e0 * e0 * e0) * dt, -1.0 / 3.0);
+            }
         }
         if (k.has_mach_ceiling) ec = fmax(ec, 0.5 * (vx * vx + vy * vy) * k.mach_ceiling_coef);
         // uc = to_conserved(Sigma, vx, vy, Sigma ec (gamma - 1)); cooling = uc - u
@@ -370,5 +381,153 @@ __device__ __forceinline__ void cell_sources(const StageArgs &A, const double *u
 }
 
 }  // namespace dev
+
+// ---------------------------------------------------------------------------
+// Launch geometry and the per-block finalize shared by the stage kernels
+
+// Tile index of workgroup wg: consecutive tiles go to one XCD (workgroups
+// are dealt round-robin over the 8 XCDs), so halo re-reads of neighbouring
+// tiles hit the same L2.
+__device__ __forceinline__ int xcd_tile(int wg, int ntiles) {
+    return (ntiles % 8 == 0) ? (wg % 8) * (ntiles / 8) + wg / 8 : wg;
+}
+
+// Run index of global tile (or block) t: the last r with begin[r] <= t.
+__device__ __forceinline__ int run_of(const int32_t *__restrict__ begin, int nruns, int t) {
+    int lo = 0, hi = nruns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begin[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// A stage that already failed (err_flag holds an earlier stage id) skips the
+// remaining stages of the call.
+__device__ __forceinline__ bool stage_skipped(const StageArgs &A) {
+    const int ef = *(volatile const int32_t *)A.err_flag;
+    return ef != 0 && ef < A.stage_id;
+}
+
+// The per-block tail of a stage, one wave per block: the block's tile sums
+// in tile order, scaled by dx*dy, the four orbital-element perturbations
+// (ItemizedChange::perturbation, src/state.rs:258-282) one per lane, and the
+// RK combine of integrated_source_terms / orbital_elements_change
+// (src/scheme.rs:307-315, src/state.rs:195-197).
+// A wave that only waits on memory: every load that does not depend on the
+// tile sums (the error flag, the block's current and base ist / oec / torque
+// values) is issued with the tile-sum loads, so the block costs two memory
+// round trips (loads, then stores) around the perturbation arithmetic.
+// One wave per block, lane = lane index; every wave of the workgroup calls
+// this (an inactive one, past the last block, only takes the two barriers).
+// Scratch in LDS: s_ds[7 nv], s_de[7 * 4], *s_perr, per wave.
+constexpr int kFinLds = 7 * 4 + 7 * 4 + 1;   // doubles of per-wave scratch (nv <= 4)
+
+template <int NV>
+__device__ __forceinline__ void finalize_block(const StageArgs &A, const int b, const int tpb, const int lane,
+                                               bool active, double *s_ds, double *s_de, int *s_perr) {
+    constexpr int NS = SumMap<NV>::n, NR = NS + 1;
+    static_assert(NS + 1 + 7 * NV <= 64, "one wave per block");
+    const int ef = active ? *(volatile const int32_t *)A.err_flag : 0;
+    const size_t oi = (size_t)b * 7 * NV, oe = (size_t)b * 7 * 4;
+    double ist_c = 0.0, ist_b = 0.0, oec_c = 0.0, oec_b = 0.0, tq_c = 0.0, tq_b = 0.0;
+    double bmax = 0.0;   // lane NS: the block's max signal speed (when A.amax_out)
+    if (active) {
+        if (lane < 7 * 4) {
+            if (lane < 7 * NV) {
+                ist_c = A.ist_c[oi + lane];
+                if (A.read_base) ist_b = A.ist_b[oi + lane];
+            }
+            oec_c = A.oec_c[oe + lane];
+            if (A.read_base) oec_b = A.oec_b[oe + lane];
+        } else if (lane >= 32 && lane < 34) {
+            tq_c = A.tq_c[(size_t)b * 2 + (lane - 32)];
+            if (A.read_base) tq_b = A.tq_b[(size_t)b * 2 + (lane - 32)];
+        }
+        if (lane < NS || (lane == NS && A.amax_out)) {
+            double t = 0.0;
+            // tiles in order; unrolled so that the loads are in flight together
+            const auto *ts = dev::gptr(A.tile_sums) + (size_t)b * tpb * NR + lane;
+            int j = 0;
+            const bool mx = lane == NS;   // the max-speed column: maxima, not sums
+            for (; j + 8 <= tpb; j += 8) {
+                double v[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) v[q] = ts[(size_t)(j + q) * NR];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) t = mx ? fmax(t, v[q]) : t + v[q];
+            }
+            for (; j < tpb; ++j) t = mx ? fmax(t, ts[(size_t)j * NR]) : t + ts[(size_t)j * NR];
+            if (!mx) s_ds[SumMap<NV>::item(lane) * NV + SumMap<NV>::comp(lane)] = t * A.area;
+            else bmax = t;
+        } else if (lane > NS && lane < NS + 1 + 7 * NV) {
+            // structurally zero components
+            const int z = lane - NS - 1;
+            bool nonzero = false;
+            for (int s = 0; s < NS; ++s) nonzero |= (SumMap<NV>::item(s) * NV + SumMap<NV>::comp(s)) == z;
+            if (!nonzero) s_ds[z] = 0.0;
+        }
+    }
+    if (lane == 0) *s_perr = 0;
+    // a run whose earlier stage already failed (err_flag holds an earlier
+    // stage id) writes nothing; the wave stays for the barriers (the waves
+    // of one workgroup may belong to different runs)
+    if (ef != 0 && ef < A.stage_id) active = false;
+    __syncthreads();
+    if (active) {
+        if (lane < 4) {
+            const int item = lane == 0 ? CDC_ITEM_SINK1 : lane == 1 ? CDC_ITEM_SINK2 : lane == 2 ? CDC_ITEM_GRAV1 : CDC_ITEM_GRAV2;
+            const double *d = s_ds + item * NV;
+            double p[4];
+            const bool ok = (lane % 2 == 0) ? perturb(A.orb, -d[0], 0.0, -d[1], 0.0, -d[2], 0.0, p)
+                                            : perturb(A.orb, 0.0, -d[0], 0.0, -d[1], 0.0, -d[2], p);
+            for (int c = 0; c < 4; ++c) s_de[item * 4 + c] = ok ? p[c] - A.el[c] : 0.0;
+            if (!ok) atomicOr(s_perr, 1);
+        } else if (lane < 4 + 3 * 4) {
+            const int z = lane - 4;   // buffer, cooling, fake_mass items: no orbital change
+            s_de[(CDC_ITEM_BUFFER + z / 4) * 4 + (z % 4)] = 0.0;
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    if (lane == NS && bmax > 0.0) {
+        // after the last barrier, so that no wait covers this atomic.  Speeds
+        // are >= 0, whose bit patterns order like their values: an integer
+        // maximum, exact and independent of the blocks' order
+        atomicMax((unsigned long long *)A.amax_out, (unsigned long long)__double_as_longlong(bmax));
+    }
+    if (*s_perr) {
+        if (lane == 0) {
+            A.perr[b] = CDC_ERR_UNBOUND_ORBIT;
+            atomicCAS(A.err_flag, 0, A.stage_id);
+        }
+        return;
+    }
+    if (lane == 0) A.perr[b] = 0;
+    if (lane >= 32 && lane < 34) {
+        // torque impulse on body k about the origin: x_k x (-dp_k), dp_k the
+        // gas momentum change from body k's gravity (no reference counterpart)
+        const int k = lane - 32;
+        const double *g = s_ds + (CDC_ITEM_GRAV1 + k) * NV;
+        const double x = k == 0 ? A.orb.x1 : A.orb.x2, y = k == 0 ? A.orb.y1 : A.orb.y2;
+        double tau;
+        {
+CDC_NO_CONTRACT
+            tau = y * g[1] - x * g[2];
+        }
+        const size_t ti = (size_t)b * 2 + k;
+        const double s1 = tq_c + tau;
+        A.tq_o[ti] = A.read_base ? s1 * A.c1 + tq_b * A.c0 : s1;
+    }
+    if (lane < 7 * 4) {
+        if (lane < 7 * NV) {
+            const double s1 = ist_c + s_ds[lane];
+            A.ist_o[oi + lane] = A.read_base ? s1 * A.c1 + ist_b * A.c0 : s1;
+        }
+        const double s1 = oec_c + s_de[lane];
+        A.oec_o[oe + lane] = A.read_base ? s1 * A.c1 + oec_b * A.c0 : s1;
+    }
+}
 
 }  // namespace cdc

@@ -1,0 +1,556 @@
+// cdc_march.hip -- the column-marching form of the fused stage kernel.
+//
+// One RK stage (the P, F and U phases of try_advance_rk, src/scheme.rs:88-173)
+// over a strip of 64 columns x H rows inside one reference block (block_size
+// a multiple of 64), one wave per strip.  Lane l owns column cj0 + l and
+// marches down the strip's rows (x): the primitives of the last three rows,
+// the left state and stress factors of the previous row's x-face, that face's
+// flux and the previous row's y-flux difference stay in registers, so every
+// cell is converted to primitives once, its PLM gradients and stress factors
+// are evaluated once, and every x- and y-face flux is evaluated once (the
+// tiled kernel re-evaluates halo cells and recomputes both cells' stress at
+// every face).  Neighbouring columns come from the adjacent lanes through DPP
+// wavefront shifts.  The strip's y-halo (two columns on each side) is
+// handled once per strip, before the march: lane q evaluates the y-face at
+// one side of the strip for one row from the 8 cells around it and leaves the
+// face flux and the halo column's primitives in LDS.  The next row's
+// conserved state and the update's single-use operands are loaded one row
+// ahead, so their latency overlaps the current row's arithmetic; no barrier
+// is taken inside the march.
+//
+// The arithmetic is the reference's (src/scheme.rs:201-316, src/physics.rs:
+// 322-340, 584-607, 776-803) with one regrouping: a face's viscous flux is
+// 0.5 f (A_L + A_R), A the cell's stress component times its density and the
+// cell-local part of nu (constant nu, or alpha times the cell's c_s^2 for
+// gamma-law gas), f the face-local factor (alpha's 1/sqrt(m1/q1^3 + m2/q2^3),
+// times the face's c_s^2 for isothermal gas), evaluated once per face.  This
+// differs from the reference's (sigma_L + sigma_R) / 2 only by rounding.
+#include <hip/hip_ext.h>
+#include <hip/hip_runtime.h>
+#include <limits.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "cdc_device.h"
+#include "cdc_internal.h"
+
+namespace cdc {
+
+namespace {
+
+// DPP wavefront shifts of a double (two 32-bit moves); a lane whose source
+// lane lies outside the wave keeps `old`.
+constexpr int kFromNext = 0x130;   // wave_shl:1, lane l receives lane l + 1's value
+constexpr int kFromPrev = 0x138;   // wave_shr:1, lane l receives lane l - 1's value
+
+template <int CTRL>
+__device__ __forceinline__ double dpp(double old, double src) {
+    const long long o = __double_as_longlong(old), v = __double_as_longlong(src);
+    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)v, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Stress factors of one cell: Ax (normal-normal component on an x-face), Ay
+// (on a y-face) and B (off-diagonal), each times Sigma and the cell-local
+// part of nu (src/physics.rs:322-340, src/scheme.rs:230-247).
+template <int NV>
+__device__ __forceinline__ void stress_factors(const StageConsts &k, bool alpha, const double *p, double id,
+                                               const double *gx, const double *gy, double &ax, double &ay,
+                                               double &b) {
+    double K = p[0] * k.visc;
+    if constexpr (NV == 4) {
+        if (alpha) K = K * (k.gm * p[3] * id);   // c_s^2 = gamma p / Sigma
+    }
+    const double gxvx = gx[1] * k.idx, gxvy = gx[2] * k.idx;
+    const double gyvx = gy[1] * k.idy, gyvy = gy[2] * k.idy;
+    const double div = gxvx + gyvy, off = gxvy + gyvx;
+    ax = K * ((4.0 / 3.0) * gxvx - (2.0 / 3.0) * gyvy + k.lam * div);
+    ay = K * ((4.0 / 3.0) * gyvy - (2.0 / 3.0) * gxvx + k.lam * div);
+    b = K * off;
+}
+
+// One face (AXIS 0: x-face, 1: y-face) from its reconstructed states pl, pr:
+// HLLE (Toro ch. 10) plus the viscous flux 0.5 f (sa, sb); (vbx, vby) the
+// mean of the two cell-centre velocities (gamma-law energy flux).
+template <int NV, int AXIS>
+__device__ __forceinline__ void march_face(const StageArgs &A, bool visc_on, bool alpha, const double *pl,
+                                           const double *pr, double vbx, double vby, double sa, double sb,
+                                           double fxp, double fyp, double *F) {
+    const StageConsts &k = A.k;
+    const Orbit &o = A.orb;
+    const double d1x = fxp - o.x1, d1y = fyp - o.y1;
+    const double d2x = fxp - o.x2, d2y = fyp - o.y2;
+    const double r1s = d1x * d1x + d1y * d1y;
+    const double r2s = d2x * d2x + d2y * d2y;
+    double cs2f = 0.0;
+    if constexpr (NV == 3) cs2f = (o.m1 * dev::frsq(r1s + k.rs2) + o.m2 * dev::frsq(r2s + k.rs2)) * k.inv_mach2;
+    double tn = 0.0, to = 0.0;
+    if (visc_on) {
+        double f = 0.5;
+        if (alpha) {
+            // 1 / sqrt(m1/q1^3 + m2/q2^3), q = r + 1e-12 dx
+            const double q1 = dev::fsqrt(r1s) + k.eps_dx;
+            const double q2 = dev::fsqrt(r2s) + k.eps_dx;
+            const double c1 = q1 * q1 * q1, c2 = q2 * q2 * q2;
+            const double itf = dev::frsq((o.m1 * c2 + o.m2 * c1) * dev::frcp(c1 * c2));
+            f = NV == 3 ? 0.5 * (cs2f * itf) : 0.5 * itf;
+        }
+        tn = f * sa;
+        to = f * sb;
+    }
+    constexpr int NM = 1 + AXIS;   // the normal momentum component
+    const double vnl = pl[NM], vnr = pr[NM];
+    double ul[NV], ur[NV], fl[NV], fr[NV], csl, csr, ppl, ppr;
+    ul[0] = pl[0]; ul[1] = pl[0] * pl[1]; ul[2] = pl[0] * pl[2];
+    ur[0] = pr[0]; ur[1] = pr[0] * pr[1]; ur[2] = pr[0] * pr[2];
+    if constexpr (NV == 3) {
+        csl = csr = dev::fsqrt(cs2f);
+        ppl = pl[0] * cs2f;
+        ppr = pr[0] * cs2f;
+    } else {
+        ul[3] = 0.5 * pl[0] * (pl[1] * pl[1] + pl[2] * pl[2]) + pl[3] * k.inv_gm1;
+        ur[3] = 0.5 * pr[0] * (pr[1] * pr[1] + pr[2] * pr[2]) + pr[3] * k.inv_gm1;
+        const double gpl = k.gm * pl[3], gpr = k.gm * pr[3];
+        csl = gpl * dev::frsq(gpl * pl[0]);   // sqrt(gamma p / Sigma)
+        csr = gpr * dev::frsq(gpr * pr[0]);
+        ppl = pl[3];
+        ppr = pr[3];
+        fl[3] = (ul[3] + ppl) * vnl;
+        fr[3] = (ur[3] + ppr) * vnr;
+    }
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        fl[v] = ul[v] * vnl;
+        fr[v] = ur[v] * vnr;
+    }
+    fl[NM] += ppl;
+    fr[NM] += ppr;
+    const double ap = fmax(fmax(vnl + csl, vnr + csr), 0.0);
+    const double am = fmin(fmin(vnl - csl, vnr - csr), 0.0);
+    const double inv = dev::frcp(ap - am);
+    const double apam = ap * am;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) F[v] = (fl[v] * ap - fr[v] * am - (ul[v] - ur[v]) * apam) * inv;
+    const double tau_x = AXIS ? to : tn, tau_y = AXIS ? tn : to;
+    F[1] -= tau_x;
+    F[2] -= tau_y;
+    if constexpr (NV == 4) F[3] -= tau_x * vbx + tau_y * vby;
+}
+
+template <int NV>
+__device__ __forceinline__ void load_cell(dev::gcdouble *u, size_t NN, size_t g, double *out) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) out[v] = u[v * NN + g];
+}
+
+// PLM theta of the three cells a, b, c (plm_density_trigger)
+__device__ __forceinline__ double theta3(const StageConsts &k, double da, double db, double dc) {
+    if (k.has_pdt && (da < k.pdt || db < k.pdt || dc < k.pdt)) return 0.0;
+    return k.plm;
+}
+
+template <int NV, int H>
+struct MarchLds {
+    static constexpr int HP = 0;                          // [2][H + 2][NV] halo-column primitives
+    static constexpr int FB = 2 * (H + 2) * NV;           // [2][H][NV] strip-edge y-face fluxes
+    static constexpr int ACC = FB + 2 * H * NV;           // [NS][64] per-lane sums of the itemized sources
+    static constexpr int END = ACC + SumMap<NV>::n * 64;
+    static constexpr int size = END > kFinLds ? END : kFinLds;
+};
+
+// One strip of one RK stage (see the file comment), tile = strip index in
+// the run: strips are numbered block by block, row segments of H rows, then
+// 64-column strips within a segment.
+template <int NV, int H, int COOL, int RB>
+__device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, double *lds, int &s_key) {
+    constexpr int NS = SumMap<NV>::n, NR = NS + 1;
+    using ML = MarchLds<NV, H>;
+    const int lane = threadIdx.x;
+    const int bs = A.bs, N = A.N;
+    const int spb = bs / 64;                          // strips across a block
+    const int tpb = spb * (bs / H);
+    const int b = tile / tpb, lt = tile - b * tpb;
+    const int bi = b / A.nb, bj = b - (b / A.nb) * A.nb;
+    const int ri0 = (lt / spb) * H, cj0 = (lt - (lt / spb) * spb) * 64;
+    const size_t NN = (size_t)N * N;
+    dev::gcdouble *guc = dev::gptr(A.uc), *gu0 = dev::gptr(A.u0), *gub = dev::gptr(A.ub), *gaux = dev::gptr(A.aux);
+    dev::gdouble *guo = dev::gptr(A.uo);
+    const StageConsts &k = A.k;
+    const dev::Axis axx = dev::block_axis(A.R, A.L, bi, bs);
+    const dev::Axis axy = dev::block_axis(A.R, A.L, bj, bs);
+    const bool alpha = k.viscosity_model == CDC_VISC_ALPHA;
+    const bool visc_on = alpha || k.visc != 0.0;
+    const bool beta = k.cooling_model == CDC_COOLING_BETA;
+    const int gi0 = bi * bs, gj0 = bj * bs;
+    // wrapped global row / column of a block-local index
+    auto grow = [&](int i) { const int g = gi0 + i; return g < 0 ? g + N : (g >= N ? g - N : g); };
+    auto gcol = [&](int c) { const int g = gj0 + c; return g < 0 ? g + N : (g >= N ? g - N : g); };
+    double *hP = lds + ML::HP;
+    double *Fb = lds + ML::FB;
+    double *Acc = lds + ML::ACC;
+
+    int key = INT_MAX, kcode = 0;
+    double kval = 0.0;
+    if (lane == 0) s_key = INT_MAX;
+
+    // ---- the strip's edges: y-faces at cj0 - 1/2 (side 0) and cj0 + 64 - 1/2
+    // (side 1) for rows ri0 .. ri0 + H - 1, and the halo columns' primitives
+    // for rows ri0 - 1 .. ri0 + H --------------------------------------------
+    for (int q = lane; q < 2 * H; q += 64) {
+        const int s = q / H, r = ri0 + (q - s * H);
+        const int cL = s == 0 ? cj0 - 1 : cj0 + 63, cR = cL + 1;
+        const size_t rm = (size_t)grow(r - 1) * N, r0 = (size_t)grow(r) * N, rp = (size_t)grow(r + 1) * N;
+        const int cLL = gcol(cL - 1), cLg = gcol(cL), cRg = gcol(cR), cRR = gcol(cR + 1);
+        // cells: 0 (r, cL-1), 1 (r, cL), 2 (r, cR), 3 (r, cR+1), 4 (r-1, cL), 5 (r+1, cL), 6 (r-1, cR), 7 (r+1, cR)
+        const size_t g8[8] = {r0 + cLL, r0 + cLg, r0 + cRg, r0 + cRR, rm + cLg, rp + cLg, rm + cRg, rp + cRg};
+        double u[8][NV];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) load_cell<NV>(guc, NN, g8[c], u[c]);
+        double p[8][NV], id[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            double val;
+            (void)dev::try_to_primitive<NV>(k, u[c], p[c], val, id[c]);
+        }
+        double gxL[NV], gyL[NV], gxR[NV], gyR[NV];
+        const double txL = theta3(k, p[4][0], p[1][0], p[5][0]), tyL = theta3(k, p[0][0], p[1][0], p[2][0]);
+        const double txR = theta3(k, p[6][0], p[2][0], p[7][0]), tyR = theta3(k, p[1][0], p[2][0], p[3][0]);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            gxL[v] = dev::plm(txL, p[4][v], p[1][v], p[5][v]);
+            gyL[v] = dev::plm(tyL, p[0][v], p[1][v], p[2][v]);
+            gxR[v] = dev::plm(txR, p[6][v], p[2][v], p[7][v]);
+            gyR[v] = dev::plm(tyR, p[1][v], p[2][v], p[3][v]);
+        }
+        double axL = 0.0, ayL = 0.0, bL = 0.0, axR = 0.0, ayR = 0.0, bR = 0.0;
+        if (visc_on) {
+            stress_factors<NV>(k, alpha, p[1], id[1], gxL, gyL, axL, ayL, bL);
+            stress_factors<NV>(k, alpha, p[2], id[2], gxR, gyR, axR, ayR, bR);
+        }
+        double pl[NV], pr[NV], F[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            pl[v] = p[1][v] + gyL[v] * 0.5;
+            pr[v] = p[2][v] - gyR[v] * 0.5;
+        }
+        march_face<NV, 1>(A, visc_on, alpha, pl, pr, 0.5 * (p[1][1] + p[2][1]), 0.5 * (p[1][2] + p[2][2]),
+                          ayL + ayR, bL + bR, axx.c(r), axy.v(cR), F);
+        const int hr = r - ri0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            Fb[(s * H + hr) * NV + v] = F[v];
+            hP[(s * (H + 2) + hr + 1) * NV + v] = s == 0 ? p[1][v] : p[2][v];
+            if (hr == 0) hP[(s * (H + 2)) * NV + v] = s == 0 ? p[4][v] : p[6][v];
+            if (hr == H - 1) hP[(s * (H + 2) + H + 1) * NV + v] = s == 0 ? p[5][v] : p[7][v];
+        }
+        (void)axL; (void)axR;
+    }
+
+    // ---- prologue: rows ri0 - 2 .. ri0 + 1 -----------------------------------
+    const int j = cj0 + lane;                 // block-local column
+    const int gj = gj0 + j;                   // global column (inside the block)
+    const double yc = axy.c(j), yv1 = axy.v(j + 1);
+    double Um2[NV], Um1[NV], U0r[NV], Uz[NV];
+    load_cell<NV>(guc, NN, (size_t)grow(ri0 - 2) * N + gj, Um2);
+    load_cell<NV>(guc, NN, (size_t)grow(ri0 - 1) * N + gj, Um1);
+    load_cell<NV>(guc, NN, (size_t)grow(ri0) * N + gj, U0r);
+    load_cell<NV>(guc, NN, (size_t)grow(ri0 + 1) * N + gj, Uz);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) Acc[s * 64 + lane] = 0.0;
+    __syncthreads();   // the edge data in LDS
+
+    double Pa[NV], Pb[NV], idb;
+    double XLa[NV], axa = 0.0, ba = 0.0;
+    {
+        double Pm2[NV], idm2, ida, val;
+        (void)dev::try_to_primitive<NV>(k, Um2, Pm2, val, idm2);
+        (void)dev::try_to_primitive<NV>(k, Um1, Pa, val, ida);
+        const int code = dev::try_to_primitive<NV>(k, U0r, Pb, val, idb);
+        if (code) { key = ri0 * bs + j; kcode = code; kval = val; }
+        // row ri0 - 1: gradients, left state of its x+ face, stress factors
+        double gx[NV], gy[NV], PL[NV], PR[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            PL[v] = dpp<kFromPrev>(hP[v], Pa[v]);
+            PR[v] = dpp<kFromNext>(hP[(H + 2) * NV + v], Pa[v]);
+        }
+        const double tx = theta3(k, Pm2[0], Pa[0], Pb[0]), ty = theta3(k, PL[0], Pa[0], PR[0]);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            gx[v] = dev::plm(tx, Pm2[v], Pa[v], Pb[v]);
+            gy[v] = dev::plm(ty, PL[v], Pa[v], PR[v]);
+            XLa[v] = Pa[v] + gx[v] * 0.5;
+        }
+        double ay_unused;
+        if (visc_on) stress_factors<NV>(k, alpha, Pa, ida, gx, gy, axa, ay_unused, ba);
+    }
+
+    // ---- the march ------------------------------------------------------------
+    // Step t (z = ri0 + t + 1): primitives of row z, gradients and stress
+    // factors of row z - 1, the x-face between rows z - 2 and z - 1, the
+    // y-faces of row z - 1 (t < H) and the update of row z - 2 (t >= 1).
+    // Step 0 and step H are peeled off, so that every step of the loop issues
+    // the same memory operations in the order their results are needed (the
+    // next row's state, the next update's operands, this row's stores): the
+    // vector memory counter retires loads and stores in order, and a wait for
+    // the next row's state then never waits for this row's stores.
+    // Carried: Pa, Pb = P(z - 2), P(z - 1); Ua, Ub, Uz = U(z - 2), U(z - 1),
+    // U(z); XLa, axa, ba = the x+ face state and stress factors of row z - 2;
+    // Ga = dFy / dy - Fx(- 1/2) / dx of row z - 2 (the update adds Fx(+ 1/2) / dx);
+    // S* = the update operands of row z - 2.
+    double Ga[NV], Ua[NV], Ub[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) { Ga[v] = 0.0; Ua[v] = Um1[v]; Ub[v] = U0r[v]; }
+    double amax = 0.0;
+    const double ndt = -A.dt;
+    double S0[NV], Sb[NV], Sr, Sc = 0.0;
+    // update operands: every step issues the same loads (the cooling plane
+    // read from a valid plane when the law is decided at run time)
+    constexpr bool kReadCool = NV == 4 && COOL != kCoolT4;
+    const size_t cool_plane = beta ? (size_t)(A.cool_factor ? 2 : 1) * NN : 0;
+    auto load_update_operands = [&](int row) {
+        const size_t g = (size_t)(gi0 + row) * N + gj;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            S0[v] = __builtin_nontemporal_load(&gu0[v * NN + g]);
+            if constexpr (RB != 0) Sb[v] = __builtin_nontemporal_load(&gub[v * NN + g]);
+        }
+        Sr = __builtin_nontemporal_load(&gaux[g]);
+        if constexpr (kReadCool) Sc = __builtin_nontemporal_load(&gaux[cool_plane + g]);
+    };
+    load_update_operands(ri0);
+
+    auto step = [&](auto UPD, auto YF, auto NEXT, const int t) {
+        constexpr bool upd = decltype(UPD)::value, yface = decltype(YF)::value, next = decltype(NEXT)::value;
+        const int z = ri0 + t + 1;
+        // next row's conserved state, one step ahead
+        double Un[NV];
+        if constexpr (next) load_cell<NV>(guc, NN, (size_t)grow(z + 1) * N + gj, Un);
+        // primitives of row z
+        double Pz[NV], idz, val;
+        const int code = dev::try_to_primitive<NV>(k, Uz, Pz, val, idz);
+        if (code && z < ri0 + H && z * bs + j < key) { key = z * bs + j; kcode = code; kval = val; }
+        // gradients of row z - 1 (the halo columns' primitives from LDS)
+        double gx[NV], gy[NV], PL[NV], PR[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            PL[v] = dpp<kFromPrev>(hP[(t + 1) * NV + v], Pb[v]);
+            PR[v] = dpp<kFromNext>(hP[((H + 2) + t + 1) * NV + v], Pb[v]);
+        }
+        const double tx = theta3(k, Pa[0], Pb[0], Pz[0]), ty = theta3(k, PL[0], Pb[0], PR[0]);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            gx[v] = dev::plm(tx, Pa[v], Pb[v], Pz[v]);
+            gy[v] = dev::plm(ty, PL[v], Pb[v], PR[v]);
+        }
+        double axb = 0.0, ayb = 0.0, bb = 0.0;
+        if (visc_on) stress_factors<NV>(k, alpha, Pb, idb, gx, gy, axb, ayb, bb);
+        // x-face between rows z - 2 and z - 1 (block-local vertex z - 1)
+        double Fxb[NV];
+        {
+            double pr[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) pr[v] = Pb[v] - gx[v] * 0.5;
+            march_face<NV, 0>(A, visc_on, alpha, XLa, pr, 0.5 * (Pa[1] + Pb[1]), 0.5 * (Pa[2] + Pb[2]), axa + axb,
+                              ba + bb, axx.v(z - 1), yc, Fxb);
+        }
+        // y-faces of row z - 1: lane l evaluates the face between columns
+        // l and l + 1 from its neighbour's right-hand state; the strip's
+        // last face and the face before its first column come from LDS
+        double dFyb[NV];
+        if constexpr (yface) {
+            double yl[NV], yr[NV], F[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                yl[v] = Pb[v] + gy[v] * 0.5;
+                const double own = Pb[v] - gy[v] * 0.5;
+                yr[v] = dpp<kFromNext>(own, own);
+            }
+            const double ayn = dpp<kFromNext>(ayb, ayb), bn = dpp<kFromNext>(bb, bb);
+            double vbx = 0.0, vby = 0.0;
+            if constexpr (NV == 4) {
+                vbx = 0.5 * (Pb[1] + dpp<kFromNext>(Pb[1], Pb[1]));
+                vby = 0.5 * (Pb[2] + dpp<kFromNext>(Pb[2], Pb[2]));
+            }
+            march_face<NV, 1>(A, visc_on, alpha, yl, yr, vbx, vby, ayb + ayn, bb + bn, axx.c(z - 1), yv1, F);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double f = lane == 63 ? Fb[(H + t) * NV + v] : F[v];
+                dFyb[v] = f - dpp<kFromPrev>(Fb[t * NV + v], f);
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) dFyb[v] = 0.0;
+        }
+        // update of row z - 2 (block-local row i = z - 2)
+        if constexpr (upd) {
+            const int i = z - 2;
+            const size_t g = (size_t)(gi0 + i) * N + gj;
+            double du[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) du[v] = (Fxb[v] * k.idx + Ga[v]) * ndt;
+            // primitive recovery of u without floors (the sources' to_primitive)
+            const double id = dev::frcp(Ua[0]);
+            double vx, vy;
+            if (A.reuse_prim) {
+                vx = Pa[1];
+                vy = Pa[2];
+            } else {
+                vx = Ua[1] * id;
+                vy = Ua[2] * id;
+            }
+            double p0 = 0.0;
+            if constexpr (NV == 4) p0 = (Ua[3] - 0.5 * Ua[0] * (vx * vx + vy * vy)) * k.gm1;
+            double it[7][NV];
+            dev::cell_sources<NV, COOL>(A, Ua, S0, axx.c(i), yc, vx, vy, id, p0, Sr, Sc, it);
+            double out[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                double total = it[CDC_ITEM_SINK1][v] + it[CDC_ITEM_SINK2][v] + it[CDC_ITEM_GRAV1][v] +
+                               it[CDC_ITEM_GRAV2][v] + it[CDC_ITEM_BUFFER][v];
+                if constexpr (NV == 4) total = total + it[CDC_ITEM_COOLING][v];
+                else total = total + it[CDC_ITEM_FAKE_MASS][v];
+                const double u1 = Ua[v] + du[v] + total;
+                if constexpr (RB != 0) out[v] = u1 * A.c1 + Sb[v] * A.c0;
+                else out[v] = u1;
+            }
+            // the next update's operands, then this row's stores
+            if constexpr (next) load_update_operands(i + 1);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) __builtin_nontemporal_store(out[v], &guo[v * NN + g]);
+            // into the lane's own LDS slots (no return value: no registers held)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) atomicAdd(&Acc[s * 64 + lane], it[SumMap<NV>::item(s)][SumMap<NV>::comp(s)]);
+            if (A.want_amax) amax = fmax(amax, dev::max_signal_speed<NV>(out, k.gm));
+        }
+        // rotate the rows
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            XLa[v] = Pb[v] + gx[v] * 0.5;
+            Pa[v] = Pb[v];
+            Pb[v] = Pz[v];
+            Ua[v] = Ub[v];
+            Ub[v] = Uz[v];
+            if constexpr (next) Uz[v] = Un[v];
+            Ga[v] = dFyb[v] * k.idy - Fxb[v] * k.idx;
+        }
+        idb = idz;
+        axa = axb;
+        ba = bb;
+        // the next row's state is complete at the end of every step (its
+        // loads were issued at the step's start): the loop is then entered
+        // from the first step and from its own end with nothing of that row
+        // in flight, and its first wait covers only that row's own loads
+        if constexpr (next) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) __asm__ volatile("" ::"v"(Uz[v]));
+        }
+    };
+
+    const std::true_type yes;
+    const std::false_type no;
+    step(no, yes, yes, 0);
+#pragma unroll 1
+    for (int t = 1; t < H; ++t) step(yes, yes, yes, t);
+    step(yes, no, no, H);
+
+    // ---- the strip's sums over its cells: lane s adds up column s in lane
+    // order; the max signal speed by a fixed butterfly --------------------------
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) amax = fmax(amax, __shfl_xor(amax, m, 64));
+    __syncthreads();
+    if (lane < NS) {
+        const double *q = Acc + lane * 64;
+        double x = q[0];
+        for (int c = 1; c < 64; ++c) x += q[c];
+        dev::gptr(A.tile_sums)[(size_t)tile * NR + lane] = x;
+    } else if (lane == NS) {
+        dev::gptr(A.tile_sums)[(size_t)tile * NR + NS] = amax;
+    }
+    // the strip's first failing cell (block-local C order)
+    if (key != INT_MAX) atomicMin(&s_key, key);
+    __syncthreads();
+    const int sk = s_key;
+    if (sk == INT_MAX) {
+        if (lane == 0) A.tile_err[tile].code = 0;
+    } else if (key == sk) {
+        TileErr te;
+        te.code = kcode;
+        te.cell = key;
+        te.value = kval;
+        A.tile_err[tile] = te;
+        atomicCAS(A.err_flag, 0, A.stage_id);
+    }
+}
+
+// One RK stage of a batch of runs (a single run is a batch of one), one wave
+// per strip; with fin, nfin workgroups after the strips finalize the earlier
+// stage, one block each (see stage_kernel_batch).  Two waves per SIMD: the march
+// keeps three rows of state and the next row's operands in registers.
+template <int NV, int H, int COOL, int RB>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+march_kernel_batch(const StageArgs *__restrict__ args, const int32_t *__restrict__ begin, int nruns,
+                   const StageArgs *__restrict__ fin, const int32_t *__restrict__ fin_begin, int fin_nruns,
+                   int nfin) {
+    __shared__ double lds[MarchLds<NV, H>::size];
+    __shared__ int s_key;
+    const int ntiles = begin[nruns];
+    if ((int)blockIdx.x >= ntiles) {
+        // after the strips: every strip is resident from the start (the
+        // register budget allows two waves per SIMD), these take the slots
+        // the first strips free
+        const int b = blockIdx.x - ntiles;
+        const bool active = b < fin_begin[fin_nruns];
+        const int r = active ? run_of(fin_begin, fin_nruns, b) : 0;
+        finalize_block<NV>(fin[r], active ? b - fin_begin[r] : 0, fin[r].tpb, threadIdx.x, active, lds, lds + 7 * 4,
+                           (int *)(lds + 7 * 4 + 7 * 4));
+        return;
+    }
+    const int t = xcd_tile((int)blockIdx.x, ntiles);
+    const int r = run_of(begin, nruns, t);
+    const StageArgs &A = args[r];
+    if (stage_skipped(A)) return;
+    march_strip<NV, H, COOL, RB>(A, t - begin[r], lds, s_key);
+}
+
+template <int NV, int H, int COOL, int RB>
+hipError_t launch_march_t(const StageArgs *args, const int32_t *begin, int nruns, int ntiles, const StageArgs *fin,
+                          const int32_t *fin_begin, int fin_nruns, int nblk, hipEvent_t ev0, hipEvent_t ev1,
+                          hipStream_t s) {
+    const int nfin = fin ? nblk : 0;   // one block per workgroup
+    if (ev0 || ev1)
+        hipExtLaunchKernelGGL((march_kernel_batch<NV, H, COOL, RB>), dim3(ntiles + nfin), dim3(64), 0, s, ev0, ev1, 0, args,
+                              begin, nruns, fin, fin_begin, fin_nruns, nfin);
+    else
+        hipLaunchKernelGGL((march_kernel_batch<NV, H, COOL, RB>), dim3(ntiles + nfin), dim3(64), 0, s, args, begin, nruns, fin,
+                           fin_begin, fin_nruns, nfin);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_march_batch(int nv, int rows, int cool, int read_base, const StageArgs *args,
+                              const int32_t *begin, int nruns, int ntiles, const StageArgs *fin,
+                              const int32_t *fin_begin, int fin_nruns, int nblk, hipEvent_t ev0, hipEvent_t ev1,
+                              hipStream_t s) {
+    const int c = nv == 3 ? (int)kCoolAny : cool;
+    const int rb = read_base ? 1 : 0;
+#define CDC_MARCH(V, R, C, B)                                                                                   \
+    if (nv == V && rows == R && c == C && rb == B)                                                              \
+        return launch_march_t<V, R, C, B>(args, begin, nruns, ntiles, fin, fin_begin, fin_nruns, nblk, ev0, ev1, s);
+#define CDC_MARCH_RB(V, R, C) CDC_MARCH(V, R, C, 0) CDC_MARCH(V, R, C, 1)
+#define CDC_MARCH_ROWS(V, C) CDC_MARCH_RB(V, 32, C) CDC_MARCH_RB(V, 16, C)
+    CDC_MARCH_ROWS(3, kCoolAny)
+    CDC_MARCH_ROWS(4, kCoolAny)
+    CDC_MARCH_ROWS(4, kCoolT4)
+    CDC_MARCH_ROWS(4, kCoolBetaFactor)
+    CDC_MARCH_ROWS(4, kCoolBetaExp)
+#undef CDC_MARCH_ROWS
+#undef CDC_MARCH_RB
+#undef CDC_MARCH
+    return hipErrorInvalidValue;
+}
+
+}  // namespace cdc

@@ -277,9 +277,14 @@ static int enqueue_stages(cdc_solver *s, size_t n, int ntiles, const cdc::TileSh
         // fused: this launch also finalizes stage i - 1 (its tile sums are in
         // the other parity buffer, its ist / oec inputs are complete)
         const StageArgs *fin = fuse && i > 0 ? s->d_args + i - 1 : nullptr;
+        const StageArgs &h = s->h_args[i];
+        const int cool = s->nv == 3 ? cdc::kCoolAny
+                         : s->hydro.cooling_model == CDC_COOLING_BETA ? (h.cool_factor ? cdc::kCoolBetaFactor
+                                                                                        : cdc::kCoolBetaExp)
+                                                                      : cdc::kCoolT4;
         HIP_TRY(cdc::launch_stage_batch(s->nv, ts, s->d_args + i, s->d_begin, 1, ntiles, fin, s->d_begin + 2,
-                                        s->nblk, ev ? ev[2 * i] : nullptr, ev ? ev[2 * i + 1] : nullptr,
-                                        s->stream));
+                                        s->nblk, cool, h.read_base, ev ? ev[2 * i] : nullptr,
+                                        ev ? ev[2 * i + 1] : nullptr, s->stream));
         if (finalize && !fuse)
             HIP_TRY(cdc::launch_finalize_batch(s->nv, s->d_args + i, s->d_begin + 2, 1, s->nblk, s->stream));
     }

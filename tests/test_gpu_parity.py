@@ -430,6 +430,73 @@ def test_every_tile_shape_matches_oracle(kind, shape):
         check_states(g, o)
 
 
+# the column-marching kernel's strips: 64 columns by 32 or 16 rows, one wave
+MARCH = [(32, 64, 64), (16, 64, 64)]
+MARCH_CASES = [
+    ("iso", 2, dict(rk_order="RK2")),
+    ("iso", 2, dict(rk_order="RK3", viscosity={"alpha": 0.1}, sink_model="torque_free", binary_eccentricity=0.4,
+                    binary_mass_ratio=0.3)),
+    ("iso", 1, dict(rk_order="RK1", lambda_multiplier=0.7, fake_mass_rate=1e-3, fake_mass_threshold=0.5,
+                    plm_density_trigger=1e-6)),
+    ("iso", 2, dict(rk_order="RK2", one_body=True)),
+    ("euler", 2, dict(rk_order="RK3", viscosity={"alpha": 0.1}, binary_eccentricity=0.5, binary_mass_ratio=0.5)),
+    ("euler", 2, dict(rk_order="RK2", sink_model="torque_free", mach_ceiling=5.0, lambda_multiplier=0.3,
+                      plm_density_trigger=1e-6)),
+    ("euler_beta", 2, dict(rk_order="RK3", viscosity={"constant_nu": 2e-3})),
+    ("euler_beta", 1, dict(rk_order="RK1", viscosity={"alpha": 0.05}, sink_model="torque_free")),
+]
+
+
+@pytest.mark.parametrize("shape", MARCH, ids=[f"{a}x{b}" for a, b, _ in MARCH])
+@pytest.mark.parametrize("kind,nb,kw", MARCH_CASES, ids=[f"{c[0]}-{i}" for i, c in enumerate(MARCH_CASES)])
+def test_march_kernel_matches_oracle(kind, nb, kw, shape):
+    """The column-marching stage kernel on blocks of 64^2 (every sink,
+    viscosity, cooling, PLM-trigger and RK variant), fold 1 and fold 3
+    calls, against the oracle (check_states after every call)."""
+    mesh, hydro, ph = H.small(kind, nb=nb, bs=64, R=4.0, **kw)
+    u0 = H.initial(mesh, hydro, ph)
+    g, o = pair(mesh, hydro, ph, u0)
+    g.set_kernel(*shape)
+    assert g.get_kernel() == shape
+    for step in range(5):
+        fold = 1 if step < 3 else 3
+        assert g.advance(fold) == pytest.approx(o.advance(fold), rel=1e-12)
+        check_states(g, o, one_body=ph.one_body)
+
+
+@pytest.mark.parametrize("shape", MARCH, ids=[f"{a}x{b}" for a, b, _ in MARCH])
+def test_march_kernel_errors_match_oracle(shape):
+    """The first failing cell of each block (block order, then C order inside
+    the block) with the march kernel: code, value and position as the
+    oracle's, the state left as it was."""
+    mesh, hydro, ph = H.small("iso", nb=2, bs=64, R=4.0)
+    u0 = H.initial(mesh, hydro, ph)
+    g, o = pair(mesh, hydro, ph, u0)
+    g.set_kernel(*shape)
+    g.advance(1); o.advance(1)
+    bad = g.get_state()["conserved"].copy()
+    # all in block 3, which block 0's first 3x3 neighbour is: cells in both
+    # row segments and at the strip's edges; the first in C order is (5, 60)
+    bad[3, 40, 63, 0] = -1e-9
+    bad[3, 33, 0, 0] = -2e-9
+    bad[3, 5, 61, 0] = -3e-9
+    bad[3, 5, 60, 0] = -4e-9
+    before = g.get_state()
+    g.set_state(bad, time=before["time"], iteration=before["iteration"], ist=before["ist"], oec=before["oec"])
+    so = o.get_state()
+    o.set_state(bad, time=so["time"], iteration=so["iteration"], ist=so["ist"], oec=so["oec"])
+    with pytest.raises(HydroError) as eg:
+        g.advance(1)
+    with pytest.raises(O.OracleError) as eo:
+        o.advance(1)
+    import ctypes
+    ref = abi.CError.from_buffer_copy(eo.value.err[:ctypes.sizeof(abi.CError)])
+    assert eg.value.kind == "NegativeDensity" and ref.code == abi.CDC_ERR_NEGATIVE_DENSITY
+    assert eg.value.position == (ref.x, ref.y)
+    assert eg.value.value == pytest.approx(ref.value, rel=1e-12) and ref.value == -4e-9
+    np.testing.assert_array_equal(g.get_state()["conserved"], bad)
+
+
 def test_autotune_keeps_state_and_picks_a_valid_shape():
     mesh, hydro, ph = H.small("euler", nb=2, bs=64, R=4.0)
     u0 = H.initial(mesh, hydro, ph)
@@ -437,7 +504,7 @@ def test_autotune_keeps_state_and_picks_a_valid_shape():
     g.advance(1); o.advance(1)
     before = g.get_state()
     tx, ty, ms = g.autotune(reps=2)
-    assert g.get_kernel() in SHAPES and ms > 0 and g.get_tile() == (tx, ty)
+    assert g.get_kernel() in SHAPES + MARCH and ms > 0 and g.get_tile() == (tx, ty)
     after = g.get_state()
     np.testing.assert_array_equal(before["conserved"], after["conserved"])
     np.testing.assert_array_equal(before["ist"], after["ist"])
