@@ -71,30 +71,52 @@ __device__ __forceinline__ void stress_factors(const StageConsts &k, bool alpha,
     b = K * off;
 }
 
+// A double held in a vector register from here on: the constants every face
+// reads, so that they neither occupy scalar registers across the march nor
+// need a copy into a vector register at each use (a vector instruction reads
+// at most one scalar operand).
+__device__ __forceinline__ double in_vgpr(double x) {
+    double y;
+    __asm__("v_mov_b64 %0, %1" : "=v"(y) : "v"(x));
+    return y;
+}
+
+// The per-stage constants of the face evaluation, in vector registers.
+struct FaceConsts {
+    double x1, y1, x2, y2, m1, m2;   // the bodies (Orbit)
+    double rs2, eps_dx, gm, inv_gm1, inv_mach2, plm;
+    __device__ __forceinline__ explicit FaceConsts(const StageArgs &A)
+        : x1(in_vgpr(A.orb.x1)), y1(in_vgpr(A.orb.y1)), x2(in_vgpr(A.orb.x2)), y2(in_vgpr(A.orb.y2)),
+          m1(in_vgpr(A.orb.m1)), m2(in_vgpr(A.orb.m2)), rs2(in_vgpr(A.k.rs2)), eps_dx(in_vgpr(A.k.eps_dx)),
+          gm(in_vgpr(A.k.gm)), inv_gm1(in_vgpr(A.k.inv_gm1)), inv_mach2(in_vgpr(A.k.inv_mach2)),
+          plm(in_vgpr(A.k.plm)) {}
+};
+
 // One face (AXIS 0: x-face, 1: y-face) from its reconstructed states pl, pr:
 // HLLE (Toro ch. 10) plus the viscous flux 0.5 f (sa, sb); (vbx, vby) the
 // mean of the two cell-centre velocities (gamma-law energy flux).
 template <int NV, int AXIS>
-__device__ __forceinline__ void march_face(const StageArgs &A, bool visc_on, bool alpha, const double *pl,
+__device__ __forceinline__ void march_face(const FaceConsts &c, bool visc_on, bool alpha, const double *pl,
                                            const double *pr, double vbx, double vby, double sa, double sb,
                                            double fxp, double fyp, double *F) {
-    const StageConsts &k = A.k;
-    const Orbit &o = A.orb;
-    const double d1x = fxp - o.x1, d1y = fyp - o.y1;
-    const double d2x = fxp - o.x2, d2y = fyp - o.y2;
+    const double d1x = fxp - c.x1, d1y = fyp - c.y1;
+    const double d2x = fxp - c.x2, d2y = fyp - c.y2;
     const double r1s = d1x * d1x + d1y * d1y;
     const double r2s = d2x * d2x + d2y * d2y;
     double cs2f = 0.0;
-    if constexpr (NV == 3) cs2f = (o.m1 * dev::frsq(r1s + k.rs2) + o.m2 * dev::frsq(r2s + k.rs2)) * k.inv_mach2;
+    if constexpr (NV == 3) cs2f = (c.m1 * dev::frsq(r1s + c.rs2) + c.m2 * dev::frsq(r2s + c.rs2)) * c.inv_mach2;
     double tn = 0.0, to = 0.0;
     if (visc_on) {
         double f = 0.5;
         if (alpha) {
             // 1 / sqrt(m1/q1^3 + m2/q2^3), q = r + 1e-12 dx
-            const double q1 = dev::fsqrt(r1s) + k.eps_dx;
-            const double q2 = dev::fsqrt(r2s) + k.eps_dx;
+            const double q1 = dev::fsqrt(r1s) + c.eps_dx;
+            const double q2 = dev::fsqrt(r2s) + c.eps_dx;
             const double c1 = q1 * q1 * q1, c2 = q2 * q2 * q2;
-            const double itf = dev::frsq((o.m1 * c2 + o.m2 * c1) * dev::frcp(c1 * c2));
+            // rsq(X / Y) = Y rsq(X Y), X = m1 q2^3 + m2 q1^3, Y = q1^3 q2^3:
+            // one transcendental instead of two
+            const double Y = c1 * c2;
+            const double itf = Y * dev::frsq((c.m1 * c2 + c.m2 * c1) * Y);
             f = NV == 3 ? 0.5 * (cs2f * itf) : 0.5 * itf;
         }
         tn = f * sa;
@@ -110,9 +132,9 @@ __device__ __forceinline__ void march_face(const StageArgs &A, bool visc_on, boo
         ppl = pl[0] * cs2f;
         ppr = pr[0] * cs2f;
     } else {
-        ul[3] = 0.5 * pl[0] * (pl[1] * pl[1] + pl[2] * pl[2]) + pl[3] * k.inv_gm1;
-        ur[3] = 0.5 * pr[0] * (pr[1] * pr[1] + pr[2] * pr[2]) + pr[3] * k.inv_gm1;
-        const double gpl = k.gm * pl[3], gpr = k.gm * pr[3];
+        ul[3] = 0.5 * pl[0] * (pl[1] * pl[1] + pl[2] * pl[2]) + pl[3] * c.inv_gm1;
+        ur[3] = 0.5 * pr[0] * (pr[1] * pr[1] + pr[2] * pr[2]) + pr[3] * c.inv_gm1;
+        const double gpl = c.gm * pl[3], gpr = c.gm * pr[3];
         csl = gpl * dev::frsq(gpl * pl[0]);   // sqrt(gamma p / Sigma)
         csr = gpr * dev::frsq(gpr * pr[0]);
         ppl = pl[3];
@@ -130,25 +152,48 @@ __device__ __forceinline__ void march_face(const StageArgs &A, bool visc_on, boo
     const double ap = fmax(fmax(vnl + csl, vnr + csr), 0.0);
     const double am = fmin(fmin(vnl - csl, vnr - csr), 0.0);
     const double inv = dev::frcp(ap - am);
-    const double apam = ap * am;
+    // F = (ap F_L - am F_R - ap am (U_L - U_R)) / (ap - am) with the three
+    // weights formed once
+    const double wl = ap * inv, wr = -am * inv, wd = -(ap * am) * inv;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) F[v] = (fl[v] * ap - fr[v] * am - (ul[v] - ur[v]) * apam) * inv;
+    for (int v = 0; v < NV; ++v) F[v] = fl[v] * wl + (fr[v] * wr + (ul[v] - ur[v]) * wd);
     const double tau_x = AXIS ? to : tn, tau_y = AXIS ? tn : to;
     F[1] -= tau_x;
     F[2] -= tau_y;
     if constexpr (NV == 4) F[3] -= tau_x * vbx + tau_y * vby;
 }
 
+// Buffer descriptors for the state arrays: one per array (4 SGPRs), the
+// cell's byte offset in a VGPR and the plane's in an SGPR, so that the
+// arrays' planes need no 64-bit base pointers of their own.  Offsets are
+// 32-bit: the kernel runs on grids whose arrays hold under 2^32 bytes.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr int kBufWord3 = 0x00020000;   // raw buffer: 32-bit data format, no swizzle
+constexpr int kNt = 2;                  // nontemporal cache policy
+
+__device__ __forceinline__ Rsrc rsrc(const double *p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, kBufWord3);
+}
+template <int AUX = 0>
+__device__ __forceinline__ double bload(Rsrc r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, AUX));
+}
+__device__ __forceinline__ void bstore_nt(double x, Rsrc r, unsigned voff, unsigned soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, x), r,
+                                          (int)voff, (int)soff, kNt);
+}
+
+// cell at byte offset voff, planes pl bytes apart
 template <int NV>
-__device__ __forceinline__ void load_cell(dev::gcdouble *u, size_t NN, size_t g, double *out) {
+__device__ __forceinline__ void load_cell(Rsrc r, unsigned voff, unsigned pl, double *out) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) out[v] = u[v * NN + g];
+    for (int v = 0; v < NV; ++v) out[v] = bload(r, voff, (unsigned)v * pl);
 }
 
 // PLM theta of the three cells a, b, c (plm_density_trigger)
-__device__ __forceinline__ double theta3(const StageConsts &k, double da, double db, double dc) {
+__device__ __forceinline__ double theta3(const StageConsts &k, double plm, double da, double db, double dc) {
     if (k.has_pdt && (da < k.pdt || db < k.pdt || dc < k.pdt)) return 0.0;
-    return k.plm;
+    return plm;
 }
 
 template <int NV, int H>
@@ -175,8 +220,10 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
     const int bi = b / A.nb, bj = b - (b / A.nb) * A.nb;
     const int ri0 = (lt / spb) * H, cj0 = (lt - (lt / spb) * spb) * 64;
     const size_t NN = (size_t)N * N;
-    dev::gcdouble *guc = dev::gptr(A.uc), *gu0 = dev::gptr(A.u0), *gub = dev::gptr(A.ub), *gaux = dev::gptr(A.aux);
-    dev::gdouble *guo = dev::gptr(A.uo);
+    const unsigned PB = __builtin_amdgcn_readfirstlane((unsigned)(NN * 8));   // plane stride in bytes
+    const Rsrc ruc = rsrc(A.uc, NV * PB), ru0 = rsrc(A.u0, NV * PB), ruo = rsrc(A.uo, NV * PB);
+    const Rsrc rub = rsrc(RB != 0 ? A.ub : A.uc, NV * PB);
+    const Rsrc raux = rsrc(A.aux, (NV == 4 && COOL != kCoolT4 ? 3u : 1u) * PB);
     const StageConsts &k = A.k;
     const dev::Axis axx = dev::block_axis(A.R, A.L, bi, bs);
     const dev::Axis axy = dev::block_axis(A.R, A.L, bj, bs);
@@ -187,6 +234,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
     // wrapped global row / column of a block-local index
     auto grow = [&](int i) { const int g = gi0 + i; return g < 0 ? g + N : (g >= N ? g - N : g); };
     auto gcol = [&](int c) { const int g = gj0 + c; return g < 0 ? g + N : (g >= N ? g - N : g); };
+    const FaceConsts fc(A);
     double *hP = lds + ML::HP;
     double *Fb = lds + ML::FB;
     double *Acc = lds + ML::ACC;
@@ -201,13 +249,13 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
     for (int q = lane; q < 2 * H; q += 64) {
         const int s = q / H, r = ri0 + (q - s * H);
         const int cL = s == 0 ? cj0 - 1 : cj0 + 63, cR = cL + 1;
-        const size_t rm = (size_t)grow(r - 1) * N, r0 = (size_t)grow(r) * N, rp = (size_t)grow(r + 1) * N;
-        const int cLL = gcol(cL - 1), cLg = gcol(cL), cRg = gcol(cR), cRR = gcol(cR + 1);
+        const unsigned rm = (unsigned)grow(r - 1) * N, r0 = (unsigned)grow(r) * N, rp = (unsigned)grow(r + 1) * N;
+        const unsigned cLL = gcol(cL - 1), cLg = gcol(cL), cRg = gcol(cR), cRR = gcol(cR + 1);
         // cells: 0 (r, cL-1), 1 (r, cL), 2 (r, cR), 3 (r, cR+1), 4 (r-1, cL), 5 (r+1, cL), 6 (r-1, cR), 7 (r+1, cR)
-        const size_t g8[8] = {r0 + cLL, r0 + cLg, r0 + cRg, r0 + cRR, rm + cLg, rp + cLg, rm + cRg, rp + cRg};
+        const unsigned g8[8] = {r0 + cLL, r0 + cLg, r0 + cRg, r0 + cRR, rm + cLg, rp + cLg, rm + cRg, rp + cRg};
         double u[8][NV];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) load_cell<NV>(guc, NN, g8[c], u[c]);
+        for (int c = 0; c < 8; ++c) load_cell<NV>(ruc, g8[c] * 8u, PB, u[c]);
         double p[8][NV], id[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
@@ -215,8 +263,8 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             (void)dev::try_to_primitive<NV>(k, u[c], p[c], val, id[c]);
         }
         double gxL[NV], gyL[NV], gxR[NV], gyR[NV];
-        const double txL = theta3(k, p[4][0], p[1][0], p[5][0]), tyL = theta3(k, p[0][0], p[1][0], p[2][0]);
-        const double txR = theta3(k, p[6][0], p[2][0], p[7][0]), tyR = theta3(k, p[1][0], p[2][0], p[3][0]);
+        const double txL = theta3(k, fc.plm, p[4][0], p[1][0], p[5][0]), tyL = theta3(k, fc.plm, p[0][0], p[1][0], p[2][0]);
+        const double txR = theta3(k, fc.plm, p[6][0], p[2][0], p[7][0]), tyR = theta3(k, fc.plm, p[1][0], p[2][0], p[3][0]);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             gxL[v] = dev::plm(txL, p[4][v], p[1][v], p[5][v]);
@@ -235,7 +283,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             pl[v] = p[1][v] + gyL[v] * 0.5;
             pr[v] = p[2][v] - gyR[v] * 0.5;
         }
-        march_face<NV, 1>(A, visc_on, alpha, pl, pr, 0.5 * (p[1][1] + p[2][1]), 0.5 * (p[1][2] + p[2][2]),
+        march_face<NV, 1>(fc, visc_on, alpha, pl, pr, 0.5 * (p[1][1] + p[2][1]), 0.5 * (p[1][2] + p[2][2]),
                           ayL + ayR, bL + bR, axx.c(r), axy.v(cR), F);
         const int hr = r - ri0;
 #pragma unroll
@@ -253,10 +301,10 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
     const int gj = gj0 + j;                   // global column (inside the block)
     const double yc = axy.c(j), yv1 = axy.v(j + 1);
     double Um2[NV], Um1[NV], U0r[NV], Uz[NV];
-    load_cell<NV>(guc, NN, (size_t)grow(ri0 - 2) * N + gj, Um2);
-    load_cell<NV>(guc, NN, (size_t)grow(ri0 - 1) * N + gj, Um1);
-    load_cell<NV>(guc, NN, (size_t)grow(ri0) * N + gj, U0r);
-    load_cell<NV>(guc, NN, (size_t)grow(ri0 + 1) * N + gj, Uz);
+    load_cell<NV>(ruc, ((unsigned)grow(ri0 - 2) * N + gj) * 8u, PB, Um2);
+    load_cell<NV>(ruc, ((unsigned)grow(ri0 - 1) * N + gj) * 8u, PB, Um1);
+    load_cell<NV>(ruc, ((unsigned)grow(ri0) * N + gj) * 8u, PB, U0r);
+    load_cell<NV>(ruc, ((unsigned)grow(ri0 + 1) * N + gj) * 8u, PB, Uz);
 #pragma unroll
     for (int s = 0; s < NS; ++s) Acc[s * 64 + lane] = 0.0;
     __syncthreads();   // the edge data in LDS
@@ -276,7 +324,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             PL[v] = dpp<kFromPrev>(hP[v], Pa[v]);
             PR[v] = dpp<kFromNext>(hP[(H + 2) * NV + v], Pa[v]);
         }
-        const double tx = theta3(k, Pm2[0], Pa[0], Pb[0]), ty = theta3(k, PL[0], Pa[0], PR[0]);
+        const double tx = theta3(k, fc.plm, Pm2[0], Pa[0], Pb[0]), ty = theta3(k, fc.plm, PL[0], Pa[0], PR[0]);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             gx[v] = dev::plm(tx, Pm2[v], Pa[v], Pb[v]);
@@ -309,16 +357,16 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
     // update operands: every step issues the same loads (the cooling plane
     // read from a valid plane when the law is decided at run time)
     constexpr bool kReadCool = NV == 4 && COOL != kCoolT4;
-    const size_t cool_plane = beta ? (size_t)(A.cool_factor ? 2 : 1) * NN : 0;
+    const unsigned cool_plane = __builtin_amdgcn_readfirstlane(beta ? (A.cool_factor ? 2u : 1u) * PB : 0u);
     auto load_update_operands = [&](int row) {
-        const size_t g = (size_t)(gi0 + row) * N + gj;
+        const unsigned g = ((unsigned)(gi0 + row) * N + gj) * 8u;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            S0[v] = __builtin_nontemporal_load(&gu0[v * NN + g]);
-            if constexpr (RB != 0) Sb[v] = __builtin_nontemporal_load(&gub[v * NN + g]);
+            S0[v] = bload<kNt>(ru0, g, (unsigned)v * PB);
+            if constexpr (RB != 0) Sb[v] = bload<kNt>(rub, g, (unsigned)v * PB);
         }
-        Sr = __builtin_nontemporal_load(&gaux[g]);
-        if constexpr (kReadCool) Sc = __builtin_nontemporal_load(&gaux[cool_plane + g]);
+        Sr = bload<kNt>(raux, g, 0u);
+        if constexpr (kReadCool) Sc = bload<kNt>(raux, g, cool_plane);
     };
     load_update_operands(ri0);
 
@@ -327,7 +375,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
         const int z = ri0 + t + 1;
         // next row's conserved state, one step ahead
         double Un[NV];
-        if constexpr (next) load_cell<NV>(guc, NN, (size_t)grow(z + 1) * N + gj, Un);
+        if constexpr (next) load_cell<NV>(ruc, ((unsigned)grow(z + 1) * N + gj) * 8u, PB, Un);
         // primitives of row z
         double Pz[NV], idz, val;
         const int code = dev::try_to_primitive<NV>(k, Uz, Pz, val, idz);
@@ -339,7 +387,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             PL[v] = dpp<kFromPrev>(hP[(t + 1) * NV + v], Pb[v]);
             PR[v] = dpp<kFromNext>(hP[((H + 2) + t + 1) * NV + v], Pb[v]);
         }
-        const double tx = theta3(k, Pa[0], Pb[0], Pz[0]), ty = theta3(k, PL[0], Pb[0], PR[0]);
+        const double tx = theta3(k, fc.plm, Pa[0], Pb[0], Pz[0]), ty = theta3(k, fc.plm, PL[0], Pb[0], PR[0]);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             gx[v] = dev::plm(tx, Pa[v], Pb[v], Pz[v]);
@@ -353,7 +401,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             double pr[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) pr[v] = Pb[v] - gx[v] * 0.5;
-            march_face<NV, 0>(A, visc_on, alpha, XLa, pr, 0.5 * (Pa[1] + Pb[1]), 0.5 * (Pa[2] + Pb[2]), axa + axb,
+            march_face<NV, 0>(fc, visc_on, alpha, XLa, pr, 0.5 * (Pa[1] + Pb[1]), 0.5 * (Pa[2] + Pb[2]), axa + axb,
                               ba + bb, axx.v(z - 1), yc, Fxb);
         }
         // y-faces of row z - 1: lane l evaluates the face between columns
@@ -374,7 +422,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
                 vbx = 0.5 * (Pb[1] + dpp<kFromNext>(Pb[1], Pb[1]));
                 vby = 0.5 * (Pb[2] + dpp<kFromNext>(Pb[2], Pb[2]));
             }
-            march_face<NV, 1>(A, visc_on, alpha, yl, yr, vbx, vby, ayb + ayn, bb + bn, axx.c(z - 1), yv1, F);
+            march_face<NV, 1>(fc, visc_on, alpha, yl, yr, vbx, vby, ayb + ayn, bb + bn, axx.c(z - 1), yv1, F);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const double f = lane == 63 ? Fb[(H + t) * NV + v] : F[v];
@@ -387,7 +435,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
         // update of row z - 2 (block-local row i = z - 2)
         if constexpr (upd) {
             const int i = z - 2;
-            const size_t g = (size_t)(gi0 + i) * N + gj;
+            const unsigned g = ((unsigned)(gi0 + i) * N + gj) * 8u;
             double du[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) du[v] = (Fxb[v] * k.idx + Ga[v]) * ndt;
@@ -419,7 +467,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             // the next update's operands, then this row's stores
             if constexpr (next) load_update_operands(i + 1);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) __builtin_nontemporal_store(out[v], &guo[v * NN + g]);
+            for (int v = 0; v < NV; ++v) bstore_nt(out[v], ruo, g, (unsigned)v * PB);
             // into the lane's own LDS slots (no return value: no registers held)
 #pragma unroll
             for (int s = 0; s < NS; ++s) atomicAdd(&Acc[s * 64 + lane], it[SumMap<NV>::item(s)][SumMap<NV>::comp(s)]);
