@@ -33,11 +33,17 @@ def algorithmic_bytes_per_step(nv, rk):
     return 8 * nv * (3 + 4 * (rk - 1))
 
 
+def kernel_name(tile):
+    """The stage kernel a tile shape selects: 64-column strips are the
+    column-marching kernel's."""
+    return "march_kernel_batch" if tile.endswith("x64") else "stage_kernel_batch"
+
+
 def committed_profile(workload, tile):
     """The committed PMC summary of the stage kernel (HBM traffic from
     calibrated FETCH_SIZE / WRITE_SIZE passes, VALU instructions per wave)
     when it covers this workload and tile; {} otherwise."""
-    path = os.path.join(ROOT, "profiles", f"stage_kernel_{workload}_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"{kernel_name(tile)}_{workload}_{tile}_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
@@ -360,7 +366,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": prof.get("traffic_bytes_per_launch"),
-                     "kernel": "stage_kernel_batch", "launches": launches,
+                     "kernel": kernel_name("%dx%d" % g.get_tile()), "launches": launches,
                      "avg_launch_ms": kernel_ms / max(launches, 1),
                      "bytes_per_launch": bytes_total / max(launches, 1),
                      "valu_issue": valu_issue(prof, kernel_ms / max(launches, 1)),
