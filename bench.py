@@ -43,7 +43,8 @@ def committed_profile(workload, tile):
     """The committed PMC summary of the stage kernel (HBM traffic from
     calibrated FETCH_SIZE / WRITE_SIZE passes, VALU instructions per wave)
     when it covers this workload and tile; {} otherwise."""
-    path = os.path.join(ROOT, "profiles", f"{kernel_name(tile)}_{workload}_{tile}_traffic.json")
+    stem = kernel_name(tile).replace("_batch", "")
+    path = os.path.join(ROOT, "profiles", f"{stem}_{workload}_{tile}_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
