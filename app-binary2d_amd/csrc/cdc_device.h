@@ -117,33 +117,38 @@ __device__ __forceinline__ double plm(double th, double a, double b, double c) {
 }
 
 // try_to_primitive: iso src/physics.rs:489-503, Euler src/physics.rs:631-660.
-// Also returns 1/Sigma of the recovered state in `id`.
+// Also returns 1/Sigma of the recovered state in `id`, and in `val` the
+// offending value when the result is an error code.  The floors compare
+// against StageConsts::*_floor_cmp (-inf when unset) and the error is chosen
+// by selects, so the recovery has no per-cell branch but the rare floor.
 template <int NV>
 __device__ __forceinline__ int try_to_primitive(const StageConsts &k, const double *u, double *p,
                                                 double &val, double &id) {
     if constexpr (NV == 3) {
         double d = u[0], sx = u[1], sy = u[2];
-        if (k.has_density_floor && d < k.density_floor) {
+        if (d < k.density_floor_cmp) {
             sx = 0.0; sy = 0.0; d = k.density_floor;
         }
         id = frcp(d);
         p[0] = d; p[1] = sx * id; p[2] = sy * id;
-        if (d < 0.0) { val = d; return CDC_ERR_NEGATIVE_DENSITY; }
-        return 0;
+        val = d;
+        return d < 0.0 ? CDC_ERR_NEGATIVE_DENSITY : 0;
     } else {
         const double d = u[0];
         id = frcp(d);
         const double vx = u[1] * id, vy = u[2] * id;
         p[0] = d; p[1] = vx; p[2] = vy;
         p[3] = (u[3] - 0.5 * d * (vx * vx + vy * vy)) * k.gm1;
-        if (k.has_density_floor && d < k.density_floor) {
+        if (d < k.density_floor_cmp) {
             p[0] = k.density_floor; p[1] = 0.0; p[2] = 0.0; p[3] = k.density_floor_pressure;
             id = frcp(k.density_floor);
         }
-        if (p[0] < 0.0) { val = p[0]; return CDC_ERR_NEGATIVE_DENSITY; }
-        if (k.has_pressure_floor && p[3] < k.pressure_floor) p[3] = k.pressure_floor;
-        if (p[3] <= 0.0) { val = p[3]; return CDC_ERR_NEGATIVE_PRESSURE; }
-        return 0;
+        // negative density first; the pressure floor then the pressure check
+        const bool nd = p[0] < 0.0;
+        const double pf = p[3] < k.pressure_floor_cmp ? k.pressure_floor_cmp : p[3];
+        p[3] = pf;
+        val = nd ? p[0] : pf;
+        return nd ? CDC_ERR_NEGATIVE_DENSITY : (pf <= 0.0 ? CDC_ERR_NEGATIVE_PRESSURE : 0);
     }
 }
 
@@ -274,6 +279,47 @@ __device__ __forceinline__ void face_flux(const StageArgs &A, int axis, const do
     }
 }
 
+// Where a cell sits relative to the two bodies, and its sink kernel rates.
+struct SinkGeom {
+    double ex1, ey1, ex2, ey2, q1, q2, sr1, sr2;
+};
+
+// The two sink items (src/physics.rs:548-569, 717-744): impartial, or
+// torque-free / acceleration-free with the gas's velocity relative to the
+// body along the line of centres.  it[SINK1], it[SINK2] (times dt).
+template <int NV>
+__device__ __forceinline__ void sink_items(const StageConsts &k, const Orbit &o, double dt, const double *u,
+                                           double vx, double vy, double id, double p0, const SinkGeom &g,
+                                           double it[7][NV]) {
+    const double sd = u[0], sr1 = g.sr1, sr2 = g.sr2;
+    double s1[NV], s2[NV];
+    if (k.sink_model == CDC_SINK_IMPARTIAL) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { s1[v] = u[v] * -sr1; s2[v] = u[v] * -sr2; }
+    } else {
+        const double vx1 = k.one_body ? 0.0 : o.vx1, vy1 = k.one_body ? 0.0 : o.vy1;
+        const double vx2 = k.one_body ? 0.0 : o.vx2, vy2 = k.one_body ? 0.0 : o.vy2;
+        const double ir1 = frsq(g.q1), ir2 = frsq(g.q2);
+        const double r1x = g.ex1 * ir1, r1y = g.ey1 * ir1, r2x = g.ex2 * ir2, r2y = g.ey2 * ir2;
+        const double dv1 = (vx - vx1) * r1x + (vy - vy1) * r1y;
+        const double dv2 = (vx - vx2) * r2x + (vy - vy2) * r2y;
+        const double vx1c = dv1 * r1x + vx1, vy1c = dv1 * r1y + vy1;
+        const double vx2c = dv2 * r2x + vx2, vy2c = dv2 * r2y + vy2;
+        s1[0] = sd * -sr1; s1[1] = (sd * vx1c) * -sr1; s1[2] = (sd * vy1c) * -sr1;
+        s2[0] = sd * -sr2; s2[1] = (sd * vx2c) * -sr2; s2[2] = (sd * vy2c) * -sr2;
+        if constexpr (NV == 4) {
+            const double e0 = p0 * (id * k.inv_gm1);
+            s1[3] = (sd * e0 + 0.5 * sd * (vx1c * vx1c + vy1c * vy1c)) * -sr1;
+            s2[3] = (sd * e0 + 0.5 * sd * (vx2c * vx2c + vy2c * vy2c)) * -sr2;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        it[CDC_ITEM_SINK1][v] = s1[v] * dt;
+        it[CDC_ITEM_SINK2][v] = s2[v] * dt;
+    }
+}
+
 // Isothermal / Euler source_terms (src/physics.rs:528-582, 683-774) with
 // Physics::source_terms (src/physics.rs:423-457).  `sd, vx, vy, id, p0` is
 // the primitive recovery of u without floors; `brate` the static buffer
@@ -281,12 +327,15 @@ __device__ __forceinline__ void face_flux(const StageArgs &A, int axis, const do
 // relaxation factor exp(-dt Omega / beta) itself.  it[item][v].
 // `cool`: the cooling law when it is fixed at compile time (StageCooling);
 // kCoolAny selects it at run time from the stage arguments.
-template <int NV, int COOL = kCoolAny>
-__device__ __forceinline__ void cell_sources(const StageArgs &A, const double *u, const double *u0, double x,
-                                             double y, double vx, double vy, double id, double p0,
-                                             double brate, double cool, double it[7][NV]) {
+// `o` is the stage's orbit (A.orb, or a copy the caller holds in registers).
+// With LAZY_SINKS the sink items are not written: `g` returns what
+// sink_items needs, and a caller evaluates them only where a rate is
+// nonzero (elsewhere they are +-0).
+template <int NV, int COOL = kCoolAny, bool LAZY_SINKS = false>
+__device__ __forceinline__ void cell_sources(const StageArgs &A, const Orbit &o, const double *u, const double *u0,
+                                             double x, double y, double vx, double vy, double id, double p0,
+                                             double brate, double cool, double it[7][NV], SinkGeom &g) {
     const StageConsts &k = A.k;
-    const Orbit &o = A.orb;
     const double dt = A.dt;
     const double sd = u[0];
 
@@ -301,35 +350,11 @@ __device__ __forceinline__ void cell_sources(const StageArgs &A, const double *u
     double sr1 = 0.0, sr2 = 0.0;
     if (q1 < k.sink_s2) { const double w = 1.0 - q1 * k.inv_sink_s2; sr1 = k.sink_rate * (w * w); }
     if (q2 < k.sink_s2) { const double w = 1.0 - q2 * k.inv_sink_s2; sr2 = k.sink_rate * (w * w); }
-
-    double s1[NV], s2[NV];
-    if (k.sink_model == CDC_SINK_IMPARTIAL) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) { s1[v] = u[v] * -sr1; s2[v] = u[v] * -sr2; }
-    } else {
-        const double vx1 = k.one_body ? 0.0 : o.vx1, vy1 = k.one_body ? 0.0 : o.vy1;
-        const double vx2 = k.one_body ? 0.0 : o.vx2, vy2 = k.one_body ? 0.0 : o.vy2;
-        const double ir1 = frsq(q1), ir2 = frsq(q2);
-        const double r1x = ex1 * ir1, r1y = ey1 * ir1, r2x = ex2 * ir2, r2y = ey2 * ir2;
-        const double dv1 = (vx - vx1) * r1x + (vy - vy1) * r1y;
-        const double dv2 = (vx - vx2) * r2x + (vy - vy2) * r2y;
-        const double vx1c = dv1 * r1x + vx1, vy1c = dv1 * r1y + vy1;
-        const double vx2c = dv2 * r2x + vx2, vy2c = dv2 * r2y + vy2;
-        s1[0] = sd * -sr1; s1[1] = (sd * vx1c) * -sr1; s1[2] = (sd * vy1c) * -sr1;
-        s2[0] = sd * -sr2; s2[1] = (sd * vx2c) * -sr2; s2[2] = (sd * vy2c) * -sr2;
-        if constexpr (NV == 4) {
-            const double e0 = p0 * (id * k.inv_gm1);
-            s1[3] = (sd * e0 + 0.5 * sd * (vx1c * vx1c + vy1c * vy1c)) * -sr1;
-            s2[3] = (sd * e0 + 0.5 * sd * (vx2c * vx2c + vy2c * vy2c)) * -sr2;
-        }
-    }
+    g.ex1 = ex1; g.ey1 = ey1; g.ex2 = ex2; g.ey2 = ey2; g.q1 = q1; g.q2 = q2; g.sr1 = sr1; g.sr2 = sr2;
+    if constexpr (!LAZY_SINKS) sink_items<NV>(k, o, dt, u, vx, vy, id, p0, g, it);
     const double bfac = -dt * brate;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        it[CDC_ITEM_SINK1][v] = s1[v] * dt;
-        it[CDC_ITEM_SINK2][v] = s2[v] * dt;
-        it[CDC_ITEM_BUFFER][v] = (u[v] - u0[v]) * bfac;
-    }
+    for (int v = 0; v < NV; ++v) it[CDC_ITEM_BUFFER][v] = (u[v] - u0[v]) * bfac;
     it[CDC_ITEM_GRAV1][0] = 0.0; it[CDC_ITEM_GRAV1][1] = fx1 * dt; it[CDC_ITEM_GRAV1][2] = fy1 * dt;
     it[CDC_ITEM_GRAV2][0] = 0.0; it[CDC_ITEM_GRAV2][1] = fx2 * dt; it[CDC_ITEM_GRAV2][2] = fy2 * dt;
 

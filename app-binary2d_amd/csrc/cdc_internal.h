@@ -34,6 +34,9 @@ struct StageConsts {
     double cooling_f;           // T^4 slow-start factor exp(-2 pi / (t + dt)) or 1
     double mach_ceiling_coef;   // 2 / gm / (gm - 1) / mach_ceiling^2
     double density_floor, pressure_floor, density_floor_pressure;
+    // the floors as compared: the floor when it is set, -inf otherwise (no
+    // state compares below -inf), so that no flag is tested per cell
+    double density_floor_cmp, pressure_floor_cmp;
     int32_t viscosity_model, sink_model, one_body, has_mach_ceiling;
     int32_t has_density_floor, has_pressure_floor, density_index, cooling_slow_start;
     int32_t cooling_model, has_pdt;

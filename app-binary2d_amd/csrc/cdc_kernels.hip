@@ -264,7 +264,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile, d
         const double brate = pbr[q];
         const double cool = pom[q];
         double it[7][NV];
-        dev::cell_sources<NV>(A, uc, u0, G[TL::XC + ci], G[TL::YC + cj], vx, vy, id, p0, brate, cool, it);
+        dev::SinkGeom sg;
+        dev::cell_sources<NV>(A, A.orb, uc, u0, G[TL::XC + ci], G[TL::YC + cj], vx, vy, id, p0, brate, cool, it, sg);
         double out[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {

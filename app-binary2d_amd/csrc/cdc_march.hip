@@ -52,6 +52,16 @@ __device__ __forceinline__ double dpp(double old, double src) {
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
+// The same shift where the out-of-wave lane's value is never used: that lane
+// receives 0 and no old value needs to be kept in the destination register.
+template <int CTRL>
+__device__ __forceinline__ double dpp0(double src) {
+    const long long v = __double_as_longlong(src);
+    const int lo = __builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(v >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 // Stress factors of one cell: Ax (normal-normal component on an x-face), Ay
 // (on a y-face) and B (off-diagonal), each times Sigma and the cell-local
 // part of nu (src/physics.rs:322-340, src/scheme.rs:230-247).
@@ -201,7 +211,9 @@ struct MarchLds {
     static constexpr int HP = 0;                          // [2][H + 2][NV] halo-column primitives
     static constexpr int FB = 2 * (H + 2) * NV;           // [2][H][NV] strip-edge y-face fluxes
     static constexpr int ACC = FB + 2 * H * NV;           // [NS][64] per-lane sums of the itemized sources
-    static constexpr int END = ACC + SumMap<NV>::n * 64;
+    static constexpr int XC = ACC + SumMap<NV>::n * 64;   // [H] x of the strip's cell centres
+    static constexpr int XV = XC + H;                     // [H + 1] x of the strip's vertices
+    static constexpr int END = XV + H + 1;
     static constexpr int size = END > kFinLds ? END : kFinLds;
 };
 
@@ -211,6 +223,9 @@ struct MarchLds {
 template <int NV, int H, int COOL, int RB>
 __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, double *lds, int &s_key) {
     constexpr int NS = SumMap<NV>::n, NR = NS + 1;
+    constexpr int NSINK = 2 * NV;   // the sink items lead the strip sums
+    static_assert(SumMap<NV>::item(NSINK - 1) == CDC_ITEM_SINK2 && SumMap<NV>::item(NSINK) != CDC_ITEM_SINK2,
+                  "sink items first");
     using ML = MarchLds<NV, H>;
     const int lane = threadIdx.x;
     const int bs = A.bs, N = A.N;
@@ -235,6 +250,10 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
     auto grow = [&](int i) { const int g = gi0 + i; return g < 0 ? g + N : (g >= N ? g - N : g); };
     auto gcol = [&](int c) { const int g = gj0 + c; return g < 0 ? g + N : (g >= N ? g - N : g); };
     const FaceConsts fc(A);
+    // the orbit of the sources, positions and masses from the face constants
+    Orbit ob = A.orb;
+    ob.x1 = fc.x1; ob.y1 = fc.y1; ob.m1 = fc.m1;
+    ob.x2 = fc.x2; ob.y2 = fc.y2; ob.m2 = fc.m2;
     double *hP = lds + ML::HP;
     double *Fb = lds + ML::FB;
     double *Acc = lds + ML::ACC;
@@ -307,7 +326,11 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
     load_cell<NV>(ruc, ((unsigned)grow(ri0 + 1) * N + gj) * 8u, PB, Uz);
 #pragma unroll
     for (int s = 0; s < NS; ++s) Acc[s * 64 + lane] = 0.0;
-    __syncthreads();   // the edge data in LDS
+    // the rows' x coordinates, read by every step instead of evaluated
+    double *Xc = lds + ML::XC, *Xv = lds + ML::XV;
+    if (lane < H) Xc[lane] = axx.c(ri0 + lane);
+    if (lane <= H) Xv[lane] = axx.v(ri0 + lane);
+    __syncthreads();   // the edge data and coordinates in LDS
 
     double Pa[NV], Pb[NV], idb;
     double XLa[NV], axa = 0.0, ba = 0.0;
@@ -402,7 +425,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
 #pragma unroll
             for (int v = 0; v < NV; ++v) pr[v] = Pb[v] - gx[v] * 0.5;
             march_face<NV, 0>(fc, visc_on, alpha, XLa, pr, 0.5 * (Pa[1] + Pb[1]), 0.5 * (Pa[2] + Pb[2]), axa + axb,
-                              ba + bb, axx.v(z - 1), yc, Fxb);
+                              ba + bb, Xv[t], yc, Fxb);
         }
         // y-faces of row z - 1: lane l evaluates the face between columns
         // l and l + 1 from its neighbour's right-hand state; the strip's
@@ -414,15 +437,17 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             for (int v = 0; v < NV; ++v) {
                 yl[v] = Pb[v] + gy[v] * 0.5;
                 const double own = Pb[v] - gy[v] * 0.5;
-                yr[v] = dpp<kFromNext>(own, own);
+                yr[v] = dpp0<kFromNext>(own);
             }
-            const double ayn = dpp<kFromNext>(ayb, ayb), bn = dpp<kFromNext>(bb, bb);
+            // lane 63's face is replaced by the strip-edge face below, so
+            // what that lane receives from outside the wave is never used
+            const double ayn = dpp0<kFromNext>(ayb), bn = dpp0<kFromNext>(bb);
             double vbx = 0.0, vby = 0.0;
             if constexpr (NV == 4) {
-                vbx = 0.5 * (Pb[1] + dpp<kFromNext>(Pb[1], Pb[1]));
-                vby = 0.5 * (Pb[2] + dpp<kFromNext>(Pb[2], Pb[2]));
+                vbx = 0.5 * (Pb[1] + PR[1]);
+                vby = 0.5 * (Pb[2] + PR[2]);
             }
-            march_face<NV, 1>(fc, visc_on, alpha, yl, yr, vbx, vby, ayb + ayn, bb + bn, axx.c(z - 1), yv1, F);
+            march_face<NV, 1>(fc, visc_on, alpha, yl, yr, vbx, vby, ayb + ayn, bb + bn, Xc[t], yv1, F);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const double f = lane == 63 ? Fb[(H + t) * NV + v] : F[v];
@@ -452,15 +477,30 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             double p0 = 0.0;
             if constexpr (NV == 4) p0 = (Ua[3] - 0.5 * Ua[0] * (vx * vx + vy * vy)) * k.gm1;
             double it[7][NV];
-            dev::cell_sources<NV, COOL>(A, Ua, S0, axx.c(i), yc, vx, vy, id, p0, Sr, Sc, it);
+            dev::SinkGeom sg;
+            dev::cell_sources<NV, COOL, true>(A, ob, Ua, S0, Xc[t - 1], yc, vx, vy, id, p0, Sr, Sc, it, sg);
+            // the sources in the reference's order (sinks, gravity, buffer,
+            // cooling or fake mass); without a sink rate the sink items are
+            // zero and left out of the sum and the strip sums
+            constexpr int LAST = NV == 4 ? CDC_ITEM_COOLING : CDC_ITEM_FAKE_MASS;
+            double total[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                total[v] = it[CDC_ITEM_GRAV1][v] + it[CDC_ITEM_GRAV2][v] + it[CDC_ITEM_BUFFER][v] + it[LAST][v];
+            if (sg.sr1 != 0.0 || sg.sr2 != 0.0) {
+                dev::sink_items<NV>(k, ob, A.dt, Ua, vx, vy, id, p0, sg, it);
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                    total[v] = it[CDC_ITEM_SINK1][v] + it[CDC_ITEM_SINK2][v] + it[CDC_ITEM_GRAV1][v] +
+                               it[CDC_ITEM_GRAV2][v] + it[CDC_ITEM_BUFFER][v] + it[LAST][v];
+#pragma unroll
+                for (int s = 0; s < NSINK; ++s)
+                    atomicAdd(&Acc[s * 64 + lane], it[SumMap<NV>::item(s)][SumMap<NV>::comp(s)]);
+            }
             double out[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                double total = it[CDC_ITEM_SINK1][v] + it[CDC_ITEM_SINK2][v] + it[CDC_ITEM_GRAV1][v] +
-                               it[CDC_ITEM_GRAV2][v] + it[CDC_ITEM_BUFFER][v];
-                if constexpr (NV == 4) total = total + it[CDC_ITEM_COOLING][v];
-                else total = total + it[CDC_ITEM_FAKE_MASS][v];
-                const double u1 = Ua[v] + du[v] + total;
+                const double u1 = Ua[v] + du[v] + total[v];
                 if constexpr (RB != 0) out[v] = u1 * A.c1 + Sb[v] * A.c0;
                 else out[v] = u1;
             }
@@ -470,7 +510,8 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             for (int v = 0; v < NV; ++v) bstore_nt(out[v], ruo, g, (unsigned)v * PB);
             // into the lane's own LDS slots (no return value: no registers held)
 #pragma unroll
-            for (int s = 0; s < NS; ++s) atomicAdd(&Acc[s * 64 + lane], it[SumMap<NV>::item(s)][SumMap<NV>::comp(s)]);
+            for (int s = NSINK; s < NS; ++s)
+                atomicAdd(&Acc[s * 64 + lane], it[SumMap<NV>::item(s)][SumMap<NV>::comp(s)]);
             if (A.want_amax) amax = fmax(amax, dev::max_signal_speed<NV>(out, k.gm));
         }
         // rotate the rows

@@ -137,6 +137,8 @@ cdc::StageConsts make_consts(const cdc_solver *s, double time, double dt) {
     k.density_floor_pressure = (h.kind == CDC_EULER && h.has_density_floor) ? pow(h.density_floor, gm) : 0.0;
     k.has_pressure_floor = h.kind == CDC_EULER ? h.has_pressure_floor : 0;
     k.pressure_floor = h.pressure_floor;
+    k.density_floor_cmp = h.has_density_floor ? h.density_floor : -HUGE_VAL;
+    k.pressure_floor_cmp = k.has_pressure_floor ? h.pressure_floor : -HUGE_VAL;
     k.density_index = h.density_index;
     k.cooling_slow_start = h.cooling_slow_start;
     k.cooling_model = h.cooling_model;
