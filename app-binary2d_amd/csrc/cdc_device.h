@@ -284,6 +284,20 @@ struct SinkGeom {
     double ex1, ey1, ex2, ey2, q1, q2, sr1, sr2;
 };
 
+// The sink geometry of a cell at (x, y): its offsets from the bodies, their
+// squares' sums and the two sink kernel rates (1 - r^2/s^2)^2 inside the sink
+// radius.
+__device__ __forceinline__ SinkGeom sink_geom(const StageConsts &k, const Orbit &o, double x, double y) {
+    SinkGeom g;
+    g.ex1 = x - o.x1; g.ey1 = y - o.y1; g.ex2 = x - o.x2; g.ey2 = y - o.y2;
+    g.q1 = g.ex1 * g.ex1 + g.ey1 * g.ey1;
+    g.q2 = g.ex2 * g.ex2 + g.ey2 * g.ey2;
+    g.sr1 = 0.0; g.sr2 = 0.0;
+    if (g.q1 < k.sink_s2) { const double w = 1.0 - g.q1 * k.inv_sink_s2; g.sr1 = k.sink_rate * (w * w); }
+    if (g.q2 < k.sink_s2) { const double w = 1.0 - g.q2 * k.inv_sink_s2; g.sr2 = k.sink_rate * (w * w); }
+    return g;
+}
+
 // The two sink items (src/physics.rs:548-569, 717-744): impartial, or
 // torque-free / acceleration-free with the gas's velocity relative to the
 // body along the line of centres.  it[SINK1], it[SINK2] (times dt).
@@ -340,17 +354,12 @@ __device__ __forceinline__ void cell_sources(const StageArgs &A, const Orbit &o,
     const double sd = u[0];
 
     // Plummer gravity of each body: a = -m d / (r^2 + r_s^2)^(3/2)
-    const double ex1 = x - o.x1, ey1 = y - o.y1, ex2 = x - o.x2, ey2 = y - o.y2;
-    const double q1 = ex1 * ex1 + ey1 * ey1, q2 = ex2 * ex2 + ey2 * ey2;
+    g = sink_geom(k, o, x, y);
+    const double ex1 = g.ex1, ey1 = g.ey1, ex2 = g.ex2, ey2 = g.ey2, q1 = g.q1, q2 = g.q2;
     const double y1 = frsq(q1 + k.rs2), y2 = frsq(q2 + k.rs2);
     const double g1 = -o.m1 * (y1 * y1 * y1) * sd, g2 = -o.m2 * (y2 * y2 * y2) * sd;
     const double fx1 = g1 * ex1, fy1 = g1 * ey1, fx2 = g2 * ex2, fy2 = g2 * ey2;
 
-    // sink kernel rate (1 - r^2/s^2)^2 inside the sink radius
-    double sr1 = 0.0, sr2 = 0.0;
-    if (q1 < k.sink_s2) { const double w = 1.0 - q1 * k.inv_sink_s2; sr1 = k.sink_rate * (w * w); }
-    if (q2 < k.sink_s2) { const double w = 1.0 - q2 * k.inv_sink_s2; sr2 = k.sink_rate * (w * w); }
-    g.ex1 = ex1; g.ey1 = ey1; g.ex2 = ex2; g.ey2 = ey2; g.q1 = q1; g.q2 = q2; g.sr1 = sr1; g.sr2 = sr2;
     if constexpr (!LAZY_SINKS) sink_items<NV>(k, o, dt, u, vx, vy, id, p0, g, it);
     const double bfac = -dt * brate;
 #pragma unroll
