@@ -192,6 +192,7 @@ SIGNATURES = {
     "cdc_closure_sample_grad": (c_int32, [c_void_p, c_int32, POINTER(c_double), c_void_p]),
     "cdc_closure_train": (c_int32, [c_void_p, c_int32, c_double, c_uint64, c_void_p]),
     "cdc_closure_advance": (c_int32, [c_void_p, c_void_p, POINTER(c_double), POINTER(CError)]),
+    "cdc_closure_advance_fold": (c_int32, [c_void_p, c_void_p, c_int32, POINTER(c_double), POINTER(CError)]),
     "cdc_orbital_state_from_time": (c_int32, [POINTER(CPhysics), c_double, POINTER(CPointMass)]),
     "cdc_model_seeded_cavity": (c_int32, [POINTER(CMesh), POINTER(CHydro), POINTER(CCavityModel), c_void_p]),
 }

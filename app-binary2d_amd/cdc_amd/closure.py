@@ -107,11 +107,12 @@ class Closure:
         _check(self.lib, self.lib.cdc_closure_train(self.handle, int(steps), float(lr), int(seed), _ptr(out)))
         return out[:steps]
 
-    def advance(self, solver):
-        """One corrected coarse step; returns dt, raises HydroError."""
+    def advance(self, solver, fold=1):
+        """`fold` corrected coarse steps at one dt (one replayed graph, one
+        host synchronisation); returns dt, raises HydroError."""
         dt = c_double()
         err = abi.CError()
-        rc = self.lib.cdc_closure_advance(self.handle, solver.handle, byref(dt), byref(err))
+        rc = self.lib.cdc_closure_advance_fold(self.handle, solver.handle, int(fold), byref(dt), byref(err))
         if rc in (abi.CDC_ERR_NEGATIVE_DENSITY, abi.CDC_ERR_NEGATIVE_PRESSURE, abi.CDC_ERR_UNBOUND_ORBIT):
             raise HydroError(err)
         _check(self.lib, rc)
@@ -393,7 +394,7 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
 def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, sample_every=100,
                     max_samples=200, train_steps=4000, lr=1e-3, seed=5, limiter=0.1, train_mask=0.05,
                     pair_steps=1, variants=((0.1, 1.0, 0.05),), heldout_pairs=16, loop_rounds=0, loop_starts=8,
-                    loop_steps=200, loop_every=10, loop_train_steps=2000, burn_in=0, log=print):
+                    loop_steps=200, loop_every=10, loop_train_steps=2000, burn_in=0, learned_fold=10, log=print):
     """Train a closure on the fine workload (train_on_fine), measure it on the
     fine states that follow the training window (heldout_one_step, from a
     device snapshot of the fine run), then run plain fine, plain coarse and
@@ -437,13 +438,15 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
     coarse.resample_from(fine)
     out["window_orbits"] = [t_start / ORBIT, t_end / ORBIT]
     # plain runs step as the reference's driver does (fold = 10 steps per
-    # advance at one dt); the learned run corrects every step
+    # advance at one dt); the learned runs correct every step, learned_fold
+    # corrected steps per call (one replayed graph)
     runs = [("fine", fine, lambda s: 10 * s.advance(10), None),
             ("coarse", coarse, lambda s: 10 * s.advance(10), None)]
     for v in variants:
         kappa, damping, mask, mass_only = (tuple(v) + (0.0, 0.0))[:4]
         name = f"learned_k{kappa:g}_d{damping:g}_m{mask:g}" + ("_mass" if mass_only else "")
-        runs.append((name, None, lambda s: net.advance(s), (kappa, damping, mask, mass_only)))
+        runs.append((name, None, lambda s: learned_fold * net.advance(s, learned_fold),
+                     (kappa, damping, mask, mass_only)))
     sig_fine = None
     for name, solver, step, var in runs:
         if var is not None:
@@ -457,7 +460,7 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
         m0, ta, tq0 = accreted(solver), solver.time(), solver.torque().sum()
         try:
             n, wall = run_to(step, solver, t_end, log=lambda m: log(f"{name}:{m}"), every=1000,
-                             fold=10 if var is None else 1)
+                             fold=10 if var is None else learned_fold)
         except (HydroError, Stalled) as e:
             log(f"{name}: failed: {e}")
             out[name] = {"failed": str(e)}

@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "../../include/cdc_solver.h"
+#include "cdc_device.h"
 #include "cdc_host.h"
 #include "cdc_internal.h"
 
@@ -511,7 +512,7 @@ struct FeatArgs {
     double gm1, a0, a1, a2;
 };
 
-__global__ void __launch_bounds__(256) features_kernel(const FeatArgs a) {
+__device__ __forceinline__ void features_at(const FeatArgs &a) {
     const int64_t NN = (int64_t)a.N * a.N;
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (p >= NN) return;
@@ -530,6 +531,44 @@ __global__ void __launch_bounds__(256) features_kernel(const FeatArgs a) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) o[c] = f2bf(fminf(fmaxf(f[c], -16.0f), 16.0f));
     for (int c = 4; c < C; ++c) o[c] = 0;
+}
+
+__global__ void __launch_bounds__(256) features_kernel(const FeatArgs a) { features_at(a); }
+
+// The state pointers and dt of step f of a corrected advance, read from
+// device memory so that a captured graph serves every call.
+struct StepIO {
+    const double *u_in;   // the state the step starts from (the net's input)
+    double *u_out;        // the step's result, corrected in place
+    double dt;
+};
+
+// Per-workgroup maxima of the signal speed of the last step's corrected
+// result (grid-stride), for the next call's dt.
+template <int NV>
+__global__ void __launch_bounds__(256) max_speed_step_kernel(const StepIO *io, int f, int64_t ncell, double gm,
+                                                             double *partial) {
+    __shared__ double red[256];
+    const double *u = io[f].u_out;
+    double m = 0.0;
+    for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * 256) {
+        double w[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) w[v] = u[v * ncell + c];
+        m = fmax(m, cdc::dev::max_signal_speed<NV>(w, gm));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) features_step_kernel(FeatArgs a, const StepIO *io, int f) {
+    a.u = io[f].u_in;
+    features_at(a);
 }
 
 struct TargetArgs {
@@ -591,7 +630,7 @@ __device__ inline double signal_speed(const ApplyArgs &a, int64_t NN, int64_t q)
     return sqrt(vx * vx + vy * vy) + sqrt(fmax(c2, 0.0));
 }
 
-__global__ void __launch_bounds__(256) limit_kernel(const ApplyArgs a) {
+__device__ __forceinline__ void limit_at(const ApplyArgs &a) {
     const int64_t NN = (int64_t)a.N * a.N;
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (p >= NN) return;
@@ -621,7 +660,7 @@ __global__ void __launch_bounds__(256) limit_kernel(const ApplyArgs a) {
     a.lam[p * 2 + 1] = l[1];
 }
 
-__global__ void __launch_bounds__(256) apply_kernel(const ApplyArgs a) {
+__device__ __forceinline__ void apply_at(const ApplyArgs &a) {
     const int64_t NN = (int64_t)a.N * a.N;
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (p >= NN) return;
@@ -634,6 +673,21 @@ __global__ void __launch_bounds__(256) apply_kernel(const ApplyArgs a) {
         const double r = -a.s[v] * (fx * a.idx + fy * a.idy);
         a.u[v * NN + p] += a.dt * r;
     }
+}
+
+__global__ void __launch_bounds__(256) limit_kernel(const ApplyArgs a) { limit_at(a); }
+__global__ void __launch_bounds__(256) apply_kernel(const ApplyArgs a) { apply_at(a); }
+
+__global__ void __launch_bounds__(256) limit_step_kernel(ApplyArgs a, const StepIO *io, int f) {
+    a.u = io[f].u_out;
+    a.dt = io[f].dt;
+    limit_at(a);
+}
+
+__global__ void __launch_bounds__(256) apply_step_kernel(ApplyArgs a, const StepIO *io, int f) {
+    a.u = io[f].u_out;
+    a.dt = io[f].dt;
+    apply_at(a);
 }
 
 }  // namespace nn
@@ -661,6 +715,19 @@ struct cdc_closure {
     float *d_hist = nullptr;
     int hist_cap = 0;
     int64_t adam_step = 0;
+    // corrected advances of `fold` steps: per-step state pointers and dt
+    // (pinned staging and device copy), and the captured graphs, each valid
+    // for one solver object, its graph epoch, fold and tile shape
+    StepIO *h_io = nullptr, *d_io = nullptr;
+    int cap_io = 0;
+    struct FoldGraph {
+        uint64_t uid, epoch;
+        int fold, nstages;
+        cdc::TileShape ts;
+        hipGraphExec_t exec;
+    };
+    std::vector<FoldGraph> graphs;
+    bool use_graph = true;
 };
 
 namespace {
@@ -673,7 +740,17 @@ void free_closure(cdc_closure *c) {
     uint16_t *hl[] = {c->d_wf, c->d_wb, c->d_act[0], c->d_act[1], c->d_act[2], c->d_act[3], c->d_g[0], c->d_g[1],
                       c->d_feat};
     for (uint16_t *p : hl) (void)hipFree(p);
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+    (void)hipFree(c->d_io);
+    if (c->h_io) (void)hipHostFree(c->h_io);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+// The captured corrected advances hold the output scales by value.
+void drop_fold_graphs(cdc_closure *c) {
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
 }
 
 int refresh_weights(cdc_closure *c, hipStream_t s) {
@@ -986,6 +1063,10 @@ int32_t cdc_closure_set_scales(cdc_closure_t *c, const double *output_scale, con
         if (output_scale) c->cfg.output_scale[v] = output_scale[v];
         if (loss_weight) c->cfg.loss_weight[v] = loss_weight[v];
     }
+    if (output_scale) {
+        if (c->stream) HIP_TRY(hipStreamSynchronize(c->stream));
+        drop_fold_graphs(c);
+    }
     return CDC_OK;
 }
 
@@ -1109,25 +1190,17 @@ int32_t cdc_closure_train(cdc_closure_t *c, int32_t steps, double lr, uint64_t s
     return CDC_OK;
 }
 
-int32_t cdc_closure_advance(cdc_closure_t *c, cdc_solver_t *s, double *dt_out, cdc_error_t *err) {
-    if (err) memset(err, 0, sizeof(*err));
-    if (!c) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null closure");
-    int rc = check_solver(c, s);
-    if (rc) return rc;
-    if (s->nv < 3) return set_msg(CDC_ERR_INVALID_ARGUMENT, "unsupported hydro");
-    // net on the state at the start of the step, on the solver's stream
-    if ((rc = features(c, s, c->d_act[0], s->stream))) return rc;
-    if ((rc = forward(c, s->stream))) return rc;
-    double dt = 0.0;
-    rc = cdc_solver_advance(s, 1, &dt, err);
-    if (dt_out) *dt_out = dt;
-    if (rc) return rc;
+}  // extern "C"
+
+namespace {
+
+// The per-closure part of the limiter / correction arguments (the state
+// pointer and dt come per step).
+ApplyArgs apply_base(const cdc_closure *c, const cdc_solver *s) {
     ApplyArgs a{};
-    a.u = s->d_u[s->cur];
     a.y = c->d_y;
     a.lam = c->d_lam;
     a.N = s->N;
-    a.dt = dt;
     const double dx = 2.0 * s->mesh.domain_radius / (double)s->N;
     a.idx = 1.0 / dx;
     a.idy = 1.0 / dx;
@@ -1136,15 +1209,164 @@ int32_t cdc_closure_advance(cdc_closure_t *c, cdc_solver_t *s, double *dt_out, c
     a.nv = s->nv;
     a.gm = s->hydro.kind == CDC_EULER ? s->hydro.gamma_law_index : 1.0;
     for (int v = 0; v < 3; ++v) a.s[v] = c->cfg.output_scale[v];
-    const dim3 grid((unsigned)(((int64_t)s->N * s->N + 255) / 256));
-    hipLaunchKernelGGL(limit_kernel, grid, dim3(256), 0, s->stream, a);
+    return a;
+}
+
+// The device work of a corrected advance of `fold` steps of rk stages each
+// (arguments already staged): per step the net on its starting state, the
+// RK stages, the limiter and the correction of the result; then the max
+// signal speed of the final state into d_amax_adv and the read-back of it
+// and of the error flag.  Fused, each stage launch also finalizes the stage
+// before it; unfused (after an error), every stage is finalized before the
+// next starts, so that a failed stage skips the rest.
+int enqueue_fold(cdc_closure *c, cdc_solver *s, int fold, int rk, bool fuse) {
+    const size_t n = (size_t)fold * rk;
+    const unsigned grid = (unsigned)(((int64_t)s->N * s->N + 255) / 256);
+    HIP_TRY(cdc::launch_reset_advance(s->d_err_flag, s->d_amax_adv, s->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_io, c->h_io, (size_t)fold * sizeof(StepIO), hipMemcpyHostToDevice, s->stream));
+    int rc = cdc::host::upload_stages(s, n);
+    if (rc) return rc;
+    FeatArgs fa{};
+    fa.x = c->d_act[0];
+    fa.N = s->N;
+    fa.nv = s->nv;
+    fa.gm1 = s->hydro.kind == CDC_EULER ? s->hydro.gamma_law_index - 1.0 : 0.0;
+    fa.a0 = c->cfg.feature_scale[0];
+    fa.a1 = c->cfg.feature_scale[1];
+    fa.a2 = c->cfg.feature_scale[2];
+    const ApplyArgs ab = apply_base(c, s);
+    for (int f = 0; f < fold; ++f) {
+        hipLaunchKernelGGL(features_step_kernel, dim3(grid), dim3(256), 0, s->stream, fa, c->d_io, f);
+        HIP_TRY(hipGetLastError());
+        if ((rc = forward(c, s->stream))) return rc;
+        if ((rc = cdc::host::enqueue_stage_range(s, n, (size_t)f * rk, (size_t)(f + 1) * rk, fuse))) return rc;
+        hipLaunchKernelGGL(limit_step_kernel, dim3(grid), dim3(256), 0, s->stream, ab, c->d_io, f);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(apply_step_kernel, dim3(grid), dim3(256), 0, s->stream, ab, c->d_io, f);
+        HIP_TRY(hipGetLastError());
+    }
+    const double gm = s->hydro.kind == CDC_EULER ? s->hydro.gamma_law_index : 1.0;
+    if (s->nv == 3)
+        hipLaunchKernelGGL(max_speed_step_kernel<3>, dim3(s->namax), dim3(256), 0, s->stream, c->d_io, fold - 1,
+                           (int64_t)s->ncell(), gm, s->d_amax);
+    else
+        hipLaunchKernelGGL(max_speed_step_kernel<4>, dim3(s->namax), dim3(256), 0, s->stream, c->d_io, fold - 1,
+                           (int64_t)s->ncell(), gm, s->d_amax);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(apply_kernel, grid, dim3(256), 0, s->stream, a);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    s->amax_valid = false;   // the correction changed the state the last stage measured
-    s->version += 1;
+    HIP_TRY(cdc::launch_column_max(s->d_amax, s->namax, 1, s->d_amax_adv, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->h_flag, s->d_err_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->h_amax, s->d_amax_adv, sizeof(double), hipMemcpyDeviceToHost, s->stream));
     return CDC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t cdc_closure_advance_fold(cdc_closure_t *c, cdc_solver_t *s, int32_t fold, double *dt_out,
+                                 cdc_error_t *err) {
+    if (err) memset(err, 0, sizeof(*err));
+    if (!c) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null closure");
+    int rc = check_solver(c, s);
+    if (rc) return rc;
+    if (s->nv < 3) return set_msg(CDC_ERR_INVALID_ARGUMENT, "unsupported hydro");
+    if (fold < 0) return set_msg(CDC_ERR_INVALID_ARGUMENT, "fold must be >= 0");
+    // one dt for the call, as cdc_solver_advance (src/scheme.rs:336-340)
+    double amax = 0.0;
+    if ((rc = cdc::host::compute_amax(s, &amax))) return rc;
+    const double dt = cdc::host::step_dt(s, amax);
+    if (dt_out) *dt_out = dt;
+    if (fold == 0) return CDC_OK;
+
+    cdc::host::StepPlan plan;
+    cdc::host::plan_steps(s, fold, dt, plan);
+    const int rk = s->phys.rk_order;
+    const size_t n = plan.args.size();
+    for (auto &a : plan.args) {
+        a.want_amax = 0;   // the result's max speed is taken after its correction
+        a.amax_out = nullptr;
+        a.cool_factor = s->naux > 2 ? 1 : 0;
+    }
+    if ((rc = cdc::host::stage_args(s, plan.args.data(), n))) return rc;
+    if (fold > c->cap_io) {
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        drop_fold_graphs(c);
+        (void)hipFree(c->d_io);
+        if (c->h_io) (void)hipHostFree(c->h_io);
+        c->d_io = nullptr;
+        c->h_io = nullptr;
+        c->cap_io = 0;
+        HIP_TRY(hipHostMalloc((void **)&c->h_io, (size_t)fold * sizeof(StepIO), hipHostMallocDefault));
+        HIP_TRY(hipMalloc((void **)&c->d_io, (size_t)fold * sizeof(StepIO)));
+        c->cap_io = fold;
+    }
+    for (int f = 0; f < fold; ++f) {
+        c->h_io[f].u_in = plan.args[(size_t)f * rk].uc;
+        c->h_io[f].u_out = plan.args[(size_t)f * rk + rk - 1].uo;
+        c->h_io[f].dt = dt;
+    }
+
+    // replay the captured call, or capture it
+    bool done = false;
+    if (c->use_graph) {
+        for (const auto &g : c->graphs)
+            if (g.uid == s->uid && g.epoch == s->graph_epoch && g.fold == fold && g.nstages == (int)n &&
+                g.ts.tx == s->ts.tx &&
+                g.ts.ty == s->ts.ty && g.ts.nt == s->ts.nt) {
+                HIP_TRY(hipGraphLaunch(g.exec, s->stream));
+                done = true;
+                break;
+            }
+        if (!done) {
+            if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                (void)hipGetLastError();
+                c->use_graph = false;
+            } else {
+                rc = enqueue_fold(c, s, fold, rk, true);
+                hipGraph_t graph = nullptr;
+                const hipError_t ec = hipStreamEndCapture(s->stream, &graph);
+                hipGraphExec_t exec = nullptr;
+                if (!rc && ec == hipSuccess && graph &&
+                    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+                    (void)hipGraphDestroy(graph);
+                    if (c->graphs.size() >= 4) {
+                        (void)hipGraphExecDestroy(c->graphs.front().exec);
+                        c->graphs.erase(c->graphs.begin());
+                    }
+                    c->graphs.push_back({s->uid, s->graph_epoch, fold, (int)n, s->ts, exec});
+                    HIP_TRY(hipGraphLaunch(exec, s->stream));
+                    done = true;
+                } else {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    (void)hipGetLastError();
+                    if (rc) return rc;
+                    c->use_graph = false;
+                }
+            }
+        }
+    }
+    if (!done && (rc = enqueue_fold(c, s, fold, rk, true))) return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    int32_t flag = *s->h_flag;
+    if (flag != 0 && n > 1) {
+        // repeated unfused from the untouched committed state, so that the
+        // flag names the first failing stage (as cdc_solver_advance does)
+        if ((rc = enqueue_fold(c, s, fold, rk, false))) return rc;
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        flag = *s->h_flag;
+    }
+    if (flag != 0) return cdc::host::resolve_error(s, flag, plan, err);
+    cdc::host::commit(s, plan);
+    // the committed state is the corrected one: its max signal speed is the
+    // one taken after the correction, not the last stage's
+    s->amax_valid = false;
+    s->amax_host = *s->h_amax;
+    s->amax_host_version = s->version;
+    return CDC_OK;
+}
+
+int32_t cdc_closure_advance(cdc_closure_t *c, cdc_solver_t *s, double *dt_out, cdc_error_t *err) {
+    return cdc_closure_advance_fold(c, s, 1, dt_out, err);
 }
 
 }  // extern "C"

@@ -105,6 +105,11 @@ struct cdc_solver {
     };
     std::vector<AdvanceGraph> graphs;
     bool use_graph = true;
+    // identity of this solver object (never reused) and a count of
+    // drop_graphs calls: a graph captured elsewhere (the closure's corrected
+    // advance) over this solver's buffers is valid while both are unchanged
+    uint64_t uid = 0;
+    uint64_t graph_epoch = 0;
 
     bool timing = false;
     int64_t launches = 0;
@@ -131,6 +136,14 @@ int launch_stages(cdc_solver *s, const StageArgs *args, size_t n, const cdc::Til
 // Copies args[0..n) into the pinned staging, growing it (and dropping the
 // cached graphs, which hold its old addresses) when needed.
 int stage_args(cdc_solver *s, const StageArgs *args, size_t n);
+// Enqueues the upload of the n staged stage arguments and the batch
+// offsets, and with beta cooling the advance's relaxation factor (every stage
+// of an advance has one dt).
+int upload_stages(cdc_solver *s, size_t n);
+// Enqueues the stage launches [i0, i1) of n uploaded stages: with fuse, each
+// stage also finalizes the one before it (the last stage's finalize follows
+// when i1 == n); without, each stage is followed by its own finalize.
+int enqueue_stage_range(cdc_solver *s, size_t n, size_t i0, size_t i1, bool fuse);
 // Drops the cached advance graphs.
 void drop_graphs(cdc_solver *s);
 // Clears the run's error flag.

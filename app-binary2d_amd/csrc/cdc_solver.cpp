@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -238,6 +239,7 @@ int launch_set_int_checked(cdc_solver *s) {
 void drop_graphs(cdc_solver *s) {
     for (auto &g : s->graphs) (void)hipGraphExecDestroy(g.exec);
     s->graphs.clear();
+    s->graph_epoch += 1;
 }
 
 int stage_args(cdc_solver *s, const StageArgs *args, size_t n) {
@@ -267,25 +269,48 @@ int stage_args(cdc_solver *s, const StageArgs *args, size_t n) {
 // stage kernel (fused: also finalizing the stage before) and, unfused, its
 // finalize; fused, the last stage's finalize at the end.  Timing events are
 // bound to each stage dispatch.
-static int enqueue_stages(cdc_solver *s, size_t n, int ntiles, const cdc::TileShape &ts, hipEvent_t *ev,
-                          bool finalize, bool fuse) {
+int upload_stages(cdc_solver *s, size_t n) {
     HIP_TRY(hipMemcpyAsync(s->d_args, s->h_args, n * sizeof(StageArgs), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_begin, s->h_begin, 4 * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
     // every stage of an advance has the first one's dt
     if (s->h_args[0].cool_factor)
         HIP_TRY(cdc::launch_cooling_factor(s->d_args, s->d_aux, (int64_t)s->ncell(), s->stream));
+    return CDC_OK;
+}
+
+// the stage's cooling law when it is fixed for the launch
+static int stage_cooling(const cdc_solver *s, const StageArgs &h) {
+    return s->nv == 3 ? cdc::kCoolAny
+           : s->hydro.cooling_model == CDC_COOLING_BETA ? (h.cool_factor ? cdc::kCoolBetaFactor : cdc::kCoolBetaExp)
+                                                        : cdc::kCoolT4;
+}
+
+int enqueue_stage_range(cdc_solver *s, size_t n, size_t i0, size_t i1, bool fuse) {
+    const int ntiles = s->h_args[0].ntiles;
+    for (size_t i = i0; i < i1; ++i) {
+        const StageArgs *fin = fuse && i > 0 ? s->d_args + i - 1 : nullptr;
+        const StageArgs &h = s->h_args[i];
+        HIP_TRY(cdc::launch_stage_batch(s->nv, s->ts, s->d_args + i, s->d_begin, 1, ntiles, fin, s->d_begin + 2,
+                                        s->nblk, stage_cooling(s, h), h.read_base, nullptr, nullptr, s->stream));
+        if (!fuse) HIP_TRY(cdc::launch_finalize_batch(s->nv, s->d_args + i, s->d_begin + 2, 1, s->nblk, s->stream));
+    }
+    if (fuse && i1 == n && n > 0)
+        HIP_TRY(cdc::launch_finalize_batch(s->nv, s->d_args + n - 1, s->d_begin + 2, 1, s->nblk, s->stream));
+    return CDC_OK;
+}
+
+static int enqueue_stages(cdc_solver *s, size_t n, int ntiles, const cdc::TileShape &ts, hipEvent_t *ev,
+                          bool finalize, bool fuse) {
+    int rc = upload_stages(s, n);
+    if (rc) return rc;
     fuse = fuse && finalize;
     for (size_t i = 0; i < n; ++i) {
         // fused: this launch also finalizes stage i - 1 (its tile sums are in
         // the other parity buffer, its ist / oec inputs are complete)
         const StageArgs *fin = fuse && i > 0 ? s->d_args + i - 1 : nullptr;
         const StageArgs &h = s->h_args[i];
-        const int cool = s->nv == 3 ? cdc::kCoolAny
-                         : s->hydro.cooling_model == CDC_COOLING_BETA ? (h.cool_factor ? cdc::kCoolBetaFactor
-                                                                                        : cdc::kCoolBetaExp)
-                                                                      : cdc::kCoolT4;
         HIP_TRY(cdc::launch_stage_batch(s->nv, ts, s->d_args + i, s->d_begin, 1, ntiles, fin, s->d_begin + 2,
-                                        s->nblk, cool, h.read_base, ev ? ev[2 * i] : nullptr,
+                                        s->nblk, stage_cooling(s, h), h.read_base, ev ? ev[2 * i] : nullptr,
                                         ev ? ev[2 * i + 1] : nullptr, s->stream));
         if (finalize && !fuse)
             HIP_TRY(cdc::launch_finalize_batch(s->nv, s->d_args + i, s->d_begin + 2, 1, s->nblk, s->stream));
@@ -641,6 +666,8 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
     if (cdc_device_count() < 1) return set_msg(CDC_ERR_NO_DEVICE, "no HIP device visible");
 
     cdc_solver *s = new cdc_solver();
+    static std::atomic<uint64_t> next_uid{1};
+    s->uid = next_uid.fetch_add(1);
     s->mesh = *mesh;
     s->hydro = *hydro;
     s->phys = *physics;

@@ -238,7 +238,7 @@ def run_learned(args):
                         sample_every=args.sample_every, max_samples=args.max_samples,
                         train_steps=args.train_steps, pair_steps=args.pair_steps, variants=variants,
                         loop_rounds=args.loop_rounds, loop_steps=args.loop_steps, loop_every=args.loop_every,
-                        burn_in=args.burn_in, log=log)
+                        burn_in=args.burn_in, learned_fold=args.learned_fold, log=log)
     # the headline: the fastest learned variant within 2% accretion-rate error
     ok = [k for k, v in r.items() if k.startswith("learned") and isinstance(v, dict)
           and v.get("mdot_rel_err", 1.0) <= 0.02]
@@ -273,6 +273,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tile", default=None, help="TXxTY stage-kernel tile (skips the autotuner)")
     ap.add_argument("--train-orbits", type=float, default=2.0)
+    ap.add_argument("--learned-fold", type=int, default=10,
+                    help="corrected coarse steps per call of the learned runs")
     ap.add_argument("--eval-orbits", type=float, default=2.0)
     ap.add_argument("--train-steps", type=int, default=4000)
     ap.add_argument("--pair-steps", type=int, default=1, help="coarse steps per training pair")

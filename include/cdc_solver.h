@@ -416,6 +416,15 @@ int32_t cdc_closure_train(cdc_closure_t *closure, int32_t steps, double lr, uint
  * result is bitwise that of cdc_solver_advance. */
 int32_t cdc_closure_advance(cdc_closure_t *closure, cdc_solver_t *solver, double *dt_out,
                             cdc_error_t *err);
+/* `fold` corrected coarse steps at one dt (cdc_solver_advance(solver, fold)
+ * with the correction after every step, the net evaluated on each step's
+ * starting state), as one captured graph replayed per call and one host
+ * synchronisation; the next call's dt is the max signal speed of the
+ * corrected result.  Transactional and with the errors of
+ * cdc_solver_advance; with an all-zero net the result is bitwise that of
+ * cdc_solver_advance(solver, fold).  cdc_closure_advance is fold 1. */
+int32_t cdc_closure_advance_fold(cdc_closure_t *closure, cdc_solver_t *solver, int32_t fold,
+                                 double *dt_out, cdc_error_t *err);
 
 /* ---- host utilities (no GPU) -------------------------------------------- */
 

@@ -206,6 +206,77 @@ def test_zero_net_is_the_plain_run():
     assert sa["time"] == sb["time"]
 
 
+def to_blocks(ug, nb, bs):
+    nv = ug.shape[-1]
+    return ug.reshape(nb, bs, nb, bs, nv).transpose(0, 2, 1, 3, 4).reshape(nb * nb, bs, bs, nv)
+
+
+def test_zero_net_fold_is_the_plain_run():
+    """An all-zero net through the corrected advance of 3 steps per call,
+    twice (the first call captures its graph, the second replays it): bitwise
+    the plain cdc_solver_advance(3), and the next dt (the corrected result's
+    max signal speed) is the plain run's.  (Two calls of 4 steps at one dt
+    lose this configuration to a negative pressure, oracle and GPU alike.)"""
+    mesh, hydro, ph, u0 = setup()
+    N = mesh.num_blocks * mesh.block_size
+    net = Closure(N, mesh.domain_radius)
+    net.set_params(np.zeros(num_params(), dtype=np.float32))
+    a, b = Solver(mesh, hydro, ph, u0), Solver(mesh, hydro, ph, u0)
+    for _ in range(2):
+        assert net.advance(a, fold=3) == b.advance(3)
+    sa, sb = a.get_state(), b.get_state()
+    for key in ("conserved", "ist", "oec"):
+        np.testing.assert_array_equal(sa[key], sb[key])
+    assert sa["time"] == sb["time"] and sa["iteration"] == sb["iteration"]
+    np.testing.assert_array_equal(a.torque(), b.torque())
+    assert net.advance(a, fold=0) == b.advance(0)
+
+
+def test_fold_matches_host_composition():
+    """Two calls of 3 corrected steps each (capture, then replay), limiter
+    engaged, isothermal gas (the study's), against the same steps composed one at a time: the net's output
+    on each step's starting state (cdc_closure_forward), a plain step at the
+    call's dt, and the oracle's limited correction of its result, uploaded
+    again.  The corrections act (they move the state by far more than the
+    bar) and the states agree to 1e-3 of the largest correction applied (the
+    host's correction rounds differently, and a bf16 feature can then round
+    the other way in a later step)."""
+    mesh, hydro, ph = H.small("iso", nb=2, bs=16, R=4.0)
+    u0 = H.initial(mesh, hydro, ph)
+    nb, bs = mesh.num_blocks, mesh.block_size
+    N = nb * bs
+    kappa, sc = 0.1, [1e-2] * 3
+    net = Closure(N, mesh.domain_radius, limiter=kappa)
+    net.set_params(rand_params(8))
+    net.set_scales(output_scale=sc)
+    a, b = Solver(mesh, hydro, ph, u0), Solver(mesh, hydro, ph, u0)
+    plain = Solver(mesh, hydro, ph, u0)
+    dx = 2 * mesh.domain_radius / N
+    biggest = 0.0
+    for _ in range(2):
+        dt = net.advance(a, fold=3)
+        assert dt == pytest.approx(b.advance(0), rel=1e-12)
+        plain.advance_dt(dt, 3)
+        for _ in range(3):
+            Y = net.forward(b).astype(np.float64)
+            b.advance_dt(dt, 1)
+            st, tq = b.get_state(), b.torque()
+            ug = to_global(st["conserved"], nb, bs)
+            lam = C.limiter(Y, sc, dx, dx, dt, ug, kappa, 3)
+            step = dt * C.correction(Y, sc, dx, dx, lam)
+            biggest = max(biggest, float(np.max(np.abs(step))))
+            ug[..., :3] += step
+            b.set_state(to_blocks(ug, nb, bs), time=st["time"], iteration=st["iteration"], ist=st["ist"],
+                        oec=st["oec"], torque=tq)
+    ua, ub = a.get_state()["conserved"], b.get_state()["conserved"]
+    up = plain.get_state()["conserved"]
+    assert a.time() == b.time()
+    err = float(np.max(np.abs(ua - ub)))
+    moved = float(np.max(np.abs(ua - up)))
+    assert moved > 100 * err, (moved, err)
+    assert err <= 1e-3 * biggest, (err, biggest)
+
+
 def test_correction_conserves_mass_and_momentum():
     mesh, hydro, ph, u0 = setup()
     N = mesh.num_blocks * mesh.block_size
