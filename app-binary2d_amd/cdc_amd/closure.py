@@ -265,10 +265,121 @@ def loop_pairs(net, fl, snap, coarse_mesh, hydro, physics, cu0, steps=200, every
     return added, failed
 
 
+def _global(u, nb, bs):
+    """[nb*nb][bs][bs][nv] block layout -> [N][N][nv] (i-major)."""
+    nv = u.shape[-1]
+    return u.reshape(nb, nb, bs, bs, nv).transpose(0, 2, 1, 3, 4).reshape(nb * bs, nb * bs, nv)
+
+
+def _blocks(u, nb, bs):
+    nv = u.shape[-1]
+    return u.reshape(nb, bs, nb, bs, nv).transpose(0, 2, 1, 3, 4).reshape(nb * nb, bs, bs, nv)
+
+
+def flux_divergence(Y, scale, dx):
+    """The correction rate r_v = -s_v ((Y_v - Y_v[i-1]) / dx + (Y_3+v - Y_3+v[j-1]) / dx)
+    of the net output Y ([N][N][6], i-major), v = 0..2: what the training
+    loss compares with its target (no limiter, no mask)."""
+    r = np.empty(Y.shape[:2] + (3,))
+    for v in range(3):
+        r[..., v] = -scale[v] * ((Y[..., v] - np.roll(Y[..., v], 1, axis=0)) / dx +
+                                 (Y[..., 3 + v] - np.roll(Y[..., 3 + v], 1, axis=1)) / dx)
+    return r
+
+
+def _box(a, w):
+    """Periodic w x w box mean of a [N][N][...] field (w = 1: unchanged)."""
+    if w <= 1:
+        return a
+    out = np.zeros_like(a)
+    for di in range(w):
+        for dj in range(w):
+            out += np.roll(np.roll(a, di - w // 2, axis=0), dj - w // 2, axis=1)
+    return out / (w * w)
+
+
+def rollout_pairs(net, fl, snap, coarse_mesh, hydro, physics, cu0, scale, steps=200, every=20, min_fit=10,
+                  gain=1.0, smooth=1):
+    """Training pairs from a rollout (a posteriori) objective.  The fine
+    solver fl restarts from the device snapshot snap; the coarse state starts
+    as its down-sampling and takes `steps` corrected steps while fl advances
+    over the same times; e_j is the corrected run's error against fl
+    down-sampled after step j.  The rollout loss sum_j |e_j|^2 / 2 has the
+    gradient dt sum_{j>m} e_j with respect to the correction rate of step m
+    when the plain step's Jacobian is taken as the identity; the target of
+    the pair at step m (every `every` steps, with at least min_fit steps
+    after it) is that rate minus b_m, the least-squares growth rate of the
+    error over the steps after m (e_j - e_m against t_j - t_m), so that a
+    correction that leaves a steady error growth behind is moved by it.
+    `gain` scales b (a damped step) and `smooth` box-filters it over smooth x
+    smooth cells (only the error growth at that scale is targeted).
+    Returns (pairs added, mean relative L1 error of Sigma over the rollout,
+    1 if the corrected run failed before the end else 0)."""
+    from .solver import Solver
+    nb, bs = coarse_mesh.num_blocks, coarse_mesh.block_size
+    N = nb * bs
+    dx = 2.0 * coarse_mesh.domain_radius / N
+    c, base, plain, truth = (Solver(coarse_mesh, hydro, physics, cu0) for _ in range(4))
+    open_ = []          # per pending pair: [state dict, dt, t_m, e_m, S1, S2]
+    added, failed, l1 = 0, 0, []
+    try:
+        fl.restore(snap)
+        c.resample_from(fl)
+        e_prev = np.zeros((N, N, 3))
+        for k in range(steps):
+            if k % every == 0 and steps - k >= min_fit and net.samples()[0] + len(open_) < net.capacity:
+                open_.append([c.get_state(), None, c.time(), e_prev.copy(), np.zeros((N, N, 3)), 0.0])
+            try:
+                dtc = net.advance(c)
+            except HydroError:
+                failed = 1
+                break
+            for o in open_:
+                if o[1] is None:
+                    o[1] = dtc
+            m = max(1, math.ceil(dtc / fl.advance(0)))
+            fl.advance_dt(dtc / m, m)
+            truth.resample_from(fl)
+            ut = _global(truth.get_state()["conserved"], nb, bs)
+            uc = _global(c.get_state()["conserved"], nb, bs)
+            e = uc[..., :3] - ut[..., :3]
+            l1.append(float(np.abs(e[..., 0]).sum() / np.abs(ut[..., 0]).sum()))
+            tj = c.time()
+            for o in open_:
+                d = tj - o[2]
+                o[4] += d * (e - o[3])
+                o[5] += d * d
+            e_prev = e
+        for st, dt, _, _, S1, S2 in open_:
+            if dt is None or S2 <= 0.0:
+                continue
+            b = gain * _box(S1 / S2, smooth)
+            base.set_state(st["conserved"], time=st["time"], iteration=st["iteration"], ist=st["ist"],
+                           oec=st["oec"])
+            plain.set_state(st["conserved"], time=st["time"], iteration=st["iteration"], ist=st["ist"],
+                            oec=st["oec"])
+            try:
+                plain.advance_dt(dt, 1)
+            except HydroError:
+                continue
+            r = flux_divergence(net.forward(base).astype(np.float64), scale, dx)
+            up = _global(plain.get_state()["conserved"], nb, bs)
+            ut = up.copy()
+            ut[..., :3] += dt * (r - b)
+            truth.set_state(_blocks(ut, nb, bs))
+            net.add_sample(base, plain, truth, dt)
+            added += 1
+    finally:
+        for sv in (c, base, plain, truth):
+            sv.close()
+    return added, (float(np.mean(l1)) if l1 else float("nan")), failed
+
+
 def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, max_samples=200,
                   train_steps=4000, lr=1e-3, seed=5, limiter=0.1, train_mask=0.05, pair_steps=1,
                   loop_rounds=0, loop_starts=8, loop_steps=200, loop_every=10, loop_train_steps=2000, burn_in=0,
-                  log=print):
+                  rollout_rounds=0, rollout_steps=200, rollout_every=20, rollout_train_steps=2000, rollout_gain=0.1,
+                  rollout_smooth=8, log=print):
     """Train a closure for the coarse grid (coarse_blocks^2 blocks, 4x coarser
     than the fine workload) on pairs taken every sample_every fine steps along
     the fine run up to train_orbits: each pair compares pair_steps coarse steps
@@ -279,7 +390,12 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
     over the next coarse step (their tendencies at that state).  Then loop_rounds
     rounds of model-in-the-loop pairs (loop_pairs) from loop_starts fine
     states spread over the training window, each round followed by
-    loop_train_steps more training steps.  Returns a dict with
+    loop_train_steps more training steps.  Then rollout_rounds rounds of
+    rollout-objective pairs (rollout_pairs: rollout_steps corrected steps from
+    each of the loop_starts states, a pair every rollout_every steps), each
+    round adding to the samples (the one-step pairs stay) and followed by
+    rollout_train_steps training steps; rollout_gain and rollout_smooth damp
+    and smooth their targets.  Returns a dict with
     the fine solver at the end of the training window, the trained closure
     (limiter kappa and mask as given), the coarse mesh, hydro, physics, the
     coarse initial state, the trained parameters and output scale, and the
@@ -297,10 +413,12 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
     N = cm.num_blocks * cm.block_size
     dxc = 2.0 * cm.domain_radius / N
     per_round = loop_starts * (-(-loop_steps // loop_every))
-    net = Closure(N, cm.domain_radius, max_samples=max_samples + loop_rounds * per_round, seed=seed,
+    per_rollout = loop_starts * (-(-rollout_steps // rollout_every))
+    net = Closure(N, cm.domain_radius, max_samples=max_samples + loop_rounds * per_round +
+                  rollout_rounds * per_rollout, seed=seed,
                   init_output_gain=0.1, limiter=limiter, mask_density=train_mask)
     snaps = []
-    snap_dt = t_snap = train_orbits * ORBIT / loop_starts if loop_rounds > 0 else None
+    snap_dt = t_snap = train_orbits * ORBIT / loop_starts if loop_rounds > 0 or rollout_rounds > 0 else None
 
     def pair(base, plain, truth, fine):
         """Fill base, plain and truth for one training pair; returns its dt."""
@@ -366,7 +484,7 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
     hist = net.train(train_steps, lr=lr, seed=seed)
     ttrain = time.perf_counter() - tt0
     log(f"trained {train_steps} steps in {ttrain:.1f} s: loss {hist[:50].mean():.4f} -> {hist[-200:].mean():.4f}")
-    loop_info = []
+    loop_info, rollout_info = [], []
     if snaps:
         fl = Solver(fm, hydro, ph, u0)
         for rnd in range(loop_rounds):
@@ -380,6 +498,29 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
             loop_info.append({"pairs": added, "failed_runs": failed, "loss_last200": float(h2[-200:].mean()),
                               "s": time.perf_counter() - tl0})
             log(f"loop round {rnd}: {added} pairs ({failed} runs failed), loss {h2[-200:].mean():.4f}")
+        for rnd in range(rollout_rounds):
+            tl0 = time.perf_counter()
+            added, failed, l1 = 0, 0, []
+            for sn in snaps:
+                a, e, f = rollout_pairs(net, fl, sn, cm, hydro, ph, cu0, scale, steps=rollout_steps,
+                                        every=rollout_every, gain=rollout_gain, smooth=rollout_smooth)
+                added += a
+                failed += f
+                l1.append(e)
+            h2 = net.train(rollout_train_steps, lr=lr, seed=seed + 101 + rnd) if added else np.zeros(1)
+            rollout_info.append({"pairs": added, "failed_runs": failed, "rollout_l1_sigma": float(np.nanmean(l1)),
+                                 "loss_last200": float(h2[-200:].mean()), "s": time.perf_counter() - tl0})
+            log(f"rollout round {rnd}: rollout L1(Sigma) {np.nanmean(l1):.4e} before training, {added} pairs "
+                f"({failed} runs failed), loss {h2[-200:].mean():.4f}")
+        if rollout_rounds > 0:
+            # the rollout loss after the last round
+            l1 = []
+            for sn in snaps:
+                _, e, _ = rollout_pairs(net, fl, sn, cm, hydro, ph, cu0, scale, steps=rollout_steps,
+                                        every=rollout_steps, min_fit=rollout_steps + 1)
+                l1.append(e)
+            rollout_info.append({"rollout_l1_sigma": float(np.nanmean(l1))})
+            log(f"rollout L1(Sigma) after training {np.nanmean(l1):.4e}")
         fl.close()
         for sn in snaps:
             sn.close()
@@ -388,13 +529,15 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
             "info": {"fine_grid": fm.num_blocks * fm.block_size, "coarse_grid": N, "samples": nsamp,
                      "train_steps": train_steps, "loss_first50": float(hist[:50].mean()),
                      "loss_last200": float(hist[-200:].mean()), "pairs_s": tpairs, "train_s": ttrain,
-                     "skipped_pairs": skipped, "loop_rounds": loop_info}}
+                     "skipped_pairs": skipped, "loop_rounds": loop_info, "rollout_rounds": rollout_info}}
 
 
 def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, sample_every=100,
                     max_samples=200, train_steps=4000, lr=1e-3, seed=5, limiter=0.1, train_mask=0.05,
                     pair_steps=1, variants=((0.1, 1.0, 0.05),), heldout_pairs=16, loop_rounds=0, loop_starts=8,
-                    loop_steps=200, loop_every=10, loop_train_steps=2000, burn_in=0, learned_fold=10, log=print):
+                    loop_steps=200, loop_every=10, loop_train_steps=2000, burn_in=0, learned_fold=10,
+                    rollout_rounds=0, rollout_steps=200, rollout_every=20, rollout_train_steps=2000,
+                    rollout_gain=0.1, rollout_smooth=8, log=print):
     """Train a closure on the fine workload (train_on_fine), measure it on the
     fine states that follow the training window (heldout_one_step, from a
     device snapshot of the fine run), then run plain fine, plain coarse and
@@ -414,7 +557,10 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
                       max_samples=max_samples, train_steps=train_steps, lr=lr, seed=seed, limiter=limiter,
                       train_mask=train_mask, pair_steps=pair_steps, loop_rounds=loop_rounds,
                       loop_starts=loop_starts, loop_steps=loop_steps, loop_every=loop_every,
-                      loop_train_steps=loop_train_steps, burn_in=burn_in, log=log)
+                      loop_train_steps=loop_train_steps, burn_in=burn_in, rollout_rounds=rollout_rounds,
+                      rollout_steps=rollout_steps, rollout_every=rollout_every,
+                      rollout_train_steps=rollout_train_steps, rollout_gain=rollout_gain,
+                      rollout_smooth=rollout_smooth, log=log)
     fine, net, cm, hydro, ph, cu0 = t["fine"], t["net"], t["coarse_mesh"], t["hydro"], t["physics"], t["cu0"]
     fm = fine_w["mesh"]
     N = cm.num_blocks * cm.block_size

@@ -510,6 +510,9 @@ struct FeatArgs {
     uint16_t *x;         // [N*N][32]
     int N, nv;
     double gm1, a0, a1, a2;
+    // isothermal: the bodies at the state's time, the softening length
+    // squared and the grid's geometry (cell i, j centred at -R + (i + 1/2) dx)
+    double x1, y1, m1, x2, y2, m2, rs2, R, dx;
 };
 
 __device__ __forceinline__ void features_at(const FeatArgs &a) {
@@ -526,6 +529,13 @@ __device__ __forceinline__ void features_at(const FeatArgs &a) {
     if (a.nv == 4) {
         const double pr = (a.u[3 * NN + p] - 0.5 * sig * (vx * vx + vy * vy)) * a.gm1;
         f[3] = (float)(a.a2 * log(fmax(pr / sig, 1e-30)));
+    } else {
+        // ln(-phi), phi the softened two-body potential at the cell centre
+        const int i = (int)(p / a.N), j = (int)(p - (p / a.N) * a.N);
+        const double x = -a.R + (i + 0.5) * a.dx, y = -a.R + (j + 0.5) * a.dx;
+        const double d1x = x - a.x1, d1y = y - a.y1, d2x = x - a.x2, d2y = y - a.y2;
+        const double mphi = a.m1 / sqrt(d1x * d1x + d1y * d1y + a.rs2) + a.m2 / sqrt(d2x * d2x + d2y * d2y + a.rs2);
+        f[3] = (float)(a.a2 * log(fmax(mphi, 1e-30)));
     }
     uint16_t *o = a.x + p * C;
 #pragma unroll
@@ -541,6 +551,7 @@ struct StepIO {
     const double *u_in;   // the state the step starts from (the net's input)
     double *u_out;        // the step's result, corrected in place
     double dt;
+    double x1, y1, m1, x2, y2, m2;   // the bodies at the step's start
 };
 
 // Per-workgroup maxima of the signal speed of the last step's corrected
@@ -567,7 +578,10 @@ __global__ void __launch_bounds__(256) max_speed_step_kernel(const StepIO *io, i
 }
 
 __global__ void __launch_bounds__(256) features_step_kernel(FeatArgs a, const StepIO *io, int f) {
-    a.u = io[f].u_in;
+    const StepIO &o = io[f];
+    a.u = o.u_in;
+    a.x1 = o.x1; a.y1 = o.y1; a.m1 = o.m1;
+    a.x2 = o.x2; a.y2 = o.y2; a.m2 = o.m2;
     features_at(a);
 }
 
@@ -925,6 +939,15 @@ int64_t logical_params() {
     return n;
 }
 
+// the softening and grid geometry of the isothermal potential feature
+void feature_geometry(const cdc_solver *s, FeatArgs &a) {
+    const cdc_physics_t &p = s->phys;
+    const double rs = p.has_softening_length ? p.softening_length : p.sink_radius;   // src/physics.rs:399-405
+    a.rs2 = rs * rs;
+    a.R = s->mesh.domain_radius;
+    a.dx = 2.0 * s->mesh.domain_radius / (double)s->N;
+}
+
 int features(cdc_closure *c, const cdc_solver *s, uint16_t *x, hipStream_t st) {
     FeatArgs a{};
     a.u = s->d_u[s->cur];
@@ -935,6 +958,12 @@ int features(cdc_closure *c, const cdc_solver *s, uint16_t *x, hipStream_t st) {
     a.a0 = c->cfg.feature_scale[0];
     a.a1 = c->cfg.feature_scale[1];
     a.a2 = c->cfg.feature_scale[2];
+    feature_geometry(s, a);
+    double el[4];
+    cdc::orbital_elements(s->phys, el);
+    const cdc::Orbit o = cdc::orbital_state(el, s->time);
+    a.x1 = o.x1; a.y1 = o.y1; a.m1 = o.m1;
+    a.x2 = o.x2; a.y2 = o.y2; a.m2 = o.m2;
     hipLaunchKernelGGL(features_kernel, dim3((unsigned)(((int64_t)s->N * s->N + 255) / 256)), dim3(256), 0, st, a);
     HIP_TRY(hipGetLastError());
     return CDC_OK;
@@ -1234,6 +1263,7 @@ int enqueue_fold(cdc_closure *c, cdc_solver *s, int fold, int rk, bool fuse) {
     fa.a0 = c->cfg.feature_scale[0];
     fa.a1 = c->cfg.feature_scale[1];
     fa.a2 = c->cfg.feature_scale[2];
+    feature_geometry(s, fa);
     const ApplyArgs ab = apply_base(c, s);
     for (int f = 0; f < fold; ++f) {
         hipLaunchKernelGGL(features_step_kernel, dim3(grid), dim3(256), 0, s->stream, fa, c->d_io, f);
@@ -1304,6 +1334,9 @@ int32_t cdc_closure_advance_fold(cdc_closure_t *c, cdc_solver_t *s, int32_t fold
         c->h_io[f].u_in = plan.args[(size_t)f * rk].uc;
         c->h_io[f].u_out = plan.args[(size_t)f * rk + rk - 1].uo;
         c->h_io[f].dt = dt;
+        const cdc::Orbit &o = plan.args[(size_t)f * rk].orb;
+        c->h_io[f].x1 = o.x1; c->h_io[f].y1 = o.y1; c->h_io[f].m1 = o.m1;
+        c->h_io[f].x2 = o.x2; c->h_io[f].y2 = o.y2; c->h_io[f].m2 = o.m2;
     }
 
     // replay the captured call, or capture it

@@ -238,7 +238,10 @@ def run_learned(args):
                         sample_every=args.sample_every, max_samples=args.max_samples,
                         train_steps=args.train_steps, pair_steps=args.pair_steps, variants=variants,
                         loop_rounds=args.loop_rounds, loop_steps=args.loop_steps, loop_every=args.loop_every,
-                        burn_in=args.burn_in, learned_fold=args.learned_fold, log=log)
+                        burn_in=args.burn_in, learned_fold=args.learned_fold, rollout_rounds=args.rollout_rounds,
+                        rollout_steps=args.rollout_steps, rollout_every=args.rollout_every,
+                        rollout_train_steps=args.rollout_train_steps, rollout_gain=args.rollout_gain,
+                        rollout_smooth=args.rollout_smooth, log=log)
     # the headline: the fastest learned variant within 2% accretion-rate error
     ok = [k for k, v in r.items() if k.startswith("learned") and isinstance(v, dict)
           and v.get("mdot_rel_err", 1.0) <= 0.02]
@@ -273,6 +276,13 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tile", default=None, help="TXxTY stage-kernel tile (skips the autotuner)")
     ap.add_argument("--train-orbits", type=float, default=2.0)
+    ap.add_argument("--rollout-rounds", type=int, default=3,
+                    help="rounds of rollout-objective training pairs after the one-step training")
+    ap.add_argument("--rollout-steps", type=int, default=200, help="corrected coarse steps per rollout")
+    ap.add_argument("--rollout-every", type=int, default=20, help="rollout steps between training pairs")
+    ap.add_argument("--rollout-train-steps", type=int, default=2000, help="training steps per rollout round")
+    ap.add_argument("--rollout-gain", type=float, default=0.1, help="step on the fitted rollout error growth")
+    ap.add_argument("--rollout-smooth", type=int, default=8, help="box filter (cells) of that growth")
     ap.add_argument("--learned-fold", type=int, default=10,
                     help="corrected coarse steps per call of the learned runs")
     ap.add_argument("--eval-orbits", type=float, default=2.0)
@@ -285,7 +295,7 @@ def main():
     ap.add_argument("--loop-every", type=int, default=10, help="coarse steps between loop training pairs")
     ap.add_argument("--burn-in", type=int, default=0,
                     help="coarse steps beside the fine run before each pair (tendency targets)")
-    ap.add_argument("--variants", default="0.1:1:0.05,0.05:1:0.05,0.02:0.5:0.05",
+    ap.add_argument("--variants", default="0.02:0.25:0.05,0.02:0.5:0.05,0.05:0.5:0.05",
                     help="learned runs as kappa:damping:mask_density[:mass_only],...")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -362,7 +362,9 @@ int32_t cdc_ensemble_get_stats(cdc_ensemble_t *ensemble, int64_t *member_steps,
  * corrections scaled down so that a face moves at most kappa min(Sigma_L,
  * Sigma_R) of mass, and that times the larger signal speed |v| + c_s of the
  * two cells of momentum, per step.  Features: a0 ln Sigma,
- * a1 vx, a1 vy, a2 ln(p / Sigma) (0 for isothermal), each clamped to [-16, 16].  Training fits r to
+ * a1 vx, a1 vy, and a2 ln(p / Sigma) (Euler) or a2 ln(-phi) (isothermal: phi the
+ * softened two-body potential at the cell centre, -R + (i + 1/2) dx, at the
+ * state's time), each clamped to [-16, 16].  Training fits r to
  * (u_truth - u_plain) / dt over stored samples with Adam on the loss
  * (1/P) sum_cells c sum_v w_v (r_v - target_v)^2, c = 1 where the base density
  * is at least mask_density (the cells the correction is applied to), else 0.

@@ -59,10 +59,13 @@ def pack(layers):
     return np.concatenate([np.concatenate([W.ravel(), b.ravel()]) for W, b in layers])
 
 
-def features(u_global, nv, gamma, scales):
+def features(u_global, nv, gamma, scales, bodies=None, R=None, rs2=None):
     """u_global: [N][N][nv] conserved.  Channels: a0 ln Sigma, a1 vx, a1 vy,
-    a2 ln(p / Sigma) (Euler; 0 for isothermal).  Arguments of the logs are
-    floored at 1e-30 and every feature is clamped to [-16, 16]."""
+    and a2 ln(p / Sigma) (Euler) or a2 ln(-phi) (isothermal, 0 without
+    `bodies`): phi = -sum_k m_k / sqrt(|x - x_k|^2 + rs2) at the cell centres
+    x = -R + (i + 1/2) dx, bodies = ((m1, x1, y1), (m2, x2, y2)).  Arguments
+    of the logs are floored at 1e-30 and every feature is clamped to
+    [-16, 16]."""
     a0, a1, a2 = scales[0], scales[1], scales[2]
     sig = u_global[..., 0]
     vx = u_global[..., 1] / sig
@@ -74,6 +77,13 @@ def features(u_global, nv, gamma, scales):
     if nv == 4:
         p = (u_global[..., 3] - 0.5 * sig * (vx * vx + vy * vy)) * (gamma - 1.0)
         f[..., 3] = a2 * np.log(np.maximum(p / sig, 1e-30))
+    elif bodies is not None:
+        N = sig.shape[0]
+        dx = 2.0 * R / N
+        c = -R + (np.arange(N) + 0.5) * dx
+        x, y = np.meshgrid(c, c, indexing="ij")
+        mphi = sum(m / np.sqrt((x - bx) ** 2 + (y - by) ** 2 + rs2) for m, bx, by in bodies)
+        f[..., 3] = a2 * np.log(np.maximum(mphi, 1e-30))
     return np.clip(f, -16.0, 16.0)
 
 

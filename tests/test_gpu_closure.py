@@ -12,6 +12,7 @@ import pytest
 from cdc_amd import Solver, abi
 from cdc_amd.closure import Closure, num_params
 from oracle import closure_oracle as C
+from oracle import oracle as O
 from tests import helpers as H
 
 pytestmark = pytest.mark.gpu
@@ -381,7 +382,12 @@ def test_forward_matches_oracle_512():
     Y = net.forward(g)
     x0 = np.zeros((N, N, 4))
     x0[:] = net.features(g)
-    want_x = C.bf16(C.features(to_global(u0, mesh.num_blocks, mesh.block_size), 3, 0.0, (0.25, 0.5, 0.5)))
+    # the potential channel from the oracle's own orbit at the state's time
+    st = O.orbital_state(ph.to_c(), g.time())
+    bodies = [(st[k, 0], st[k, 1], st[k, 2]) for k in range(2)]
+    want_x = C.bf16(C.features(to_global(u0, mesh.num_blocks, mesh.block_size), 3, 0.0, (0.25, 0.5, 0.5),
+                               bodies=bodies, R=mesh.domain_radius, rs2=ph.softening() ** 2))
+    assert np.ptp(want_x[..., 3]) > 1.0
     np.testing.assert_allclose(x0, want_x, rtol=2.0 ** -8, atol=1e-30)
     want = C.forward(oracle_layers(p), x0)[-1]
     err = np.max(np.abs(Y - want)) / np.max(np.abs(want))
