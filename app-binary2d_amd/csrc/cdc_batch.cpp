@@ -249,6 +249,11 @@ int32_t cdc_ensemble_advance(cdc_ensemble_t *e, const int32_t *active, double *d
         for (int k = 0; k < e->rk; ++k) e->h_args[(size_t)k * nrun + r] = plans[(size_t)r].args[(size_t)k];
     }
     const int total_tiles = tile_begin[(size_t)nrun], total_blocks = block_begin[(size_t)nrun];
+    // marching members read the strip-edge columns of their committed state
+    for (int r = 0; r < nrun; ++r) {
+        const int rc = cdc::host::ensure_edges(e->members[(size_t)runs[r]], e->stream);
+        if (rc) return rc;
+    }
 
     // (re)build the graph when the active set changed
     const bool same = e->graph_runs == runs && e->exec;

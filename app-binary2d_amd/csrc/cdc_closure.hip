@@ -552,7 +552,16 @@ struct StepIO {
     double *u_out;        // the step's result, corrected in place
     double dt;
     double x1, y1, m1, x2, y2, m2;   // the bodies at the step's start
+    double *e_out;                   // the strip-edge columns of u_out (null: not marching)
 };
+
+// The strip-edge columns of step f's corrected result (the next step's
+// column-marching stage reads them).
+__global__ void __launch_bounds__(256) edge_step_kernel(const StepIO *io, int f, int N, int nv) {
+    const int64_t total = (int64_t)nv * N * (N / 64) * 4;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e < total) io[f].e_out[e] = cdc::edge_entry(io[f].u_out, e, N);
+}
 
 // Per-workgroup maxima of the signal speed of the last step's corrected
 // result (grid-stride), for the next call's dt.
@@ -1274,6 +1283,12 @@ int enqueue_fold(cdc_closure *c, cdc_solver *s, int fold, int rk, bool fuse) {
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(apply_step_kernel, dim3(grid), dim3(256), 0, s->stream, ab, c->d_io, f);
         HIP_TRY(hipGetLastError());
+        if (cdc::host::marching(s)) {
+            const int64_t ne = (int64_t)s->nv * s->N * (s->N / 64) * 4;
+            hipLaunchKernelGGL(edge_step_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s->stream, c->d_io,
+                               f, s->N, s->nv);
+            HIP_TRY(hipGetLastError());
+        }
     }
     const double gm = s->hydro.kind == CDC_EULER ? s->hydro.gamma_law_index : 1.0;
     if (s->nv == 3)
@@ -1318,6 +1333,7 @@ int32_t cdc_closure_advance_fold(cdc_closure_t *c, cdc_solver_t *s, int32_t fold
         a.cool_factor = s->naux > 2 ? 1 : 0;
     }
     if ((rc = cdc::host::stage_args(s, plan.args.data(), n))) return rc;
+    if ((rc = cdc::host::ensure_edges(s, s->stream))) return rc;
     if (fold > c->cap_io) {
         HIP_TRY(hipStreamSynchronize(s->stream));
         drop_fold_graphs(c);
@@ -1337,6 +1353,7 @@ int32_t cdc_closure_advance_fold(cdc_closure_t *c, cdc_solver_t *s, int32_t fold
         const cdc::Orbit &o = plan.args[(size_t)f * rk].orb;
         c->h_io[f].x1 = o.x1; c->h_io[f].y1 = o.y1; c->h_io[f].m1 = o.m1;
         c->h_io[f].x2 = o.x2; c->h_io[f].y2 = o.y2; c->h_io[f].m2 = o.m2;
+        c->h_io[f].e_out = plan.args[(size_t)f * rk + rk - 1].eo;
     }
 
     // replay the captured call, or capture it

@@ -416,6 +416,19 @@ __device__ __forceinline__ void cell_sources(const StageArgs &A, const Orbit &o,
 
 }  // namespace dev
 
+// Entry e of the strip-edge columns of state u (StageArgs::ec layout
+// [nv][N][N / 64][4]: per plane and row, the columns 64 k - 2 .. 64 k + 1 of
+// boundary k, periodic).
+__device__ __forceinline__ double edge_entry(const double *u, int64_t e, int N) {
+    const int NK = N / 64;
+    const int c = (int)(e & 3);
+    const int64_t rest = e >> 2;
+    const int k = (int)(rest % NK);
+    int col = 64 * k - 2 + c;
+    col = col < 0 ? col + N : (col >= N ? col - N : col);
+    return u[(rest / NK) * N + col];
+}
+
 // ---------------------------------------------------------------------------
 // Launch geometry and the per-block finalize shared by the stage kernels
 

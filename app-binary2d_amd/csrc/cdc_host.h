@@ -110,6 +110,11 @@ struct cdc_solver {
     // advance) over this solver's buffers is valid while both are unchanged
     uint64_t uid = 0;
     uint64_t graph_epoch = 0;
+    // strip-edge columns of each state buffer (block sizes that are multiples
+    // of 64 only), and the committed-state version whose d_edge[cur] is
+    // current
+    double *d_edge[cdc::host::NBUF] = {};
+    uint64_t edge_version = UINT64_MAX;
 
     bool timing = false;
     int64_t launches = 0;
@@ -156,6 +161,12 @@ double step_dt(const cdc_solver *s, double amax);
 void plan_steps(const cdc_solver *s, int fold, double dt, StepPlan &plan);
 // Makes the plan's result the committed state.
 void commit(cdc_solver *s, const StepPlan &plan);
+// Whether the stages launch the column-marching kernel (which reads and
+// writes the strip-edge arrays).
+bool marching(const cdc_solver *s);
+// Before stages that march: enqueues, on stream st, the strip-edge columns
+// of the committed state into d_edge[cur] unless they are current.
+int ensure_edges(cdc_solver *s, hipStream_t st);
 // After a failed advance (err_flag = flag): the error join_all would report.
 int resolve_error(cdc_solver *s, int flag, const StepPlan &plan, cdc_error_t *err);
 

@@ -72,6 +72,13 @@ struct StageArgs {
     // when set (the last stage of an advance), the finalize also folds the
     // blocks' max signal speeds into *amax_out (zeroed beforehand)
     double *amax_out;
+    // the strip-edge columns of uc and uo (column-marching kernel; null when
+    // the block size is not a multiple of 64): per plane, row and boundary
+    // k between 64-column strips (global column 64 k), the four columns
+    // 64 k - 2 .. 64 k + 1 (periodic), [nv][N][N / 64][4]; the kernel reads
+    // ec and writes eo
+    const double *ec;
+    double *eo;
     int32_t N, nb, bs, ntiles, tpb;  // tpb: tiles per block
     int32_t stage_id, read_base, want_amax, reuse_prim;
     int32_t cool_factor;   // beta cooling reads exp(-dt Omega / beta) from aux plane 2
@@ -93,6 +100,8 @@ struct TileShape {
     int tx, ty, nt;   // tile of tx x ty cells, nt threads
 };
 TileShape tile_shape_for(int bs, int nv);
+// the shapes of the column-marching kernel: 64-column strips, one wave each
+inline bool march_shape(const TileShape &ts) { return ts.ty == 64 && ts.nt == 64; }
 int tile_shapes(const TileShape **out);
 
 int nsum_for(int nv);
@@ -117,6 +126,9 @@ hipError_t launch_march_batch(int nv, int rows, int cool, int read_base, const S
                               const int32_t *begin, int nruns, int ntiles, const StageArgs *fin,
                               const int32_t *fin_begin, int fin_nruns, int nblk, hipEvent_t ev0, hipEvent_t ev1,
                               hipStream_t s);
+// The strip-edge columns of the state u (SoA [nv][N][N]) into E (the layout
+// of StageArgs::ec), N a multiple of 64.
+hipError_t launch_edge_build(int nv, int N, const double *u, double *E, hipStream_t s);
 hipError_t launch_finalize_batch(int nv, const StageArgs *args, const int32_t *block_begin, int nruns,
                                  int nblocks, hipStream_t s);
 hipError_t launch_batch_reset(const StageArgs *args, int nruns, hipStream_t s);

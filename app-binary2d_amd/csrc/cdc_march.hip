@@ -213,7 +213,8 @@ struct MarchLds {
     static constexpr int ACC = FB + 2 * H * NV;           // [NS][64] per-lane sums of the itemized sources
     static constexpr int XC = ACC + SumMap<NV>::n * 64;   // [H] x of the strip's cell centres
     static constexpr int XV = XC + H;                     // [H + 1] x of the strip's vertices
-    static constexpr int END = XV + H + 1;
+    static constexpr int EO = XV + H + 1;                 // [4][H][NV] the result's edge columns 0, 1, 62, 63
+    static constexpr int END = EO + 4 * H * NV;
     static constexpr int size = END > kFinLds ? END : kFinLds;
 };
 
@@ -248,7 +249,6 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
     const int gi0 = bi * bs, gj0 = bj * bs;
     // wrapped global row / column of a block-local index
     auto grow = [&](int i) { const int g = gi0 + i; return g < 0 ? g + N : (g >= N ? g - N : g); };
-    auto gcol = [&](int c) { const int g = gj0 + c; return g < 0 ? g + N : (g >= N ? g - N : g); };
     const FaceConsts fc(A);
     // the orbit of the sources, positions and masses from the face constants
     Orbit ob = A.orb;
@@ -257,6 +257,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
     double *hP = lds + ML::HP;
     double *Fb = lds + ML::FB;
     double *Acc = lds + ML::ACC;
+    double *Eo = lds + ML::EO;
 
     int key = INT_MAX, kcode = 0;
     double kval = 0.0;
@@ -265,16 +266,25 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
     // ---- the strip's edges: y-faces at cj0 - 1/2 (side 0) and cj0 + 64 - 1/2
     // (side 1) for rows ri0 .. ri0 + H - 1, and the halo columns' primitives
     // for rows ri0 - 1 .. ri0 + H --------------------------------------------
+    // The cells come from the input's strip-edge columns (StageArgs::ec): at
+    // the boundary k = kL (side 0) or kL + 1 (side 1) the four columns
+    // cL - 1, cL, cR, cR + 1 of a row are 32 contiguous bytes there, against
+    // two 128-byte lines of the state per row and plane.
+    const int NK = N / 64;
+    const int kL = (gj0 + cj0) / 64, kR = kL + 1 == NK ? 0 : kL + 1;
+    const unsigned PE = __builtin_amdgcn_readfirstlane((unsigned)NK * (unsigned)N * 32u);   // edge plane stride
+    const Rsrc rec = rsrc(A.ec, NV * PE);
     for (int q = lane; q < 2 * H; q += 64) {
         const int s = q / H, r = ri0 + (q - s * H);
-        const int cL = s == 0 ? cj0 - 1 : cj0 + 63, cR = cL + 1;
-        const unsigned rm = (unsigned)grow(r - 1) * N, r0 = (unsigned)grow(r) * N, rp = (unsigned)grow(r + 1) * N;
-        const unsigned cLL = gcol(cL - 1), cLg = gcol(cL), cRg = gcol(cR), cRR = gcol(cR + 1);
+        const unsigned kk = s == 0 ? kL : kR;
+        const int cR = s == 0 ? cj0 : cj0 + 64;   // block-local column right of the face
+        const unsigned rm = ((unsigned)grow(r - 1) * NK + kk) * 4u, r0 = ((unsigned)grow(r) * NK + kk) * 4u,
+                       rp = ((unsigned)grow(r + 1) * NK + kk) * 4u;
         // cells: 0 (r, cL-1), 1 (r, cL), 2 (r, cR), 3 (r, cR+1), 4 (r-1, cL), 5 (r+1, cL), 6 (r-1, cR), 7 (r+1, cR)
-        const unsigned g8[8] = {r0 + cLL, r0 + cLg, r0 + cRg, r0 + cRR, rm + cLg, rp + cLg, rm + cRg, rp + cRg};
+        const unsigned g8[8] = {r0, r0 + 1u, r0 + 2u, r0 + 3u, rm + 1u, rp + 1u, rm + 2u, rp + 2u};
         double u[8][NV];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) load_cell<NV>(ruc, g8[c] * 8u, PB, u[c]);
+        for (int c = 0; c < 8; ++c) load_cell<NV>(rec, g8[c] * 8u, PE, u[c]);
         double p[8][NV], id[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
@@ -511,6 +521,12 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
                 if constexpr (RB != 0) out[v] = u1 * A.c1 + Sb[v] * A.c0;
                 else out[v] = u1;
             }
+            // the result's edge columns, for the next stage's strip edges
+            if (lane < 2 || lane >= 62) {
+                const int slot = lane < 2 ? lane : lane - 60;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) Eo[(slot * H + (t - 1)) * NV + v] = out[v];
+            }
             // the next update's operands, then this row's stores
             if constexpr (next) load_update_operands(i + 1);
 #pragma unroll
@@ -554,6 +570,17 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) amax = fmax(amax, __shfl_xor(amax, m, 64));
     __syncthreads();
+    // the result's edge columns: per plane and row, columns 0, 1 at boundary
+    // kL (entries 2, 3) and 62, 63 at kR (entries 0, 1), 32 contiguous bytes
+    // unless kR wraps to 0
+    if (A.eo) {
+        for (int e = lane; e < 4 * H * NV; e += 64) {
+            const int j = e & 3, row = (e >> 2) % H, v = (e >> 2) / H;
+            const size_t at = ((size_t)v * N + (size_t)(gi0 + ri0 + row)) * NK;
+            const size_t o = j < 2 ? (at + kL) * 4 + 2 + j : (at + kR) * 4 + (j - 2);
+            dev::gptr(A.eo)[o] = Eo[(j * H + row) * NV + v];
+        }
+    }
     if (lane < NS) {
         const double *q = Acc + lane * 64;
         double x = q[0];
@@ -623,6 +650,20 @@ hipError_t launch_march_t(const StageArgs *args, const int32_t *begin, int nruns
 }
 
 }  // namespace
+
+// The strip-edge columns of a state (see StageArgs::ec), one thread per entry.
+__global__ void __launch_bounds__(256) edge_build_kernel(const double *__restrict__ u, double *__restrict__ E, int N,
+                                                         int nv) {
+    const int64_t total = (int64_t)nv * N * (N / 64) * 4;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e < total) E[e] = edge_entry(u, e, N);
+}
+
+hipError_t launch_edge_build(int nv, int N, const double *u, double *E, hipStream_t s) {
+    const int64_t total = (int64_t)nv * N * (N / 64) * 4;
+    hipLaunchKernelGGL(edge_build_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, u, E, N, nv);
+    return hipGetLastError();
+}
 
 hipError_t launch_march_batch(int nv, int rows, int cool, int read_base, const StageArgs *args,
                               const int32_t *begin, int nruns, int ntiles, const StageArgs *fin,

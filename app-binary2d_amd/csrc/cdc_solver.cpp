@@ -158,6 +158,7 @@ void free_all(cdc_solver *s) {
         (void)hipFree(s->d_ist[i]);
         (void)hipFree(s->d_oec[i]);
         (void)hipFree(s->d_tq[i]);
+        (void)hipFree(s->d_edge[i]);
     }
     (void)hipFree(s->d_tile_sums);
     (void)hipFree(s->d_tile_err);
@@ -465,6 +466,8 @@ void plan_steps(const cdc_solver *s, int fold, double dt, StepPlan &plan) {
         a.tq_b = s->d_tq[base.idx];
         a.tq_o = s->d_tq[out.idx];
         a.perr = s->d_perr;
+        a.ec = s->d_edge[in.idx];
+        a.eo = s->d_edge[out.idx];
         a.reuse_prim = s->hydro.has_density_floor ? 0 : 1;
         a.N = s->N;
         a.nb = s->nb;
@@ -530,6 +533,17 @@ void commit(cdc_solver *s, const StepPlan &plan) {
     s->den = out.den;
     s->amax_valid = true;
     s->version += 1;
+    // a marching stage wrote the result's strip-edge columns
+    if (marching(s)) s->edge_version = s->version;
+}
+
+bool marching(const cdc_solver *s) { return cdc::march_shape(s->ts) && s->d_edge[0] != nullptr; }
+
+int ensure_edges(cdc_solver *s, hipStream_t st) {
+    if (!marching(s) || s->edge_version == s->version) return CDC_OK;
+    HIP_TRY(cdc::launch_edge_build(s->nv, s->N, s->d_u[s->cur], s->d_edge[s->cur], st));
+    s->edge_version = s->version;
+    return CDC_OK;
 }
 
 int resolve_error(cdc_solver *s, int flag, const StepPlan &plan, cdc_error_t *err) {
@@ -713,6 +727,7 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
         ALLOC(s->d_ist[i], (size_t)s->nblk * 7 * s->nv * sizeof(double));
         ALLOC(s->d_oec[i], (size_t)s->nblk * 7 * 4 * sizeof(double));
         ALLOC(s->d_tq[i], (size_t)s->nblk * 2 * sizeof(double));
+        if (s->bs % 64 == 0) ALLOC(s->d_edge[i], (size_t)s->nv * s->N * (s->N / 64) * 4 * sizeof(double));
     }
     s->ts_stride = (size_t)s->max_tiles * (s->nsum + 1);
     ALLOC(s->d_tile_sums, 2 * s->ts_stride * sizeof(double));
@@ -867,6 +882,11 @@ int32_t cdc_solver_autotune(cdc_solver_t *s, int32_t reps, int32_t *tx, int32_t 
     std::vector<hipEvent_t> marks(2 * ((size_t)reps + 1), nullptr);
     for (auto &m : marks) HIP_TRY(hipEventCreate(&m));
     int rc = CDC_OK;
+    // the marching shapes read the committed state's strip-edge columns
+    if (s->d_edge[0] && s->edge_version != s->version) {
+        HIP_TRY(cdc::launch_edge_build(s->nv, s->N, s->d_u[s->cur], s->d_edge[s->cur], s->stream));
+        s->edge_version = s->version;
+    }
     for (int i = 0; i < n && rc == CDC_OK; ++i) {
         const cdc::TileShape t = shapes[i];
         if (s->bs % t.tx != 0 || s->bs % t.ty != 0) continue;
@@ -879,6 +899,7 @@ int32_t cdc_solver_autotune(cdc_solver_t *s, int32_t reps, int32_t *tx, int32_t 
         a.ub = s->d_u[s->cur];
         a.aux = s->d_aux;
         a.uo = s->d_u[scratch];
+        a.ec = s->d_edge[s->cur];   // null when no shape marches; eo stays null (not written)
         // the tile-sum buffer that does not hold the committed max speed
         a.tile_sums = s->ts_last == s->d_tile_sums ? s->d_tile_sums + s->ts_stride : s->d_tile_sums;
         a.tile_err = s->d_tile_err;
@@ -1104,6 +1125,7 @@ static int advance_at(cdc_solver_t *s, int32_t fold, double dt, cdc_error_t *err
     }
     rc = cdc::host::stage_args(s, plan.args.data(), nst);
     if (rc) return rc;
+    if ((rc = cdc::host::ensure_edges(s, s->stream))) return rc;
     rc = cdc::host::enqueue_advance(s, nst, plan.args[0].ntiles, s->timing ? s->ev.data() : nullptr);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(s->stream));
