@@ -30,6 +30,7 @@ struct cdc_ensemble {
     std::vector<cdc_solver *> members;
     int nv = 0, rk = 0;
     int cool = cdc::kCoolAny;   // the members' shared cooling law (StageCooling)
+    int visc = -1;              // the viscosity law the captured step launches with (StageViscosity)
     cdc::TileShape ts{};
     hipStream_t stream = nullptr;
     bool use_graph = true;
@@ -111,7 +112,7 @@ int enqueue_step(cdc_ensemble *e, int nrun, int total_tiles, int total_blocks) {
     for (int k = 0; k < e->rk; ++k) {
         const cdc::StageArgs *a = e->d_args + (size_t)k * nrun;
         HIP_TRY(cdc::launch_stage_batch(e->nv, e->ts, a, e->d_tile_begin, nrun, total_tiles, nullptr, nullptr, 0,
-                                        e->cool, k > 0 ? 1 : 0, nullptr, nullptr, e->stream));
+                                        e->cool, e->visc, k > 0 ? 1 : 0, nullptr, nullptr, e->stream));
         HIP_TRY(cdc::launch_finalize_batch(e->nv, a, e->d_block_begin, nrun, total_blocks, e->stream));
     }
     HIP_TRY(cdc::launch_batch_tail(e->nv, e->d_args + (size_t)(e->rk - 1) * nrun, nrun, e->d_amax, e->d_flags,
@@ -255,7 +256,18 @@ int32_t cdc_ensemble_advance(cdc_ensemble_t *e, const int32_t *active, double *d
         if (rc) return rc;
     }
 
-    // (re)build the graph when the active set changed
+    // the active members' shared viscosity law (their physics may change
+    // between steps), and the graph rebuilt when it or the active set changed
+    int visc = -1;
+    for (int r = 0; r < nrun; ++r) {
+        const int v = e->members[(size_t)runs[r]]->phys.viscosity_model == CDC_VISC_ALPHA ? cdc::kViscAlpha
+                                                                                           : cdc::kViscConst;
+        visc = visc < 0 ? v : (visc == v ? v : (int)cdc::kViscAny);
+    }
+    if (visc != e->visc) {
+        e->visc = visc;
+        e->graph_runs.clear();
+    }
     const bool same = e->graph_runs == runs && e->exec;
     if (!same) {
         if (e->exec) (void)hipGraphExecDestroy(e->exec);

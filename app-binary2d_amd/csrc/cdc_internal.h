@@ -94,6 +94,9 @@ struct StageArgs {
 // (cool_factor), beta cooling evaluating it; kCoolAny decides per run at run
 // time (isothermal gas has no cooling).
 enum StageCooling { kCoolAny = 0, kCoolT4 = 1, kCoolBetaFactor = 2, kCoolBetaExp = 3 };
+// The viscosity law every run of a stage launch shares, for the column-
+// marching kernel (constant nu includes nu = 0); kViscAny decides per run.
+enum StageViscosity { kViscAny = 0, kViscConst = 1, kViscAlpha = 2 };
 
 // Tile shape chosen for a block size (the largest shape that divides it).
 struct TileShape {
@@ -114,15 +117,16 @@ hipError_t launch_aux(const AuxArgs &a, hipStream_t s);
 // (the earlier stage's per-run arguments), the launch also finalizes that
 // stage over the nblk blocks that block_begin divides among the runs, in
 // workgroups ahead of the tiles.  ev0 / ev1 (either may be null) take the
-// launch's start and end.  `cool` (StageCooling) and `read_base` (whether
-// the stages read the step base, the same for every run) let the launch pick
-// a kernel compiled for them; the tiled kernel decides both at run time.
+// launch's start and end.  `cool` (StageCooling), `visc` (StageViscosity)
+// and `read_base` (whether the stages read the step base, the same for every
+// run) let the launch pick a kernel compiled for them; the tiled kernel
+// decides them at run time.
 hipError_t launch_stage_batch(int nv, const TileShape &ts, const StageArgs *args, const int32_t *tile_begin,
                               int nruns, int ntiles, const StageArgs *fin, const int32_t *block_begin, int nblk,
-                              int cool, int read_base, hipEvent_t ev0, hipEvent_t ev1, hipStream_t s);
+                              int cool, int visc, int read_base, hipEvent_t ev0, hipEvent_t ev1, hipStream_t s);
 // The column-marching stage kernel (cdc_march.hip): strips of 64 columns x
 // `rows` rows (32 or 16), one wave each; otherwise as launch_stage_batch.
-hipError_t launch_march_batch(int nv, int rows, int cool, int read_base, const StageArgs *args,
+hipError_t launch_march_batch(int nv, int rows, int cool, int visc, int read_base, const StageArgs *args,
                               const int32_t *begin, int nruns, int ntiles, const StageArgs *fin,
                               const int32_t *fin_begin, int fin_nruns, int nblk, hipEvent_t ev0, hipEvent_t ev1,
                               hipStream_t s);

@@ -587,10 +587,11 @@ static hipError_t launch_batch_t(const StageArgs *args, const int32_t *begin, in
 template <int NV>
 static hipError_t launch_batch_nv(const TileShape &ts, const StageArgs *args, const int32_t *begin, int nruns,
                                   int ntiles, const StageArgs *fin, const int32_t *fin_begin, int fin_nruns,
-                                  int nblk, int cool, int read_base, hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
-    if (ts.ty == 64 && ts.nt == 64)
-        return launch_march_batch(NV, ts.tx, cool, read_base, args, begin, nruns, ntiles, fin, fin_begin, fin_nruns,
-                                  nblk, e0, e1, s);
+                                  int nblk, int cool, int visc, int read_base, hipEvent_t e0, hipEvent_t e1,
+                                  hipStream_t s) {
+    if (march_shape(ts))
+        return launch_march_batch(NV, ts.tx, cool, visc, read_base, args, begin, nruns, ntiles, fin, fin_begin,
+                                  fin_nruns, nblk, e0, e1, s);
 #define CDC_SHAPE(X, Y, T)                                                                        \
     if (ts.tx == X && ts.ty == Y && ts.nt == T)                                                   \
         return launch_batch_t<NV, X, Y, T>(args, begin, nruns, ntiles, fin, fin_begin, fin_nruns, nblk, e0, e1, s);
@@ -608,11 +609,11 @@ static hipError_t launch_batch_nv(const TileShape &ts, const StageArgs *args, co
 
 hipError_t launch_stage_batch(int nv, const TileShape &ts, const StageArgs *args, const int32_t *tile_begin,
                               int nruns, int ntiles, const StageArgs *fin, const int32_t *block_begin, int nblk,
-                              int cool, int read_base, hipEvent_t ev0, hipEvent_t ev1, hipStream_t s) {
+                              int cool, int visc, int read_base, hipEvent_t ev0, hipEvent_t ev1, hipStream_t s) {
     return nv == 3 ? launch_batch_nv<3>(ts, args, tile_begin, nruns, ntiles, fin, block_begin, nruns, nblk, cool,
-                                        read_base, ev0, ev1, s)
+                                        visc, read_base, ev0, ev1, s)
                    : launch_batch_nv<4>(ts, args, tile_begin, nruns, ntiles, fin, block_begin, nruns, nblk, cool,
-                                        read_base, ev0, ev1, s);
+                                        visc, read_base, ev0, ev1, s);
 }
 
 hipError_t launch_finalize_batch(int nv, const StageArgs *args, const int32_t *block_begin, int nruns,

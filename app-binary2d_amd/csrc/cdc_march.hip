@@ -221,7 +221,7 @@ struct MarchLds {
 // One strip of one RK stage (see the file comment), tile = strip index in
 // the run: strips are numbered block by block, row segments of H rows, then
 // 64-column strips within a segment.
-template <int NV, int H, int COOL, int RB>
+template <int NV, int H, int COOL, int RB, int VISC>
 __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, double *lds, int &s_key) {
     constexpr int NS = SumMap<NV>::n, NR = NS + 1;
     constexpr int NSINK = 2 * NV;   // the sink items lead the strip sums
@@ -243,8 +243,10 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
     const StageConsts &k = A.k;
     const dev::Axis axx = dev::block_axis(A.R, A.L, bi, bs);
     const dev::Axis axy = dev::block_axis(A.R, A.L, bj, bs);
-    const bool alpha = k.viscosity_model == CDC_VISC_ALPHA;
-    const bool visc_on = alpha || k.visc != 0.0;
+    // the viscosity law: fixed at compile time (constant nu, nu = 0 included:
+    // its stress terms are then zero) or read from the run
+    const bool alpha = VISC == kViscAlpha || (VISC == kViscAny && k.viscosity_model == CDC_VISC_ALPHA);
+    const bool visc_on = VISC != kViscAny || alpha || k.visc != 0.0;
     const bool beta = k.cooling_model == CDC_COOLING_BETA;
     const int gi0 = bi * bs, gj0 = bj * bs;
     // wrapped global row / column of a block-local index
@@ -609,7 +611,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
 // per strip; with fin, nfin workgroups after the strips finalize the earlier
 // stage, one block each (see stage_kernel_batch).  Two waves per SIMD: the march
 // keeps three rows of state and the next row's operands in registers.
-template <int NV, int H, int COOL, int RB>
+template <int NV, int H, int COOL, int RB, int VISC>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 march_kernel_batch(const StageArgs *__restrict__ args, const int32_t *__restrict__ begin, int nruns,
                    const StageArgs *__restrict__ fin, const int32_t *__restrict__ fin_begin, int fin_nruns,
@@ -632,19 +634,21 @@ march_kernel_batch(const StageArgs *__restrict__ args, const int32_t *__restrict
     const int r = run_of(begin, nruns, t);
     const StageArgs &A = args[r];
     if (stage_skipped(A)) return;
-    march_strip<NV, H, COOL, RB>(A, t - begin[r], lds, s_key);
+    march_strip<NV, H, COOL, RB, VISC>(A, t - begin[r], lds, s_key);
 }
 
-template <int NV, int H, int COOL, int RB>
+template <int NV, int H, int COOL, int RB, int VISC>
 hipError_t launch_march_t(const StageArgs *args, const int32_t *begin, int nruns, int ntiles, const StageArgs *fin,
                           const int32_t *fin_begin, int fin_nruns, int nblk, hipEvent_t ev0, hipEvent_t ev1,
                           hipStream_t s) {
     const int nfin = fin ? nblk : 0;   // one block per workgroup
     if (ev0 || ev1)
-        hipExtLaunchKernelGGL((march_kernel_batch<NV, H, COOL, RB>), dim3(ntiles + nfin), dim3(64), 0, s, ev0, ev1, 0, args,
+        hipExtLaunchKernelGGL((march_kernel_batch<NV, H, COOL, RB, VISC>), dim3(ntiles + nfin), dim3(64), 0, s, ev0, ev1, 0,
+                              args,
                               begin, nruns, fin, fin_begin, fin_nruns, nfin);
     else
-        hipLaunchKernelGGL((march_kernel_batch<NV, H, COOL, RB>), dim3(ntiles + nfin), dim3(64), 0, s, args, begin, nruns, fin,
+        hipLaunchKernelGGL((march_kernel_batch<NV, H, COOL, RB, VISC>), dim3(ntiles + nfin), dim3(64), 0, s, args, begin,
+                           nruns, fin,
                            fin_begin, fin_nruns, nfin);
     return hipGetLastError();
 }
@@ -665,15 +669,18 @@ hipError_t launch_edge_build(int nv, int N, const double *u, double *E, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_march_batch(int nv, int rows, int cool, int read_base, const StageArgs *args,
+hipError_t launch_march_batch(int nv, int rows, int cool, int visc, int read_base, const StageArgs *args,
                               const int32_t *begin, int nruns, int ntiles, const StageArgs *fin,
                               const int32_t *fin_begin, int fin_nruns, int nblk, hipEvent_t ev0, hipEvent_t ev1,
                               hipStream_t s) {
     const int c = nv == 3 ? (int)kCoolAny : cool;
     const int rb = read_base ? 1 : 0;
-#define CDC_MARCH(V, R, C, B)                                                                                   \
-    if (nv == V && rows == R && c == C && rb == B)                                                              \
-        return launch_march_t<V, R, C, B>(args, begin, nruns, ntiles, fin, fin_begin, fin_nruns, nblk, ev0, ev1, s);
+#define CDC_MARCH_V(V, R, C, B, W)                                                                            \
+    if (nv == V && rows == R && c == C && rb == B && visc == W)                                               \
+        return launch_march_t<V, R, C, B, W>(args, begin, nruns, ntiles, fin, fin_begin, fin_nruns, nblk, ev0, ev1, \
+                                             s);
+#define CDC_MARCH(V, R, C, B) \
+    CDC_MARCH_V(V, R, C, B, kViscAny) CDC_MARCH_V(V, R, C, B, kViscConst) CDC_MARCH_V(V, R, C, B, kViscAlpha)
 #define CDC_MARCH_RB(V, R, C) CDC_MARCH(V, R, C, 0) CDC_MARCH(V, R, C, 1)
 #define CDC_MARCH_ROWS(V, C) CDC_MARCH_RB(V, 32, C) CDC_MARCH_RB(V, 16, C)
     CDC_MARCH_ROWS(3, kCoolAny)
@@ -684,6 +691,7 @@ hipError_t launch_march_batch(int nv, int rows, int cool, int read_base, const S
 #undef CDC_MARCH_ROWS
 #undef CDC_MARCH_RB
 #undef CDC_MARCH
+#undef CDC_MARCH_V
     return hipErrorInvalidValue;
 }
 

@@ -279,6 +279,12 @@ int upload_stages(cdc_solver *s, size_t n) {
     return CDC_OK;
 }
 
+// the run's viscosity law (its physics may change between calls; every
+// change drops the cached graphs)
+static int stage_viscosity(const cdc_solver *s) {
+    return s->phys.viscosity_model == CDC_VISC_ALPHA ? cdc::kViscAlpha : cdc::kViscConst;
+}
+
 // the stage's cooling law when it is fixed for the launch
 static int stage_cooling(const cdc_solver *s, const StageArgs &h) {
     return s->nv == 3 ? cdc::kCoolAny
@@ -292,7 +298,8 @@ int enqueue_stage_range(cdc_solver *s, size_t n, size_t i0, size_t i1, bool fuse
         const StageArgs *fin = fuse && i > 0 ? s->d_args + i - 1 : nullptr;
         const StageArgs &h = s->h_args[i];
         HIP_TRY(cdc::launch_stage_batch(s->nv, s->ts, s->d_args + i, s->d_begin, 1, ntiles, fin, s->d_begin + 2,
-                                        s->nblk, stage_cooling(s, h), h.read_base, nullptr, nullptr, s->stream));
+                                        s->nblk, stage_cooling(s, h), stage_viscosity(s), h.read_base, nullptr,
+                                        nullptr, s->stream));
         if (!fuse) HIP_TRY(cdc::launch_finalize_batch(s->nv, s->d_args + i, s->d_begin + 2, 1, s->nblk, s->stream));
     }
     if (fuse && i1 == n && n > 0)
@@ -311,7 +318,7 @@ static int enqueue_stages(cdc_solver *s, size_t n, int ntiles, const cdc::TileSh
         const StageArgs *fin = fuse && i > 0 ? s->d_args + i - 1 : nullptr;
         const StageArgs &h = s->h_args[i];
         HIP_TRY(cdc::launch_stage_batch(s->nv, ts, s->d_args + i, s->d_begin, 1, ntiles, fin, s->d_begin + 2,
-                                        s->nblk, stage_cooling(s, h), h.read_base, ev ? ev[2 * i] : nullptr,
+                                        s->nblk, stage_cooling(s, h), stage_viscosity(s), h.read_base, ev ? ev[2 * i] : nullptr,
                                         ev ? ev[2 * i + 1] : nullptr, s->stream));
         if (finalize && !fuse)
             HIP_TRY(cdc::launch_finalize_batch(s->nv, s->d_args + i, s->d_begin + 2, 1, s->nblk, s->stream));
@@ -780,6 +787,11 @@ int32_t cdc_solver_set_physics(cdc_solver_t *s, const cdc_physics_t *physics) {
     if (!s) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null solver");
     int rc = validate(&s->mesh, &s->hydro, physics);
     if (rc) return rc;
+    if (physics->viscosity_model != s->phys.viscosity_model) {
+        // the captured advances launch a kernel compiled for the old law
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        cdc::host::drop_graphs(s);
+    }
     s->phys = *physics;
     return compute_aux(s);
 }
