@@ -5,9 +5,10 @@
 // Every member is an ordinary cdc_solver: the batch reads and writes the
 // members' own device buffers, plans each member's stages with the same host
 // code as cdc_solver_advance(member, 1), and launches each RK stage once for
-// all members (stage_kernel_batch / finalize_kernel_batch, whose per-run
-// arithmetic is the single-run kernels').  A member therefore ends bitwise
-// where advancing it alone would have left it.  The whole step -- argument
+// all members that share a cooling and viscosity law (the stage kernels are
+// compiled for the law; finalize_kernel_batch once for all members), with
+// the per-run arithmetic of the single-run launches: a member therefore ends
+// bitwise where advancing it alone would have left it.  The whole step -- argument
 // upload, flag reset, the stages, the per-member max signal speed and error
 // flag read-back -- is captured once in a HIP graph and replayed while the
 // set of active members stays the same.
@@ -29,8 +30,6 @@ using cdc::host::StepPlan;
 struct cdc_ensemble {
     std::vector<cdc_solver *> members;
     int nv = 0, rk = 0;
-    int cool = cdc::kCoolAny;   // the members' shared cooling law (StageCooling)
-    int visc = -1;              // the viscosity law the captured step launches with (StageViscosity)
     cdc::TileShape ts{};
     hipStream_t stream = nullptr;
     bool use_graph = true;
@@ -39,7 +38,8 @@ struct cdc_ensemble {
     int cap_runs = 0;           // runs the buffers below hold
     cdc::StageArgs *h_args = nullptr;   // pinned, [stage][run]
     cdc::StageArgs *d_args = nullptr;
-    int32_t *d_tile_begin = nullptr, *d_block_begin = nullptr;
+    int32_t *d_tile_begin = nullptr;   // per law group, each from 0: [nrun + groups]
+    int32_t *d_block_begin = nullptr;  // [nrun + 1]
     double *d_amax = nullptr;
     int32_t *d_flags = nullptr;
     double *h_amax = nullptr;   // pinned
@@ -50,10 +50,17 @@ struct cdc_ensemble {
     std::vector<double> amax;
     std::vector<uint64_t> amax_of;
 
-    // graph of the last active set
+    // the active runs of a step in launch order (member indexes, grouped by
+    // law) and the law groups [r0, r1) of that order
+    struct Group {
+        int r0, r1, cool, visc, ntiles;
+    };
+    std::vector<Group> groups;
+
+    // graph of the last active set, and the laws it was captured with
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    std::vector<int> graph_runs;
+    std::vector<int> graph_runs, graph_laws;
 
     // statistics
     int64_t steps = 0;
@@ -68,6 +75,7 @@ void free_buffers(cdc_ensemble *e) {
     e->exec = nullptr;
     e->graph = nullptr;
     e->graph_runs.clear();
+    e->graph_laws.clear();
     if (e->h_args) (void)hipHostFree(e->h_args);
     if (e->h_amax) (void)hipHostFree(e->h_amax);
     if (e->h_flags) (void)hipHostFree(e->h_flags);
@@ -92,7 +100,7 @@ int alloc_buffers(cdc_ensemble *e, int n) {
         hipHostMalloc((void **)&e->h_amax, n * sizeof(double), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&e->h_flags, n * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void **)&e->d_args, na * sizeof(cdc::StageArgs)) != hipSuccess ||
-        hipMalloc((void **)&e->d_tile_begin, (n + 1) * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void **)&e->d_tile_begin, 2 * (size_t)n * sizeof(int32_t)) != hipSuccess ||
         hipMalloc((void **)&e->d_block_begin, (n + 1) * sizeof(int32_t)) != hipSuccess ||
         hipMalloc((void **)&e->d_amax, n * sizeof(double)) != hipSuccess ||
         hipMalloc((void **)&e->d_flags, n * sizeof(int32_t)) != hipSuccess) {
@@ -105,14 +113,18 @@ int alloc_buffers(cdc_ensemble *e, int n) {
 }
 
 // The device work of one step for `runs` (indexes into members), in order.
-int enqueue_step(cdc_ensemble *e, int nrun, int total_tiles, int total_blocks) {
+int enqueue_step(cdc_ensemble *e, int nrun, int total_blocks) {
     const size_t na = (size_t)e->rk * nrun;
     HIP_TRY(hipMemcpyAsync(e->d_args, e->h_args, na * sizeof(cdc::StageArgs), hipMemcpyHostToDevice, e->stream));
     HIP_TRY(cdc::launch_batch_reset(e->d_args, nrun, e->stream));
     for (int k = 0; k < e->rk; ++k) {
         const cdc::StageArgs *a = e->d_args + (size_t)k * nrun;
-        HIP_TRY(cdc::launch_stage_batch(e->nv, e->ts, a, e->d_tile_begin, nrun, total_tiles, nullptr, nullptr, 0,
-                                        e->cool, e->visc, k > 0 ? 1 : 0, nullptr, nullptr, e->stream));
+        for (size_t g = 0; g < e->groups.size(); ++g) {
+            const cdc_ensemble::Group &G = e->groups[g];
+            HIP_TRY(cdc::launch_stage_batch(e->nv, e->ts, a + G.r0, e->d_tile_begin + G.r0 + g, G.r1 - G.r0,
+                                            G.ntiles, nullptr, nullptr, 0, G.cool, G.visc, k > 0 ? 1 : 0, nullptr,
+                                            nullptr, e->stream));
+        }
         HIP_TRY(cdc::launch_finalize_batch(e->nv, a, e->d_block_begin, nrun, total_blocks, e->stream));
     }
     HIP_TRY(cdc::launch_batch_tail(e->nv, e->d_args + (size_t)(e->rk - 1) * nrun, nrun, e->d_amax, e->d_flags,
@@ -162,12 +174,6 @@ int32_t cdc_ensemble_create(cdc_solver_t *const *members, int32_t n, cdc_ensembl
     e->members.assign(members, members + n);
     e->nv = m0->nv;
     e->rk = m0->phys.rk_order;
-    if (e->nv == 4) {
-        // the ensemble's stages evaluate exp(-dt Omega / beta) themselves
-        bool same = true;
-        for (int i = 1; i < n; ++i) same = same && members[i]->hydro.cooling_model == m0->hydro.cooling_model;
-        if (same) e->cool = m0->hydro.cooling_model == CDC_COOLING_BETA ? cdc::kCoolBetaExp : cdc::kCoolT4;
-    }
     e->ts = ts;
     e->amax.assign(n, 0.0);
     e->amax_of.assign(n, UINT64_MAX);
@@ -221,14 +227,29 @@ int32_t cdc_ensemble_advance(cdc_ensemble_t *e, const int32_t *active, double *d
     if (codes_out)
         for (int i = 0; i < n; ++i) codes_out[i] = CDC_OK;
 
-    // active members, their dt and their stage plans
-    std::vector<int> runs;
+    // active members in launch order: grouped by their cooling and viscosity
+    // laws (their physics may change between steps), member order within a
+    // group; each group's stages launch the kernels a member's own advance
+    // launches
+    std::vector<int> runs, laws;
     for (int i = 0; i < n; ++i)
         if (!active || active[i]) runs.push_back(i);
     if (runs.empty()) return CDC_OK;
     const int nrun = (int)runs.size();
+    auto law = [&](int i) {
+        const cdc_solver *m = e->members[(size_t)i];
+        const int cool = m->nv == 3 ? cdc::kCoolAny
+                         : m->hydro.cooling_model == CDC_COOLING_BETA ? cdc::kCoolBetaExp
+                                                                      : cdc::kCoolT4;
+        return cool * 4 + (m->phys.viscosity_model == CDC_VISC_ALPHA ? cdc::kViscAlpha : cdc::kViscConst);
+    };
+    std::stable_sort(runs.begin(), runs.end(), [&](int a, int b) { return law(a) < law(b); });
+    for (int r = 0; r < nrun; ++r) laws.push_back(law(runs[(size_t)r]));
+
+    // their dt and stage plans
     std::vector<StepPlan> plans((size_t)nrun);
-    std::vector<int32_t> tile_begin((size_t)nrun + 1, 0), block_begin((size_t)nrun + 1, 0);
+    std::vector<int32_t> tile_begin, block_begin((size_t)nrun + 1, 0);
+    e->groups.clear();
     for (int r = 0; r < nrun; ++r) {
         cdc_solver *m = e->members[(size_t)runs[r]];
         if (m->ts.tx != e->ts.tx || m->ts.ty != e->ts.ty || m->ts.nt != e->ts.nt)
@@ -245,44 +266,42 @@ int32_t cdc_ensemble_advance(cdc_ensemble_t *e, const int32_t *active, double *d
         const double dt = cdc::host::step_dt(m, amax);
         if (dt_out) dt_out[runs[r]] = dt;
         cdc::host::plan_steps(m, 1, dt, plans[(size_t)r]);
-        tile_begin[(size_t)r + 1] = tile_begin[(size_t)r] + m->ntiles;
+        // tile offsets of each group from 0, one group after the other
+        if (r == 0 || laws[(size_t)r] != laws[(size_t)r - 1]) {
+            e->groups.push_back({r, r, laws[(size_t)r] / 4, laws[(size_t)r] % 4, 0});
+            tile_begin.push_back(0);
+        }
+        cdc_ensemble::Group &G = e->groups.back();
+        G.r1 = r + 1;
+        G.ntiles += m->ntiles;
+        tile_begin.push_back(G.ntiles);
         block_begin[(size_t)r + 1] = block_begin[(size_t)r] + m->nblk;
         for (int k = 0; k < e->rk; ++k) e->h_args[(size_t)k * nrun + r] = plans[(size_t)r].args[(size_t)k];
     }
-    const int total_tiles = tile_begin[(size_t)nrun], total_blocks = block_begin[(size_t)nrun];
+    const int total_blocks = block_begin[(size_t)nrun];
     // marching members read the strip-edge columns of their committed state
     for (int r = 0; r < nrun; ++r) {
         const int rc = cdc::host::ensure_edges(e->members[(size_t)runs[r]], e->stream);
         if (rc) return rc;
     }
 
-    // the active members' shared viscosity law (their physics may change
-    // between steps), and the graph rebuilt when it or the active set changed
-    int visc = -1;
-    for (int r = 0; r < nrun; ++r) {
-        const int v = e->members[(size_t)runs[r]]->phys.viscosity_model == CDC_VISC_ALPHA ? cdc::kViscAlpha
-                                                                                           : cdc::kViscConst;
-        visc = visc < 0 ? v : (visc == v ? v : (int)cdc::kViscAny);
-    }
-    if (visc != e->visc) {
-        e->visc = visc;
-        e->graph_runs.clear();
-    }
-    const bool same = e->graph_runs == runs && e->exec;
+    // the graph is rebuilt when the active set or a law changed
+    const bool same = e->graph_runs == runs && e->graph_laws == laws && e->exec;
     if (!same) {
         if (e->exec) (void)hipGraphExecDestroy(e->exec);
         if (e->graph) (void)hipGraphDestroy(e->graph);
         e->exec = nullptr;
         e->graph = nullptr;
         e->graph_runs.clear();
-        HIP_TRY(hipMemcpyAsync(e->d_tile_begin, tile_begin.data(), (nrun + 1) * sizeof(int32_t),
+        e->graph_laws.clear();
+        HIP_TRY(hipMemcpyAsync(e->d_tile_begin, tile_begin.data(), tile_begin.size() * sizeof(int32_t),
                                hipMemcpyHostToDevice, e->stream));
         HIP_TRY(hipMemcpyAsync(e->d_block_begin, block_begin.data(), (nrun + 1) * sizeof(int32_t),
                                hipMemcpyHostToDevice, e->stream));
         HIP_TRY(hipStreamSynchronize(e->stream));
         if (e->use_graph) {
             HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-            const int rc = enqueue_step(e, nrun, total_tiles, total_blocks);
+            const int rc = enqueue_step(e, nrun, total_blocks);
             hipGraph_t g = nullptr;
             const hipError_t ec = hipStreamEndCapture(e->stream, &g);
             if (rc) {
@@ -293,17 +312,19 @@ int32_t cdc_ensemble_advance(cdc_ensemble_t *e, const int32_t *active, double *d
             e->graph = g;
             HIP_TRY(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
             e->graph_runs = runs;
+            e->graph_laws = laws;
         }
     }
     if (e->use_graph) {
         HIP_TRY(hipGraphLaunch(e->exec, e->stream));
     } else {
-        const int rc = enqueue_step(e, nrun, total_tiles, total_blocks);
+        const int rc = enqueue_step(e, nrun, total_blocks);
         if (rc) return rc;
     }
     HIP_TRY(hipStreamSynchronize(e->stream));
 
-    int first = CDC_OK;
+    // the returned code is the lowest-indexed failing member's
+    int first = CDC_OK, first_i = n;
     for (int r = 0; r < nrun; ++r) {
         const int i = runs[(size_t)r];
         cdc_solver *m = e->members[(size_t)i];
@@ -319,7 +340,10 @@ int32_t cdc_ensemble_advance(cdc_ensemble_t *e, const int32_t *active, double *d
         e->amax_of[(size_t)i] = UINT64_MAX;
         const int code = cdc::host::resolve_error(m, flag, plans[(size_t)r], errs_out ? &errs_out[i] : nullptr);
         if (codes_out) codes_out[i] = code;
-        if (first == CDC_OK) first = code;
+        if (i < first_i) {
+            first = code;
+            first_i = i;
+        }
     }
     return first;
 }

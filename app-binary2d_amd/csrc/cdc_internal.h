@@ -91,12 +91,12 @@ struct StageArgs {
 
 // The cooling law every run of a stage launch shares, for kernels compiled
 // for one: T^4, beta cooling reading exp(-dt Omega / beta) from aux plane 2
-// (cool_factor), beta cooling evaluating it; kCoolAny decides per run at run
-// time (isothermal gas has no cooling).
+// (cool_factor), beta cooling evaluating it; kCoolAny: isothermal gas (no
+// cooling), and the tiled kernel, which reads the law from each run.
 enum StageCooling { kCoolAny = 0, kCoolT4 = 1, kCoolBetaFactor = 2, kCoolBetaExp = 3 };
 // The viscosity law every run of a stage launch shares, for the column-
-// marching kernel (constant nu includes nu = 0); kViscAny decides per run.
-enum StageViscosity { kViscAny = 0, kViscConst = 1, kViscAlpha = 2 };
+// marching kernel (constant nu includes nu = 0).
+enum StageViscosity { kViscConst = 1, kViscAlpha = 2 };
 
 // Tile shape chosen for a block size (the largest shape that divides it).
 struct TileShape {

@@ -106,7 +106,7 @@ struct FaceConsts {
 // HLLE (Toro ch. 10) plus the viscous flux 0.5 f (sa, sb); (vbx, vby) the
 // mean of the two cell-centre velocities (gamma-law energy flux).
 template <int NV, int AXIS>
-__device__ __forceinline__ void march_face(const FaceConsts &c, bool visc_on, bool alpha, const double *pl,
+__device__ __forceinline__ void march_face(const FaceConsts &c, bool alpha, const double *pl,
                                            const double *pr, double vbx, double vby, double sa, double sb,
                                            double fxp, double fyp, double *F) {
     const double d1x = fxp - c.x1, d1y = fyp - c.y1;
@@ -115,23 +115,20 @@ __device__ __forceinline__ void march_face(const FaceConsts &c, bool visc_on, bo
     const double r2s = d2x * d2x + d2y * d2y;
     double cs2f = 0.0;
     if constexpr (NV == 3) cs2f = (c.m1 * dev::frsq(r1s + c.rs2) + c.m2 * dev::frsq(r2s + c.rs2)) * c.inv_mach2;
-    double tn = 0.0, to = 0.0;
-    if (visc_on) {
-        double f = 0.5;
-        if (alpha) {
-            // 1 / sqrt(m1/q1^3 + m2/q2^3), q = r + 1e-12 dx
-            const double q1 = dev::fsqrt(r1s) + c.eps_dx;
-            const double q2 = dev::fsqrt(r2s) + c.eps_dx;
-            const double c1 = q1 * q1 * q1, c2 = q2 * q2 * q2;
-            // rsq(X / Y) = Y rsq(X Y), X = m1 q2^3 + m2 q1^3, Y = q1^3 q2^3:
-            // one transcendental instead of two
-            const double Y = c1 * c2;
-            const double itf = Y * dev::frsq((c.m1 * c2 + c.m2 * c1) * Y);
-            f = NV == 3 ? 0.5 * (cs2f * itf) : 0.5 * itf;
-        }
-        tn = f * sa;
-        to = f * sb;
+    double f = 0.5;
+    if (alpha) {
+        // 1 / sqrt(m1/q1^3 + m2/q2^3), q = r + 1e-12 dx
+        const double q1 = dev::fsqrt(r1s) + c.eps_dx;
+        const double q2 = dev::fsqrt(r2s) + c.eps_dx;
+        const double c1 = q1 * q1 * q1, c2 = q2 * q2 * q2;
+        // rsq(X / Y) = Y rsq(X Y), X = m1 q2^3 + m2 q1^3, Y = q1^3 q2^3:
+        // one transcendental instead of two
+        const double Y = c1 * c2;
+        const double itf = Y * dev::frsq((c.m1 * c2 + c.m2 * c1) * Y);
+        f = NV == 3 ? 0.5 * (cs2f * itf) : 0.5 * itf;
     }
+    const double tn = f * sa;
+    const double to = f * sb;
     constexpr int NM = 1 + AXIS;   // the normal momentum component
     const double vnl = pl[NM], vnr = pr[NM];
     double ul[NV], ur[NV], fl[NV], fr[NV], csl, csr, ppl, ppr;
@@ -243,10 +240,10 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
     const StageConsts &k = A.k;
     const dev::Axis axx = dev::block_axis(A.R, A.L, bi, bs);
     const dev::Axis axy = dev::block_axis(A.R, A.L, bj, bs);
-    // the viscosity law: fixed at compile time (constant nu, nu = 0 included:
-    // its stress terms are then zero) or read from the run
-    const bool alpha = VISC == kViscAlpha || (VISC == kViscAny && k.viscosity_model == CDC_VISC_ALPHA);
-    const bool visc_on = VISC != kViscAny || alpha || k.visc != 0.0;
+    // the viscosity law, fixed at compile time (constant nu, nu = 0 included:
+    // its stress terms are then zero)
+    static_assert(VISC == kViscConst || VISC == kViscAlpha, "a march kernel is compiled for one viscosity law");
+    constexpr bool alpha = VISC == kViscAlpha;
     const bool beta = k.cooling_model == CDC_COOLING_BETA;
     const int gi0 = bi * bs, gj0 = bj * bs;
     // wrapped global row / column of a block-local index
@@ -303,18 +300,16 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             gxR[v] = dev::plm(txR, p[6][v], p[2][v], p[7][v]);
             gyR[v] = dev::plm(tyR, p[1][v], p[2][v], p[3][v]);
         }
-        double axL = 0.0, ayL = 0.0, bL = 0.0, axR = 0.0, ayR = 0.0, bR = 0.0;
-        if (visc_on) {
-            stress_factors<NV>(k, alpha, p[1], id[1], gxL, gyL, axL, ayL, bL);
-            stress_factors<NV>(k, alpha, p[2], id[2], gxR, gyR, axR, ayR, bR);
-        }
+        double axL, ayL, bL, axR, ayR, bR;
+        stress_factors<NV>(k, alpha, p[1], id[1], gxL, gyL, axL, ayL, bL);
+        stress_factors<NV>(k, alpha, p[2], id[2], gxR, gyR, axR, ayR, bR);
         double pl[NV], pr[NV], F[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             pl[v] = p[1][v] + gyL[v] * 0.5;
             pr[v] = p[2][v] - gyR[v] * 0.5;
         }
-        march_face<NV, 1>(fc, visc_on, alpha, pl, pr, 0.5 * (p[1][1] + p[2][1]), 0.5 * (p[1][2] + p[2][2]),
+        march_face<NV, 1>(fc, alpha, pl, pr, 0.5 * (p[1][1] + p[2][1]), 0.5 * (p[1][2] + p[2][2]),
                           ayL + ayR, bL + bR, axx.c(r), axy.v(cR), F);
         const int hr = r - ri0;
 #pragma unroll
@@ -367,7 +362,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             XLa[v] = Pa[v] + gx[v] * 0.5;
         }
         double ay_unused;
-        if (visc_on) stress_factors<NV>(k, alpha, Pa, ida, gx, gy, axa, ay_unused, ba);
+        stress_factors<NV>(k, alpha, Pa, ida, gx, gy, axa, ay_unused, ba);
     }
 
     // ---- the march ------------------------------------------------------------
@@ -430,15 +425,15 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             gx[v] = dev::plm(tx, Pa[v], Pb[v], Pz[v]);
             gy[v] = dev::plm(ty, PL[v], Pb[v], PR[v]);
         }
-        double axb = 0.0, ayb = 0.0, bb = 0.0;
-        if (visc_on) stress_factors<NV>(k, alpha, Pb, idb, gx, gy, axb, ayb, bb);
+        double axb, ayb, bb;
+        stress_factors<NV>(k, alpha, Pb, idb, gx, gy, axb, ayb, bb);
         // x-face between rows z - 2 and z - 1 (block-local vertex z - 1)
         double Fxb[NV];
         {
             double pr[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) pr[v] = Pb[v] - gx[v] * 0.5;
-            march_face<NV, 0>(fc, visc_on, alpha, XLa, pr, 0.5 * (Pa[1] + Pb[1]), 0.5 * (Pa[2] + Pb[2]), axa + axb,
+            march_face<NV, 0>(fc, alpha, XLa, pr, 0.5 * (Pa[1] + Pb[1]), 0.5 * (Pa[2] + Pb[2]), axa + axb,
                               ba + bb, Xv[t], yc, Fxb);
         }
         // y-faces of row z - 1: lane l evaluates the face between columns
@@ -461,7 +456,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
                 vbx = 0.5 * (Pb[1] + PR[1]);
                 vby = 0.5 * (Pb[2] + PR[2]);
             }
-            march_face<NV, 1>(fc, visc_on, alpha, yl, yr, vbx, vby, ayb + ayn, bb + bn, Xc[t], yv1, F);
+            march_face<NV, 1>(fc, alpha, yl, yr, vbx, vby, ayb + ayn, bb + bn, Xc[t], yv1, F);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const double f = lane == 63 ? Fb[(H + t) * NV + v] : F[v];
@@ -679,12 +674,10 @@ hipError_t launch_march_batch(int nv, int rows, int cool, int visc, int read_bas
     if (nv == V && rows == R && c == C && rb == B && visc == W)                                               \
         return launch_march_t<V, R, C, B, W>(args, begin, nruns, ntiles, fin, fin_begin, fin_nruns, nblk, ev0, ev1, \
                                              s);
-#define CDC_MARCH(V, R, C, B) \
-    CDC_MARCH_V(V, R, C, B, kViscAny) CDC_MARCH_V(V, R, C, B, kViscConst) CDC_MARCH_V(V, R, C, B, kViscAlpha)
+#define CDC_MARCH(V, R, C, B) CDC_MARCH_V(V, R, C, B, kViscConst) CDC_MARCH_V(V, R, C, B, kViscAlpha)
 #define CDC_MARCH_RB(V, R, C) CDC_MARCH(V, R, C, 0) CDC_MARCH(V, R, C, 1)
 #define CDC_MARCH_ROWS(V, C) CDC_MARCH_RB(V, 32, C) CDC_MARCH_RB(V, 16, C)
     CDC_MARCH_ROWS(3, kCoolAny)
-    CDC_MARCH_ROWS(4, kCoolAny)
     CDC_MARCH_ROWS(4, kCoolT4)
     CDC_MARCH_ROWS(4, kCoolBetaFactor)
     CDC_MARCH_ROWS(4, kCoolBetaExp)
