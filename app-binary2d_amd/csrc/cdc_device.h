@@ -61,12 +61,13 @@ __device__ __forceinline__ double frcp(double x) {
 }
 
 // 1 / sqrt(x), x > 0: one step of the coupled Newton iteration on
-// (g ~ sqrt x, h ~ 1/(2 sqrt x))
+// (g ~ sqrt x, h ~ 1/(2 sqrt x)); the step's result 2 (h + h r) is formed
+// as y + y r (y = 2 h exactly: the same bits, one multiplication fewer)
 __device__ __forceinline__ double frsq(double x) {
     const double y = __builtin_amdgcn_rsq(x);
     const double h = 0.5 * y, g = x * y;
     const double r = __builtin_fma(-h, g, 0.5);
-    return 2.0 * __builtin_fma(h, r, h);
+    return __builtin_fma(y, r, y);
 }
 
 // sqrt(x), x >= 0: one coupled step, then one residual correction of g
@@ -105,10 +106,12 @@ __device__ __forceinline__ Axis block_axis(double R, double L, int bi, int bs) {
     return ax;
 }
 
+// minmod of three values: the median of (0, min, max), i.e. the smallest
+// magnitude when all three share a sign and 0 otherwise
 __device__ __forceinline__ double minmod3(double a, double b, double c) {
     const double mn = fmin(a, fmin(b, c));
     const double mx = fmax(a, fmax(b, c));
-    return fmax(mn, 0.0) + fmin(mx, 0.0);
+    return fmax(mn, fmin(mx, 0.0));
 }
 
 // godunov-core plm_gradient: minmod(theta (b - a), (c - a) / 2, theta (c - b))
@@ -117,20 +120,24 @@ __device__ __forceinline__ double plm(double th, double a, double b, double c) {
 }
 
 // try_to_primitive: iso src/physics.rs:489-503, Euler src/physics.rs:631-660.
-// Also returns 1/Sigma of the recovered state in `id`, and in `val` the
-// offending value when the result is an error code.  The floors compare
-// against StageConsts::*_floor_cmp (-inf when unset) and the error is chosen
-// by selects, so the recovery has no per-cell branch but the rare floor.
-template <int NV>
+// Also returns 1/Sigma of the recovered state in `id`, the pressure before
+// the pressure floor in `praw` (Euler; the source terms' recovery of the
+// same state has no floors), and in `val` the offending value when the
+// result is an error code.  The floors compare against
+// StageConsts::*_floor_cmp (-inf when unset) and the error is chosen by
+// selects, so the recovery has no per-cell branch but the rare floor.
+// DFLOOR = false: the run has no density floor (the test is left out).
+template <int NV, bool DFLOOR = true>
 __device__ __forceinline__ int try_to_primitive(const StageConsts &k, const double *u, double *p,
-                                                double &val, double &id) {
+                                                double &val, double &id, double &praw) {
     if constexpr (NV == 3) {
         double d = u[0], sx = u[1], sy = u[2];
-        if (d < k.density_floor_cmp) {
+        if (DFLOOR && d < k.density_floor_cmp) {
             sx = 0.0; sy = 0.0; d = k.density_floor;
         }
         id = frcp(d);
         p[0] = d; p[1] = sx * id; p[2] = sy * id;
+        praw = 0.0;
         val = d;
         return d < 0.0 ? CDC_ERR_NEGATIVE_DENSITY : 0;
     } else {
@@ -139,7 +146,8 @@ __device__ __forceinline__ int try_to_primitive(const StageConsts &k, const doub
         const double vx = u[1] * id, vy = u[2] * id;
         p[0] = d; p[1] = vx; p[2] = vy;
         p[3] = (u[3] - 0.5 * d * (vx * vx + vy * vy)) * k.gm1;
-        if (d < k.density_floor_cmp) {
+        praw = p[3];
+        if (DFLOOR && d < k.density_floor_cmp) {
             p[0] = k.density_floor; p[1] = 0.0; p[2] = 0.0; p[3] = k.density_floor_pressure;
             id = frcp(k.density_floor);
         }
@@ -150,6 +158,13 @@ __device__ __forceinline__ int try_to_primitive(const StageConsts &k, const doub
         val = nd ? p[0] : pf;
         return nd ? CDC_ERR_NEGATIVE_DENSITY : (pf <= 0.0 ? CDC_ERR_NEGATIVE_PRESSURE : 0);
     }
+}
+
+template <int NV>
+__device__ __forceinline__ int try_to_primitive(const StageConsts &k, const double *u, double *p,
+                                                double &val, double &id) {
+    double praw;
+    return try_to_primitive<NV, true>(k, u, p, val, id, praw);
 }
 
 // Hydrodynamics::max_signal_speed of a conserved state: iso
@@ -356,16 +371,19 @@ __device__ __forceinline__ void cell_sources(const StageArgs &A, const Orbit &o,
     // Plummer gravity of each body: a = -m d / (r^2 + r_s^2)^(3/2)
     g = sink_geom(k, o, x, y);
     const double ex1 = g.ex1, ey1 = g.ey1, ex2 = g.ex2, ey2 = g.ey2, q1 = g.q1, q2 = g.q2;
+    // (the items are impulses: the factor -m dt Sigma / (r^2 + r_s^2)^(3/2)
+    // is formed once, dt folded in before the offsets; a rounding-level
+    // regrouping of the reference's f dt)
     const double y1 = frsq(q1 + k.rs2), y2 = frsq(q2 + k.rs2);
-    const double g1 = -o.m1 * (y1 * y1 * y1) * sd, g2 = -o.m2 * (y2 * y2 * y2) * sd;
+    const double g1 = (o.m1 * -dt) * (y1 * y1 * y1) * sd, g2 = (o.m2 * -dt) * (y2 * y2 * y2) * sd;
     const double fx1 = g1 * ex1, fy1 = g1 * ey1, fx2 = g2 * ex2, fy2 = g2 * ey2;
 
     if constexpr (!LAZY_SINKS) sink_items<NV>(k, o, dt, u, vx, vy, id, p0, g, it);
     const double bfac = -dt * brate;
 #pragma unroll
     for (int v = 0; v < NV; ++v) it[CDC_ITEM_BUFFER][v] = (u[v] - u0[v]) * bfac;
-    it[CDC_ITEM_GRAV1][0] = 0.0; it[CDC_ITEM_GRAV1][1] = fx1 * dt; it[CDC_ITEM_GRAV1][2] = fy1 * dt;
-    it[CDC_ITEM_GRAV2][0] = 0.0; it[CDC_ITEM_GRAV2][1] = fx2 * dt; it[CDC_ITEM_GRAV2][2] = fy2 * dt;
+    it[CDC_ITEM_GRAV1][0] = 0.0; it[CDC_ITEM_GRAV1][1] = fx1; it[CDC_ITEM_GRAV1][2] = fy1;
+    it[CDC_ITEM_GRAV2][0] = 0.0; it[CDC_ITEM_GRAV2][1] = fx2; it[CDC_ITEM_GRAV2][2] = fy2;
 
     if constexpr (NV == 3) {
 #pragma unroll
@@ -378,8 +396,8 @@ __device__ __forceinline__ void cell_sources(const StageArgs &A, const Orbit &o,
             for (int v = 0; v < 3; ++v) it[CDC_ITEM_FAKE_MASS][v] = u0[v] * k.fake_rate * dt;
         }
     } else {
-        it[CDC_ITEM_GRAV1][3] = (fx1 * vx + fy1 * vy) * dt;
-        it[CDC_ITEM_GRAV2][3] = (fx2 * vx + fy2 * vy) * dt;
+        it[CDC_ITEM_GRAV1][3] = fx1 * vx + fy1 * vy;
+        it[CDC_ITEM_GRAV2][3] = fx2 * vx + fy2 * vy;
         const double e0 = p0 * (id * k.inv_gm1);
         double ec;
         const bool beta = COOL == kCoolAny ? k.cooling_model == CDC_COOLING_BETA : COOL != kCoolT4;

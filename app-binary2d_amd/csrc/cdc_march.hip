@@ -62,23 +62,32 @@ __device__ __forceinline__ double dpp0(double src) {
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
+// The stress coefficients of a stage: the normal components are
+// (4/3 + lambda) d_n v_n + (lambda - 2/3) d_t v_t, with the 1/dx, 1/dy of
+// the per-cell gradients folded in.
+struct StressCoef {
+    double nx, ty, ny, tx;   // (4/3 + l)/dx, (l - 2/3)/dy, (4/3 + l)/dy, (l - 2/3)/dx
+    __device__ __forceinline__ explicit StressCoef(const StageConsts &k)
+        : nx((4.0 / 3.0 + k.lam) * k.idx), ty((k.lam - 2.0 / 3.0) * k.idy), ny((4.0 / 3.0 + k.lam) * k.idy),
+          tx((k.lam - 2.0 / 3.0) * k.idx) {}
+};
+
 // Stress factors of one cell: Ax (normal-normal component on an x-face), Ay
 // (on a y-face) and B (off-diagonal), each times Sigma and the cell-local
-// part of nu (src/physics.rs:322-340, src/scheme.rs:230-247).
+// part of nu (src/physics.rs:322-340, src/scheme.rs:230-247), the
+// divergence term regrouped into the two coefficients of each normal
+// component (rounding-level differences).
 template <int NV>
-__device__ __forceinline__ void stress_factors(const StageConsts &k, bool alpha, const double *p, double id,
-                                               const double *gx, const double *gy, double &ax, double &ay,
-                                               double &b) {
+__device__ __forceinline__ void stress_factors(const StageConsts &k, const StressCoef &sc, bool alpha,
+                                               const double *p, double id, const double *gx, const double *gy,
+                                               double &ax, double &ay, double &b) {
     double K = p[0] * k.visc;
     if constexpr (NV == 4) {
         if (alpha) K = K * (k.gm * p[3] * id);   // c_s^2 = gamma p / Sigma
     }
-    const double gxvx = gx[1] * k.idx, gxvy = gx[2] * k.idx;
-    const double gyvx = gy[1] * k.idy, gyvy = gy[2] * k.idy;
-    const double div = gxvx + gyvy, off = gxvy + gyvx;
-    ax = K * ((4.0 / 3.0) * gxvx - (2.0 / 3.0) * gyvy + k.lam * div);
-    ay = K * ((4.0 / 3.0) * gyvy - (2.0 / 3.0) * gxvx + k.lam * div);
-    b = K * off;
+    ax = K * (gx[1] * sc.nx + gy[2] * sc.ty);
+    ay = K * (gy[2] * sc.ny + gx[1] * sc.tx);
+    b = K * (gx[2] * k.idx + gy[1] * k.idy);
 }
 
 // A double held in a vector register from here on: the constants every face
@@ -131,7 +140,7 @@ __device__ __forceinline__ void march_face(const FaceConsts &c, bool alpha, cons
     const double to = f * sb;
     constexpr int NM = 1 + AXIS;   // the normal momentum component
     const double vnl = pl[NM], vnr = pr[NM];
-    double ul[NV], ur[NV], fl[NV], fr[NV], csl, csr, ppl, ppr;
+    double ul[NV], ur[NV], csl, csr, ppl, ppr;
     ul[0] = pl[0]; ul[1] = pl[0] * pl[1]; ul[2] = pl[0] * pl[2];
     ur[0] = pr[0]; ur[1] = pr[0] * pr[1]; ur[2] = pr[0] * pr[2];
     if constexpr (NV == 3) {
@@ -146,24 +155,23 @@ __device__ __forceinline__ void march_face(const FaceConsts &c, bool alpha, cons
         csr = gpr * dev::frsq(gpr * pr[0]);
         ppl = pl[3];
         ppr = pr[3];
-        fl[3] = (ul[3] + ppl) * vnl;
-        fr[3] = (ur[3] + ppr) * vnr;
     }
-#pragma unroll
-    for (int v = 0; v < 3; ++v) {
-        fl[v] = ul[v] * vnl;
-        fr[v] = ur[v] * vnr;
-    }
-    fl[NM] += ppl;
-    fr[NM] += ppr;
     const double ap = fmax(fmax(vnl + csl, vnr + csr), 0.0);
     const double am = fmin(fmin(vnl - csl, vnr - csr), 0.0);
     const double inv = dev::frcp(ap - am);
-    // F = (ap F_L - am F_R - ap am (U_L - U_R)) / (ap - am) with the three
-    // weights formed once
+    // F = (ap F_L - am F_R - ap am (U_L - U_R)) / (ap - am) with the
+    // weights wl = ap / (ap - am), wr = -am / (ap - am) and
+    // wd = -ap am / (ap - am) formed once.  With F_L = U_L vnl + p_L e_n
+    // (+ p_L vnl in the energy) this is U_L A + U_R B + the pressure terms,
+    // A = wl vnl + wd, B = wr vnr - wd: two operations per component
+    // (a regrouping, rounding-level differences).
     const double wl = ap * inv, wr = -am * inv, wd = -(ap * am) * inv;
+    const double cA = wl * vnl + wd, cB = wr * vnr - wd;
+    const double pwl = ppl * wl, pwr = ppr * wr;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) F[v] = fl[v] * wl + (fr[v] * wr + (ul[v] - ur[v]) * wd);
+    for (int v = 0; v < NV; ++v) F[v] = ul[v] * cA + ur[v] * cB;
+    F[NM] += pwl + pwr;
+    if constexpr (NV == 4) F[3] += pwl * vnl + pwr * vnr;
     const double tau_x = AXIS ? to : tn, tau_y = AXIS ? tn : to;
     F[1] -= tau_x;
     F[2] -= tau_y;
@@ -197,9 +205,13 @@ __device__ __forceinline__ void load_cell(Rsrc r, unsigned voff, unsigned pl, do
     for (int v = 0; v < NV; ++v) out[v] = bload(r, voff, (unsigned)v * pl);
 }
 
-// PLM theta of the three cells a, b, c (plm_density_trigger)
+// PLM theta of the three cells a, b, c (plm_density_trigger); PDT = false:
+// the run has no trigger
+template <bool PDT>
 __device__ __forceinline__ double theta3(const StageConsts &k, double plm, double da, double db, double dc) {
-    if (k.has_pdt && (da < k.pdt || db < k.pdt || dc < k.pdt)) return 0.0;
+    if constexpr (PDT) {
+        if (k.has_pdt && (da < k.pdt || db < k.pdt || dc < k.pdt)) return 0.0;
+    }
     return plm;
 }
 
@@ -217,8 +229,12 @@ struct MarchLds {
 
 // One strip of one RK stage (see the file comment), tile = strip index in
 // the run: strips are numbered block by block, row segments of H rows, then
-// 64-column strips within a segment.
-template <int NV, int H, int COOL, int RB, int VISC>
+// 64-column strips within a segment.  PLAIN: the run has neither a density
+// floor nor a PLM density trigger, so the per-cell tests for them are left
+// out and the update reuses the primitive recovery of its row (1/Sigma, the
+// velocities and the pressure before the floor are the source terms'
+// recovery of the same state).
+template <int NV, int H, int COOL, int RB, int VISC, bool PLAIN>
 __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, double *lds, int &s_key) {
     constexpr int NS = SumMap<NV>::n, NR = NS + 1;
     constexpr int NSINK = 2 * NV;   // the sink items lead the strip sums
@@ -249,6 +265,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
     // wrapped global row / column of a block-local index
     auto grow = [&](int i) { const int g = gi0 + i; return g < 0 ? g + N : (g >= N ? g - N : g); };
     const FaceConsts fc(A);
+    const StressCoef sc(k);
     // the orbit of the sources, positions and masses from the face constants
     Orbit ob = A.orb;
     ob.x1 = fc.x1; ob.y1 = fc.y1; ob.m1 = fc.m1;
@@ -288,11 +305,14 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             double val;
-            (void)dev::try_to_primitive<NV>(k, u[c], p[c], val, id[c]);
+            double praw;
+            (void)dev::try_to_primitive<NV, !PLAIN>(k, u[c], p[c], val, id[c], praw);
         }
         double gxL[NV], gyL[NV], gxR[NV], gyR[NV];
-        const double txL = theta3(k, fc.plm, p[4][0], p[1][0], p[5][0]), tyL = theta3(k, fc.plm, p[0][0], p[1][0], p[2][0]);
-        const double txR = theta3(k, fc.plm, p[6][0], p[2][0], p[7][0]), tyR = theta3(k, fc.plm, p[1][0], p[2][0], p[3][0]);
+        const double txL = theta3<!PLAIN>(k, fc.plm, p[4][0], p[1][0], p[5][0]);
+        const double tyL = theta3<!PLAIN>(k, fc.plm, p[0][0], p[1][0], p[2][0]);
+        const double txR = theta3<!PLAIN>(k, fc.plm, p[6][0], p[2][0], p[7][0]);
+        const double tyR = theta3<!PLAIN>(k, fc.plm, p[1][0], p[2][0], p[3][0]);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             gxL[v] = dev::plm(txL, p[4][v], p[1][v], p[5][v]);
@@ -301,8 +321,8 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             gyR[v] = dev::plm(tyR, p[1][v], p[2][v], p[3][v]);
         }
         double axL, ayL, bL, axR, ayR, bR;
-        stress_factors<NV>(k, alpha, p[1], id[1], gxL, gyL, axL, ayL, bL);
-        stress_factors<NV>(k, alpha, p[2], id[2], gxR, gyR, axR, ayR, bR);
+        stress_factors<NV>(k, sc, alpha, p[1], id[1], gxL, gyL, axL, ayL, bL);
+        stress_factors<NV>(k, sc, alpha, p[2], id[2], gxR, gyR, axR, ayR, bR);
         double pl[NV], pr[NV], F[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -339,13 +359,14 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
     if (lane <= H) Xv[lane] = axx.v(ri0 + lane);
     __syncthreads();   // the edge data and coordinates in LDS
 
-    double Pa[NV], Pb[NV], idb;
+    // 1/Sigma and the pressure before the floor of rows z - 2 (a) and z - 1 (b)
+    double Pa[NV], Pb[NV], ida, idb, pra, prb;
     double XLa[NV], axa = 0.0, ba = 0.0;
     {
-        double Pm2[NV], idm2, ida, val;
-        (void)dev::try_to_primitive<NV>(k, Um2, Pm2, val, idm2);
-        (void)dev::try_to_primitive<NV>(k, Um1, Pa, val, ida);
-        const int code = dev::try_to_primitive<NV>(k, U0r, Pb, val, idb);
+        double Pm2[NV], idm2, prm2, val;
+        (void)dev::try_to_primitive<NV, !PLAIN>(k, Um2, Pm2, val, idm2, prm2);
+        (void)dev::try_to_primitive<NV, !PLAIN>(k, Um1, Pa, val, ida, pra);
+        const int code = dev::try_to_primitive<NV, !PLAIN>(k, U0r, Pb, val, idb, prb);
         if (code) { key = ri0 * bs + j; kcode = code; kval = val; }
         // row ri0 - 1: gradients, left state of its x+ face, stress factors
         double gx[NV], gy[NV], PL[NV], PR[NV];
@@ -354,7 +375,8 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             PL[v] = dpp<kFromPrev>(hP[v], Pa[v]);
             PR[v] = dpp<kFromNext>(hP[(H + 2) * NV + v], Pa[v]);
         }
-        const double tx = theta3(k, fc.plm, Pm2[0], Pa[0], Pb[0]), ty = theta3(k, fc.plm, PL[0], Pa[0], PR[0]);
+        const double tx = theta3<!PLAIN>(k, fc.plm, Pm2[0], Pa[0], Pb[0]);
+        const double ty = theta3<!PLAIN>(k, fc.plm, PL[0], Pa[0], PR[0]);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             gx[v] = dev::plm(tx, Pm2[v], Pa[v], Pb[v]);
@@ -362,7 +384,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             XLa[v] = Pa[v] + gx[v] * 0.5;
         }
         double ay_unused;
-        stress_factors<NV>(k, alpha, Pa, ida, gx, gy, axa, ay_unused, ba);
+        stress_factors<NV>(k, sc, alpha, Pa, ida, gx, gy, axa, ay_unused, ba);
     }
 
     // ---- the march ------------------------------------------------------------
@@ -382,7 +404,9 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
 #pragma unroll
     for (int v = 0; v < NV; ++v) { Ga[v] = 0.0; Ua[v] = Um1[v]; Ub[v] = U0r[v]; }
     double amax = 0.0;
-    const double ndt = -A.dt;
+    // -dt/dx, -dt/dy: Ga = -dt (dFy/dy - Fx(- 1/2)/dx), the update adds
+    // -dt Fx(+ 1/2)/dx
+    const double idxdt = k.idx * -A.dt, idydt = k.idy * -A.dt;
     double S0[NV], Sb[NV], Sr, Sc = 0.0;
     // update operands, one row ahead of their update (two rows ahead, in a
     // second register set, measured 6% slower): every step issues the same
@@ -409,8 +433,8 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
         double Un[NV];
         if constexpr (next) load_cell<NV>(ruc, ((unsigned)grow(z + 1) * N + gj) * 8u, PB, Un);
         // primitives of row z
-        double Pz[NV], idz, val;
-        const int code = dev::try_to_primitive<NV>(k, Uz, Pz, val, idz);
+        double Pz[NV], idz, prz, val;
+        const int code = dev::try_to_primitive<NV, !PLAIN>(k, Uz, Pz, val, idz, prz);
         if (code && z < ri0 + H && z * bs + j < key) { key = z * bs + j; kcode = code; kval = val; }
         // gradients of row z - 1 (the halo columns' primitives from LDS)
         double gx[NV], gy[NV], PL[NV], PR[NV];
@@ -419,14 +443,15 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             PL[v] = dpp<kFromPrev>(hP[(t + 1) * NV + v], Pb[v]);
             PR[v] = dpp<kFromNext>(hP[((H + 2) + t + 1) * NV + v], Pb[v]);
         }
-        const double tx = theta3(k, fc.plm, Pa[0], Pb[0], Pz[0]), ty = theta3(k, fc.plm, PL[0], Pb[0], PR[0]);
+        const double tx = theta3<!PLAIN>(k, fc.plm, Pa[0], Pb[0], Pz[0]);
+        const double ty = theta3<!PLAIN>(k, fc.plm, PL[0], Pb[0], PR[0]);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             gx[v] = dev::plm(tx, Pa[v], Pb[v], Pz[v]);
             gy[v] = dev::plm(ty, PL[v], Pb[v], PR[v]);
         }
         double axb, ayb, bb;
-        stress_factors<NV>(k, alpha, Pb, idb, gx, gy, axb, ayb, bb);
+        stress_factors<NV>(k, sc, alpha, Pb, idb, gx, gy, axb, ayb, bb);
         // x-face between rows z - 2 and z - 1 (block-local vertex z - 1)
         double Fxb[NV];
         {
@@ -457,10 +482,19 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
                 vby = 0.5 * (Pb[2] + PR[2]);
             }
             march_face<NV, 1>(fc, alpha, yl, yr, vbx, vby, ayb + ayn, bb + bn, Xc[t], yv1, F);
+            // the strip-edge faces of this row, read by every lane (one
+            // address: a broadcast) so that no lane-dependent branch waits
+            // on LDS
+            double fe0[NV], fe1[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                const double f = lane == 63 ? Fb[(H + t) * NV + v] : F[v];
-                dFyb[v] = f - dpp<kFromPrev>(Fb[t * NV + v], f);
+                fe0[v] = Fb[t * NV + v];
+                fe1[v] = Fb[(H + t) * NV + v];
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double f = lane == 63 ? fe1[v] : F[v];
+                dFyb[v] = f - dpp<kFromPrev>(fe0[v], f);
             }
         } else {
 #pragma unroll
@@ -472,19 +506,25 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             const unsigned g = ((unsigned)(gi0 + i) * N + gj) * 8u;
             double du[NV];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) du[v] = (Fxb[v] * k.idx + Ga[v]) * ndt;
+            for (int v = 0; v < NV; ++v) du[v] = Fxb[v] * idxdt + Ga[v];
             // primitive recovery of u without floors (the sources' to_primitive)
-            const double id = dev::frcp(Ua[0]);
-            double vx, vy;
-            if (A.reuse_prim) {
+            double id, vx, vy, p0 = 0.0;
+            if constexpr (PLAIN) {
+                id = ida;
                 vx = Pa[1];
                 vy = Pa[2];
+                if constexpr (NV == 4) p0 = pra;
             } else {
-                vx = Ua[1] * id;
-                vy = Ua[2] * id;
+                id = dev::frcp(Ua[0]);
+                if (A.reuse_prim) {
+                    vx = Pa[1];
+                    vy = Pa[2];
+                } else {
+                    vx = Ua[1] * id;
+                    vy = Ua[2] * id;
+                }
+                if constexpr (NV == 4) p0 = (Ua[3] - 0.5 * Ua[0] * (vx * vx + vy * vy)) * k.gm1;
             }
-            double p0 = 0.0;
-            if constexpr (NV == 4) p0 = (Ua[3] - 0.5 * Ua[0] * (vx * vx + vy * vy)) * k.gm1;
             double it[7][NV];
             dev::SinkGeom sg;
             dev::cell_sources<NV, COOL, true>(A, ob, Ua, S0, Xc[t - 1], yc, vx, vy, id, p0, Sr, Sc, it, sg);
@@ -540,9 +580,12 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             Ua[v] = Ub[v];
             Ub[v] = Uz[v];
             if constexpr (next) Uz[v] = Un[v];
-            Ga[v] = dFyb[v] * k.idy - Fxb[v] * k.idx;
+            Ga[v] = dFyb[v] * idydt - Fxb[v] * idxdt;
         }
+        ida = idb;
         idb = idz;
+        pra = prb;
+        prb = prz;
         axa = axb;
         ba = bb;
         // the next row's state is complete at the end of every step (its
@@ -629,7 +672,8 @@ march_kernel_batch(const StageArgs *__restrict__ args, const int32_t *__restrict
     const int r = run_of(begin, nruns, t);
     const StageArgs &A = args[r];
     if (stage_skipped(A)) return;
-    march_strip<NV, H, COOL, RB, VISC>(A, t - begin[r], lds, s_key);
+    if (A.k.has_density_floor || A.k.has_pdt) march_strip<NV, H, COOL, RB, VISC, false>(A, t - begin[r], lds, s_key);
+    else march_strip<NV, H, COOL, RB, VISC, true>(A, t - begin[r], lds, s_key);
 }
 
 template <int NV, int H, int COOL, int RB, int VISC>
