@@ -379,7 +379,7 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
                   train_steps=4000, lr=1e-3, seed=5, limiter=0.1, train_mask=0.05, pair_steps=1,
                   loop_rounds=0, loop_starts=8, loop_steps=200, loop_every=10, loop_train_steps=2000, burn_in=0,
                   rollout_rounds=0, rollout_steps=200, rollout_every=20, rollout_train_steps=2000, rollout_gain=0.1,
-                  rollout_smooth=8, log=print):
+                  rollout_smooth=8, pair_sinks=True, log=print):
     """Train a closure for the coarse grid (coarse_blocks^2 blocks, 4x coarser
     than the fine workload) on pairs taken every sample_every fine steps along
     the fine run up to train_orbits: each pair compares pair_steps coarse steps
@@ -395,7 +395,12 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
     each of the loop_starts states, a pair every rollout_every steps), each
     round adding to the samples (the one-step pairs stay) and followed by
     rollout_train_steps training steps; rollout_gain and rollout_smooth damp
-    and smooth their targets.  Returns a dict with
+    and smooth their targets.  With pair_sinks False the one-step pairs are
+    stepped with the sinks off in both grids (from states of the run with
+    sinks): a flux correction can only move mass, so a target that holds
+    the two grids' different sink removals would teach it to move mass out
+    of the sink cells (or into them) instead of correcting the transport;
+    the coarse run's own sinks then do the accreting.  Returns a dict with
     the fine solver at the end of the training window, the trained closure
     (limiter kappa and mask as given), the coarse mesh, hydro, physics, the
     coarse initial state, the trained parameters and output scale, and the
@@ -409,7 +414,15 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
     cm = Mesh(coarse_blocks, fm.num_blocks * fm.block_size // (4 * coarse_blocks), fm.domain_radius)
     cu0, _, _ = resample_state(fm, cm, u0)
     fine = Solver(fm, hydro, ph, u0)
-    base, plain, truth = (Solver(cm, hydro, ph, cu0) for _ in range(3))
+    base, truth = (Solver(cm, hydro, ph, cu0) for _ in range(2))
+    # the solvers that take the pair steps (the same physics, or without sinks)
+    ph_pair = ph
+    if not pair_sinks:
+        import copy
+        ph_pair = copy.deepcopy(ph)
+        ph_pair.sink_rate = 0.0
+    plain = Solver(cm, hydro, ph_pair, cu0)
+    fine_pair = None if pair_sinks else Solver(fm, hydro, ph_pair, u0)
     N = cm.num_blocks * cm.block_size
     dxc = 2.0 * cm.domain_radius / N
     per_round = loop_starts * (-(-loop_steps // loop_every))
@@ -440,10 +453,16 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
         plain.resample_from(fine)
         # pair_steps coarse steps of dtc against 4 pair_steps fine steps of dtc / 4:
         # the target is the mean rate of the difference over that window
-        dtc = min(plain.advance(0), 4.0 * fine.advance(0))
+        # (without sinks: the fine steps from a copy of the fine state, which
+        # itself goes on with its sinks)
+        fp = fine
+        if fine_pair is not None:
+            fine_pair.resample_from(fine)
+            fp = fine_pair
+        dtc = min(plain.advance(0), 4.0 * fp.advance(0))
         plain.advance_dt(dtc, pair_steps)
-        fine.advance_dt(dtc / 4.0, 4 * pair_steps)
-        truth.resample_from(fine)
+        fp.advance_dt(dtc / 4.0, 4 * pair_steps)
+        truth.resample_from(fp)
         return dtc * pair_steps
 
     # training pairs along the fine run
@@ -472,7 +491,7 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
         k = net.samples()[0]
         if k % 20 == 0:
             log(f"  {k} samples, t = {fine.time() / ORBIT:.3f} orbits")
-    for sv in (base, plain, truth):
+    for sv in (base, plain, truth) + ((fine_pair,) if fine_pair is not None else ()):
         sv.close()
     tpairs = time.perf_counter() - tp0
     nsamp, ms = net.samples()
@@ -532,12 +551,50 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
                      "skipped_pairs": skipped, "loop_rounds": loop_info, "rollout_rounds": rollout_info}}
 
 
+def _perturbed(solver, rel, seed):
+    """The solver's committed state with every conserved value multiplied by
+    (1 + rel n), n uniform in [-1, 1] from a seeded generator (time,
+    iteration, block sums and torque totals kept)."""
+    st = solver.get_state()
+    rng = np.random.default_rng(seed)
+    u = st["conserved"] * (1.0 + rel * rng.uniform(-1.0, 1.0, st["conserved"].shape))
+    solver.set_state(u, time=st["time"], iteration=st["iteration"], ist=st["ist"], oec=st["oec"],
+                     torque=solver.torque())
+
+
+def mdot_spread(fine, snap, coarse_mesh, hydro, physics, cu0, t_end, runs, rel=1e-10, log=print):
+    """Time-averaged accretion rates of `runs` plain fine runs (the fine
+    solver restored from snap, its state perturbed by _perturbed with seeds
+    1..runs) and of plain coarse runs from each perturbed state
+    down-sampled, over [snap's time, t_end], stepped as learned_vs_fine steps
+    its plain runs.  Returns the rates and their relative spread (max - min
+    over the mean) for both grids."""
+    from .solver import Solver
+    fr, cr = [], []
+    coarse = Solver(coarse_mesh, hydro, physics, cu0)
+    try:
+        for k in range(1, runs + 1):
+            fine.restore(snap)
+            _perturbed(fine, rel, k)
+            coarse.resample_from(fine)
+            for rates, s in ((cr, coarse), (fr, fine)):
+                m0, ta = accreted(s), s.time()
+                run_to(lambda v: 10 * v.advance(10), s, t_end, fold=10)
+                rates.append((accreted(s) - m0) / (s.time() - ta))
+            log(f"spread run {k}: fine mdot {fr[-1]:.6e}, coarse mdot {cr[-1]:.6e}")
+    finally:
+        coarse.close()
+    spread = lambda r: float((max(r) - min(r)) / np.mean(r))
+    return {"rel_perturbation": rel, "fine_mdot": fr, "coarse_mdot": cr, "fine_rel_spread": spread(fr),
+            "coarse_rel_spread": spread(cr)}
+
+
 def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, sample_every=100,
                     max_samples=200, train_steps=4000, lr=1e-3, seed=5, limiter=0.1, train_mask=0.05,
                     pair_steps=1, variants=((0.1, 1.0, 0.05),), heldout_pairs=16, loop_rounds=0, loop_starts=8,
                     loop_steps=200, loop_every=10, loop_train_steps=2000, burn_in=0, learned_fold=10,
                     rollout_rounds=0, rollout_steps=200, rollout_every=20, rollout_train_steps=2000,
-                    rollout_gain=0.1, rollout_smooth=8, log=print):
+                    rollout_gain=0.1, rollout_smooth=8, pair_sinks=True, spread_runs=0, log=print):
     """Train a closure on the fine workload (train_on_fine), measure it on the
     fine states that follow the training window (heldout_one_step, from a
     device snapshot of the fine run), then run plain fine, plain coarse and
@@ -550,7 +607,13 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
     accretion rate and torque on the binary over the window, and for the
     coarse runs the L1 error of Sigma at the window's end against the
     down-sampled fine state, with the relative errors of the rates against
-    the fine run."""
+    the fine run.  With spread_runs > 0, that many more plain fine and plain
+    coarse runs over the same window start from the window's starting state
+    with a seeded relative perturbation of 1e-10 in every conserved field:
+    their time-averaged accretion rates (`mdot_spread`) show how far a run's
+    own rate moves when only the rounding of its start differs (the flow is
+    chaotic over many orbits), the scale against which an error of a few
+    percent in a 20-orbit mean is to be read."""
     from .solver import Snapshot, Solver, resample_state
 
     t = train_on_fine(fine_w, coarse_blocks=coarse_blocks, train_orbits=train_orbits, sample_every=sample_every,
@@ -560,7 +623,7 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
                       loop_train_steps=loop_train_steps, burn_in=burn_in, rollout_rounds=rollout_rounds,
                       rollout_steps=rollout_steps, rollout_every=rollout_every,
                       rollout_train_steps=rollout_train_steps, rollout_gain=rollout_gain,
-                      rollout_smooth=rollout_smooth, log=log)
+                      rollout_smooth=rollout_smooth, pair_sinks=pair_sinks, log=log)
     fine, net, cm, hydro, ph, cu0 = t["fine"], t["net"], t["coarse_mesh"], t["hydro"], t["physics"], t["cu0"]
     fm = fine_w["mesh"]
     N = cm.num_blocks * cm.block_size
@@ -583,6 +646,10 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
     coarse = Solver(cm, hydro, ph, cu0)
     coarse.resample_from(fine)
     out["window_orbits"] = [t_start / ORBIT, t_end / ORBIT]
+    snap0 = None
+    if spread_runs > 0:
+        snap0 = Snapshot(fine)
+        fine.save(snap0)
     # plain runs step as the reference's driver does (fold = 10 steps per
     # advance at one dt); the learned runs correct every step, learned_fold
     # corrected steps per call (one replayed graph)
@@ -622,6 +689,9 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
         elif sig_fine is not None:
             out[name]["sigma_l1_rel"] = float(np.abs(u[..., 0] - sig_fine).sum() / np.abs(sig_fine).sum())
         log(f"{name}: {n} calls, {wall:.2f} s, mdot {mdot:.6e}, torque {torque:.6e}")
+    if snap0 is not None:
+        out["mdot_spread"] = mdot_spread(fine, snap0, cm, hydro, ph, cu0, t_end, spread_runs, log=log)
+        snap0.close()
     if "mdot" in out.get("fine", {}):
         ref, reft = out["fine"]["mdot"], out["fine"]["torque"]
         for name in [r[0] for r in runs[1:]]:

@@ -241,7 +241,8 @@ def run_learned(args):
                         burn_in=args.burn_in, learned_fold=args.learned_fold, rollout_rounds=args.rollout_rounds,
                         rollout_steps=args.rollout_steps, rollout_every=args.rollout_every,
                         rollout_train_steps=args.rollout_train_steps, rollout_gain=args.rollout_gain,
-                        rollout_smooth=args.rollout_smooth, log=log)
+                        rollout_smooth=args.rollout_smooth, pair_sinks=not args.pair_no_sinks,
+                        seed=args.closure_seed, spread_runs=args.mdot_spread, log=log)
     # the headline: the fastest learned variant within 2% accretion-rate error
     ok = [k for k, v in r.items() if k.startswith("learned") and isinstance(v, dict)
           and v.get("mdot_rel_err", 1.0) <= 0.02]
@@ -286,6 +287,11 @@ def main():
     ap.add_argument("--learned-fold", type=int, default=10,
                     help="corrected coarse steps per call of the learned runs")
     ap.add_argument("--eval-orbits", type=float, default=2.0)
+    ap.add_argument("--closure-seed", type=int, default=5, help="seed of the net's initial weights and batches")
+    ap.add_argument("--mdot-spread", type=int, default=0,
+                    help="plain fine and coarse runs from perturbed copies of the window's start")
+    ap.add_argument("--pair-no-sinks", action="store_true",
+                    help="one-step training pairs stepped with the sinks off in both grids")
     ap.add_argument("--train-steps", type=int, default=4000)
     ap.add_argument("--pair-steps", type=int, default=1, help="coarse steps per training pair")
     ap.add_argument("--sample-every", type=int, default=100, help="fine steps between training pairs")
