@@ -134,9 +134,10 @@ __device__ __forceinline__ void march_face(const FaceConsts &c, bool alpha, cons
     if constexpr (NV == 3) cs2f = (c.m1 * dev::frsq(r1s + c.rs2) + c.m2 * dev::frsq(r2s + c.rs2)) * c.inv_mach2;
     double f = 0.5;
     if (alpha) {
-        // 1 / sqrt(m1/q1^3 + m2/q2^3), q = r + 1e-12 dx
-        const double q1 = dev::fsqrt(r1s) + c.eps_dx;
-        const double q2 = dev::fsqrt(r2s) + c.eps_dx;
+        // 1 / sqrt(m1/q1^3 + m2/q2^3), q = r + 1e-12 dx (r = 0 gives q =
+        // 1e-150 + 1e-12 dx, which rounds to 1e-12 dx as the reference's does)
+        const double q1 = dev::fsqrt_nn(r1s) + c.eps_dx;
+        const double q2 = dev::fsqrt_nn(r2s) + c.eps_dx;
         const double c1 = q1 * q1 * q1, c2 = q2 * q2 * q2;
         // rsq(X / Y) = Y rsq(X Y), X = m1 q2^3 + m2 q1^3, Y = q1^3 q2^3:
         // one transcendental instead of two
