@@ -584,7 +584,9 @@ def mdot_spread(fine, snap, coarse_mesh, hydro, physics, cu0, t_end, runs, rel=1
             log(f"spread run {k}: fine mdot {fr[-1]:.6e}, coarse mdot {cr[-1]:.6e}")
     finally:
         coarse.close()
-    spread = lambda r: float((max(r) - min(r)) / np.mean(r))
+    def spread(r):
+        m = float(np.mean(r))
+        return float((max(r) - min(r)) / m) if m != 0.0 else 0.0
     return {"rel_perturbation": rel, "fine_mdot": fr, "coarse_mdot": cr, "fine_rel_spread": spread(fr),
             "coarse_rel_spread": spread(cr)}
 

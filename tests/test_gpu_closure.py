@@ -476,3 +476,63 @@ def test_trained_closure_gains_on_heldout_states_2048_to_512():
     finally:
         t["net"].close()
         t["fine"].close()
+
+
+def test_sink_free_pairs_gain_on_transport():
+    """Pairs stepped with the sinks off in both grids (pair_sinks=False):
+    the net trains, and on the fine states that follow the training window,
+    stepped without sinks in all three runs, the corrected coarse step lands
+    closer to the down-sampled fine run than the plain one."""
+    import copy
+    from cdc_amd import workloads
+    from cdc_amd.closure import heldout_one_step, train_on_fine
+    t = train_on_fine(workloads.get("iso256"), coarse_blocks=2, train_orbits=1.0, sample_every=20,
+                      max_samples=80, train_steps=2000, pair_sinks=False, log=lambda m: None)
+    try:
+        assert t["info"]["samples"] > 0
+        assert t["info"]["loss_last200"] < 0.2 * t["info"]["loss_first50"]
+        ph = copy.deepcopy(t["physics"])
+        ph.sink_rate = 0.0
+        t["fine"].set_physics(ph)
+        h = heldout_one_step(t["fine"], t["net"], t["coarse_mesh"], t["hydro"], ph, t["cu0"], pairs=12)
+        print("sink-free held-out one-step L1, learned / plain:", h["ratio"])
+        assert all(r < 0.9 for r in h["ratio"]), h
+    finally:
+        t["net"].close()
+        t["fine"].close()
+
+
+def test_mdot_spread_of_perturbed_starts():
+    """mdot_spread: plain fine and coarse runs from copies of one state
+    perturbed by 1e-10 relative; the perturbation keeps time, iteration and
+    block sums, moves the state by at most 1e-10 relative, and the runs'
+    accretion rates over a short window agree closely."""
+    from cdc_amd import Mesh, Snapshot, workloads
+    from cdc_amd.closure import ORBIT, _perturbed, mdot_spread
+    from cdc_amd.solver import resample_state
+    w = workloads.get("iso256")
+    u0 = workloads.initial_state(w)
+    cm = Mesh(2, 32, w["mesh"].domain_radius)
+    cu0, _, _ = resample_state(w["mesh"], cm, u0)
+    fine = Solver(w["mesh"], w["hydro"], w["physics"], u0)
+    try:
+        fine.advance(40)
+        s0 = fine.get_state()
+        snap = Snapshot(fine)
+        fine.save(snap)
+        _perturbed(fine, 1e-10, 1)
+        s1 = fine.get_state()
+        assert s1["time"] == s0["time"] and s1["iteration"] == s0["iteration"]
+        np.testing.assert_array_equal(s1["ist"], s0["ist"])
+        rel = np.abs(s1["conserved"] - s0["conserved"]) / np.maximum(np.abs(s0["conserved"]), 1e-300)
+        assert 0.0 < rel.max() <= 1e-10 * (1 + 1e-6)
+        r = mdot_spread(fine, snap, cm, w["hydro"], w["physics"], cu0, s0["time"] + 0.05 * ORBIT, 2,
+                        log=lambda m: None)
+        snap.close()
+        assert len(r["fine_mdot"]) == 2 and len(r["coarse_mdot"]) == 2
+        # (the 64^2 grid's cell centres all lie outside the 0.05 sink radius:
+        # its sinks take nothing)
+        assert all(m > 0 for m in r["fine_mdot"]) and all(m >= 0 for m in r["coarse_mdot"])
+        assert r["fine_rel_spread"] < 1e-6, r
+    finally:
+        fine.close()
