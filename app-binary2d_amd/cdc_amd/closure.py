@@ -591,6 +591,70 @@ def mdot_spread(fine, snap, coarse_mesh, hydro, physics, cu0, t_end, runs, rel=1
             "coarse_rel_spread": spread(cr)}
 
 
+def _sink_rates(mesh, physics, t):
+    """Physics::sink_kernel (src/physics.rs:388-397) of both bodies at time t
+    at every cell centre of mesh, [N][N] (i-major)."""
+    from .solver import orbital_state_from_time
+    N = mesh.num_blocks * mesh.block_size
+    dx = 2.0 * mesh.domain_radius / N
+    x = -mesh.domain_radius + dx * (np.arange(N) + 0.5)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    st = orbital_state_from_time(physics, t)
+    s2 = physics.sink_radius ** 2
+    k = np.zeros_like(X)
+    for b in range(2):
+        r2 = (X - st[b, 1]) ** 2 + (Y - st[b, 2]) ** 2
+        k += physics.sink_rate * np.where(r2 < s2, (1.0 - r2 / s2) ** 2, 0.0)
+    return k, dx
+
+
+def sink_quadrature(fine_w, coarse_blocks=8, t0_orbits=2.0, t1_orbits=22.0, every_orbits=0.25, log=print):
+    """How the coarse grid's sinks sample a density field, apart from its
+    transport: along the plain fine run over [t0_orbits, t1_orbits], at the
+    end of the first 10-step call past every `every_orbits`, the
+    mass-removal rate of the sinks (sum over cells of sink rate x Sigma x
+    cell area) on the fine grid, and on the coarse grid (4x coarser) with the
+    fine Sigma down-sampled.  (The samples fall where the calls end, at
+    orbital phases that drift: sampling at exact fractions of an orbit would
+    put a circular binary at the same few places on the grid every time.)  A
+    closure that reproduced the down-sampled fine state exactly would accrete
+    at the coarse rate; its ratio to the fine rate is the bias left to it.
+    Returns the per-state rates, their ratios and the ratio of the window
+    means."""
+    from .config import Mesh
+    from .solver import Solver, resample_state
+    from . import workloads
+    fm, hydro, ph = fine_w["mesh"], fine_w["hydro"], fine_w["physics"]
+    cm = Mesh(coarse_blocks, fm.num_blocks * fm.block_size // (4 * coarse_blocks), fm.domain_radius)
+    fine = Solver(fm, hydro, ph, workloads.initial_state(fine_w))
+    rows = []
+    try:
+        t_next = t0_orbits * ORBIT
+        while t_next <= t1_orbits * ORBIT * (1.0 + 1e-12):
+            fine.advance(10)
+            if fine.time() < t_next:
+                continue
+            t = fine.time()
+            u = fine.get_state()["conserved"]
+            uc, _, _ = resample_state(fm, cm, u)
+            kf, dxf = _sink_rates(fm, ph, t)
+            kc, dxc = _sink_rates(cm, ph, t)
+            mf = float((kf * _global(u, fm.num_blocks, fm.block_size)[..., 0]).sum() * dxf * dxf)
+            mc = float((kc * _global(uc, cm.num_blocks, cm.block_size)[..., 0]).sum() * dxc * dxc)
+            rows.append([t / ORBIT, mf, mc])
+            if len(rows) % 8 == 0:
+                log(f"  t = {t / ORBIT:.2f} orbits: fine {mf:.4e}, coarse sampling {mc:.4e}")
+            t_next += every_orbits * ORBIT
+    finally:
+        fine.close()
+    a = np.array(rows)
+    r = a[:, 2] / np.where(a[:, 1] > 0, a[:, 1], np.nan)
+    return {"orbits": [t0_orbits, t1_orbits], "every_orbits": every_orbits, "states": len(rows),
+            "fine_rate": a[:, 1].tolist(), "coarse_sampling_rate": a[:, 2].tolist(), "time_orbits": a[:, 0].tolist(),
+            "ratio_min": float(np.nanmin(r)), "ratio_max": float(np.nanmax(r)),
+            "ratio_of_means": float(a[:, 2].mean() / a[:, 1].mean())}
+
+
 def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, sample_every=100,
                     max_samples=200, train_steps=4000, lr=1e-3, seed=5, limiter=0.1, train_mask=0.05,
                     pair_steps=1, variants=((0.1, 1.0, 0.05),), heldout_pairs=16, loop_rounds=0, loop_starts=8,

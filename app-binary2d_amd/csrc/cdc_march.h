@@ -86,13 +86,12 @@ struct StressCoef {
 // formed as p (alpha / sqrt(gamma)) gamma.
 template <int NV>
 __device__ __forceinline__ void stress_factors(const StageConsts &k, const StressCoef &sc, bool alpha,
-                                               const double *p, double id, const double *gx, const double *gy,
-                                               double &ax, double &ay, double &b) {
+                                               const double *p, const double *gx, const double *gy, double &ax,
+                                               double &ay, double &b) {
     double K = p[0] * k.visc;
     if constexpr (NV == 4) {
         if (alpha) K = p[3] * sc.vg;
     }
-    (void)id;
     ax = K * (gx[1] * sc.nx + gy[2] * sc.ty);
     ay = K * (gy[2] * sc.ny + gx[1] * sc.tx);
     b = K * (gx[2] * k.idx + gy[1] * k.idy);
@@ -331,8 +330,8 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             gyR[v] = dev::plm(tyR, p[1][v], p[2][v], p[3][v]);
         }
         double axL, ayL, bL, axR, ayR, bR;
-        stress_factors<NV>(k, sc, alpha, p[1], id[1], gxL, gyL, axL, ayL, bL);
-        stress_factors<NV>(k, sc, alpha, p[2], id[2], gxR, gyR, axR, ayR, bR);
+        stress_factors<NV>(k, sc, alpha, p[1], gxL, gyL, axL, ayL, bL);
+        stress_factors<NV>(k, sc, alpha, p[2], gxR, gyR, axR, ayR, bR);
         double pl[NV], pr[NV], F[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -403,7 +402,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             XLa[v] = Pa[v] + gx[v] * sg;
         }
         double ay_unused;
-        stress_factors<NV>(k, sc, alpha, Pa, ida, gx, gy, axa, ay_unused, ba);
+        stress_factors<NV>(k, sc, alpha, Pa, gx, gy, axa, ay_unused, ba);
     }
 
     // ---- the march ------------------------------------------------------------
@@ -475,7 +474,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             gy[v] = dev::plm(ty, PL[v], Pb[v], PR[v]);
         }
         double axb, ayb, bb;
-        stress_factors<NV>(k, sc, alpha, Pb, idb, gx, gy, axb, ayb, bb);
+        stress_factors<NV>(k, sc, alpha, Pb, gx, gy, axb, ayb, bb);
         // x-face between rows a and b (block-local vertex ri0 + t going
         // down, ri0 + H - t going up)
         double Fxb[NV];

@@ -225,6 +225,20 @@ def run_ensemble(args):
     print(json.dumps(line), flush=True)
 
 
+def run_sink_quadrature(args):
+    """The coarse grid's sink sampling of the fine run's density (see
+    closure.sink_quadrature), as one JSON line."""
+    from cdc_amd import workloads
+    from cdc_amd.closure import sink_quadrature
+    w = workloads.get("iso2048")
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    r = sink_quadrature(w, t0_orbits=args.train_orbits, t1_orbits=args.train_orbits + args.eval_orbits, log=log)
+    print(json.dumps({"metric": "coarse (512^2) sink sampling of the down-sampled 2048^2 density / 2048^2 sinks",
+                      "value": r["ratio_of_means"], "unit": "ratio of window means", "n_gpus": 1,
+                      "data": "synthetic (seeded cavity disk, seed %d)" % w["seed"],
+                      "config": {"workload": "sinkq", "fine": "iso2048", **r}}), flush=True)
+
+
 def run_learned(args):
     """Learned 512^2 closure vs plain 2048^2 (and plain 512^2), isothermal:
     wall time per orbit, accretion rate, torque and Sigma error over a
@@ -315,6 +329,9 @@ def main():
         return
     if args.workload == "learned512":
         run_learned(args)
+        return
+    if args.workload == "sinkq":
+        run_sink_quadrature(args)
         return
     from cdc_amd import HydroError, Solver, workloads
     w = workloads.get(args.workload)
