@@ -156,3 +156,23 @@ def test_corrected_advance_with_strips():
     assert moved > 0.0
     assert err <= 1e-3 * moved, (err, moved)
     net.close()
+
+
+@pytest.mark.parametrize("rows", [32, 16])
+@pytest.mark.parametrize("kind,kw", [("iso", dict(rk_order="RK2")),
+                                     ("euler", dict(rk_order="RK3", viscosity={"alpha": 0.1},
+                                                    binary_eccentricity=0.5, binary_mass_ratio=0.5))],
+                         ids=["iso", "euler"])
+def test_wide_blocks_alternating_segments(kind, kw, rows):
+    """Blocks of 128^2: two 64-column strips across each block and 4 (32-row
+    strips) or 8 (16-row strips) row segments that march down and up in
+    turn, against the oracle after every call."""
+    mesh, hydro, ph = H.small(kind, nb=2, bs=128, R=4.0, **kw)
+    u0 = H.initial(mesh, hydro, ph)
+    g, o = pair(mesh, hydro, ph, u0)
+    g.set_kernel(rows, 64, 64)
+    assert g.get_kernel() == (rows, 64, 64)
+    for step in range(4):
+        fold = 1 if step < 2 else 3
+        assert g.advance(fold) == pytest.approx(o.advance(fold), rel=1e-12)
+        check_states(g, o)
