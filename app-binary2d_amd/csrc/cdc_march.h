@@ -5,7 +5,8 @@
 // One RK stage (the P, F and U phases of try_advance_rk, src/scheme.rs:88-173)
 // over a strip of 64 columns x H rows inside one reference block (block_size
 // a multiple of 64), one wave per strip.  Lane l owns column cj0 + l and
-// marches down the strip's rows (x): the primitives of the last three rows,
+// marches along the strip's rows (x; down the block, or up it for every
+// second row segment of a plain run): the primitives of the last three rows,
 // the left state and stress factors of the previous row's x-face, that face's
 // flux and the previous row's y-flux difference stay in registers, so every
 // cell is converted to primitives once, its PLM gradients and stress factors
@@ -21,12 +22,14 @@
 // is taken inside the march.
 //
 // The arithmetic is the reference's (src/scheme.rs:201-316, src/physics.rs:
-// 322-340, 584-607, 776-803) with one regrouping: a face's viscous flux is
-// 0.5 f (A_L + A_R), A the cell's stress component times its density and the
-// cell-local part of nu (constant nu, or alpha times the cell's c_s^2 for
-// gamma-law gas), f the face-local factor (alpha's 1/sqrt(m1/q1^3 + m2/q2^3),
-// times the face's c_s^2 for isothermal gas), evaluated once per face.  This
-// differs from the reference's (sigma_L + sigma_R) / 2 only by rounding.
+// 322-340, 584-607, 776-803) with regroupings that differ from it only by
+// rounding: a face's viscous flux is 0.5 f (A_L + A_R), A the cell's stress
+// component times its density and the cell-local part of nu (constant nu,
+// or alpha times the cell's c_s^2 for gamma-law gas), f the face-local
+// factor (alpha's 1/sqrt(m1/q1^3 + m2/q2^3), times the face's c_s^2 for
+// isothermal gas), evaluated once per face, in place of the reference's
+// (sigma_L + sigma_R) / 2; the HLLE flux as U_L A + U_R B plus the pressure
+// terms (march_face); the stress factors with folded coefficients.
 #pragma once
 
 #include <hip/hip_ext.h>
