@@ -379,7 +379,7 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
                   train_steps=4000, lr=1e-3, seed=5, limiter=0.1, train_mask=0.05, pair_steps=1,
                   loop_rounds=0, loop_starts=8, loop_steps=200, loop_every=10, loop_train_steps=2000, burn_in=0,
                   rollout_rounds=0, rollout_steps=200, rollout_every=20, rollout_train_steps=2000, rollout_gain=0.1,
-                  rollout_smooth=8, pair_sinks=True, log=print):
+                  rollout_smooth=8, pair_sinks=True, pair_start_orbits=0.0, log=print):
     """Train a closure for the coarse grid (coarse_blocks^2 blocks, 4x coarser
     than the fine workload) on pairs taken every sample_every fine steps along
     the fine run up to train_orbits: each pair compares pair_steps coarse steps
@@ -400,7 +400,9 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
     sinks): a flux correction can only move mass, so a target that holds
     the two grids' different sink removals would teach it to move mass out
     of the sink cells (or into them) instead of correcting the transport;
-    the coarse run's own sinks then do the accreting.  Returns a dict with
+    the coarse run's own sinks then do the accreting.  Pairs are taken only
+    from pair_start_orbits on (the fine run advances through the earlier
+    part without them).  Returns a dict with
     the fine solver at the end of the training window, the trained closure
     (limiter kappa and mask as given), the coarse mesh, hydro, physics, the
     coarse initial state, the trained parameters and output scale, and the
@@ -477,7 +479,7 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
             snaps.append(Snapshot(fine))
             fine.save(snaps[-1])
             t_snap += snap_dt
-        if net.samples()[0] >= max_samples:
+        if net.samples()[0] >= max_samples or fine.time() < pair_start_orbits * ORBIT:
             continue
         try:
             dtc = pair(base, plain, truth, fine)
@@ -660,7 +662,8 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
                     pair_steps=1, variants=((0.1, 1.0, 0.05),), heldout_pairs=16, loop_rounds=0, loop_starts=8,
                     loop_steps=200, loop_every=10, loop_train_steps=2000, burn_in=0, learned_fold=10,
                     rollout_rounds=0, rollout_steps=200, rollout_every=20, rollout_train_steps=2000,
-                    rollout_gain=0.1, rollout_smooth=8, pair_sinks=True, spread_runs=0, log=print):
+                    rollout_gain=0.1, rollout_smooth=8, pair_sinks=True, pair_start_orbits=0.0, spread_runs=0,
+                    log=print):
     """Train a closure on the fine workload (train_on_fine), measure it on the
     fine states that follow the training window (heldout_one_step, from a
     device snapshot of the fine run), then run plain fine, plain coarse and
@@ -689,7 +692,8 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
                       loop_train_steps=loop_train_steps, burn_in=burn_in, rollout_rounds=rollout_rounds,
                       rollout_steps=rollout_steps, rollout_every=rollout_every,
                       rollout_train_steps=rollout_train_steps, rollout_gain=rollout_gain,
-                      rollout_smooth=rollout_smooth, pair_sinks=pair_sinks, log=log)
+                      rollout_smooth=rollout_smooth, pair_sinks=pair_sinks, pair_start_orbits=pair_start_orbits,
+                      log=log)
     fine, net, cm, hydro, ph, cu0 = t["fine"], t["net"], t["coarse_mesh"], t["hydro"], t["physics"], t["cu0"]
     fm = fine_w["mesh"]
     N = cm.num_blocks * cm.block_size
