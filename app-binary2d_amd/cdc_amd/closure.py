@@ -669,9 +669,10 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
     device snapshot of the fine run), then run plain fine, plain coarse and
     learned coarse runs over the held-out window
     [train_orbits, train_orbits + eval_orbits] from the same fine state; one
-    learned run per (limiter kappa, damping, mask density[, mass only])
-    variant, the damping scaling the trained output and a nonzero fourth
-    entry keeping only the mass-flux correction.  Every run ends exactly at the
+    learned run per (limiter kappa, damping, mask density[, components])
+    variant, the damping scaling the trained output and the fourth entry
+    keeping only the mass-flux correction (1) or only the momentum-flux
+    corrections (2).  Every run ends exactly at the
     window's end.  Reported per run: wall time per orbit, the time-averaged
     accretion rate and torque on the binary over the window, and for the
     coarse runs the L1 error of Sigma at the window's end against the
@@ -727,7 +728,7 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
             ("coarse", coarse, lambda s: 10 * s.advance(10), None)]
     for v in variants:
         kappa, damping, mask, mass_only = (tuple(v) + (0.0, 0.0))[:4]
-        name = f"learned_k{kappa:g}_d{damping:g}_m{mask:g}" + ("_mass" if mass_only else "")
+        name = f"learned_k{kappa:g}_d{damping:g}_m{mask:g}" + {0: "", 1: "_mass", 2: "_mom"}[int(mass_only)]
         runs.append((name, None, lambda s: learned_fold * net.advance(s, learned_fold),
                      (kappa, damping, mask, mass_only)))
     sig_fine = None
@@ -736,7 +737,9 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
             net.close()
             net = Closure(N, cm.domain_radius, limiter=var[0], mask_density=var[2])
             net.set_params(params)
-            comp = np.array([1.0, 0.0, 0.0]) if var[3] else np.ones(3)
+            # the corrected flux components: all, the mass flux only (1) or
+            # the momentum fluxes only (2)
+            comp = {0: np.ones(3), 1: np.array([1.0, 0.0, 0.0]), 2: np.array([0.0, 1.0, 1.0])}[int(var[3])]
             net.set_scales(output_scale=trained_scale * var[1] * comp)
             solver = Solver(cm, hydro, ph, cu0)
             solver.resample_from(start)

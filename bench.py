@@ -318,7 +318,8 @@ def main():
     ap.add_argument("--burn-in", type=int, default=0,
                     help="coarse steps beside the fine run before each pair (tendency targets)")
     ap.add_argument("--variants", default="0.02:0.25:0.05,0.02:0.5:0.05,0.05:0.5:0.05",
-                    help="learned runs as kappa:damping:mask_density[:mass_only],...")
+                    help="learned runs as kappa:damping:mask_density[:components],...; components 1: the "
+                         "mass flux only, 2: the momentum fluxes only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU only; --gpus {args.gpus} is not supported (the mesh would "
