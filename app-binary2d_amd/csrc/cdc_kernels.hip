@@ -553,7 +553,7 @@ __global__ void __launch_bounds__(256) column_max_kernel(const double *x, int n,
 // 64-column, 64-thread shapes are the column-marching kernel's strips.
 static const TileShape kShapes[] = {{16, 32, 256}, {32, 16, 256}, {16, 16, 256}, {8, 32, 256},
                                     {32, 8, 256},  {8, 16, 128},  {16, 8, 128},  {8, 8, 64},
-                                    {32, 64, 64},  {16, 64, 64}};
+                                    {32, 64, 64},  {16, 64, 64}, {8, 64, 64}};
 
 int tile_shapes(const TileShape **out) {
     *out = kShapes;

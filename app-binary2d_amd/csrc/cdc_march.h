@@ -1,6 +1,6 @@
 // cdc_march.h -- the column-marching form of the fused stage kernel (the
-// kernel template; cdc_march.hip and cdc_march16.hip instantiate it for
-// strips of 32 and 16 rows).
+// kernel template; cdc_march.hip, cdc_march16.hip and cdc_march8.hip
+// instantiate it for strips of 32, 16 and 8 rows).
 //
 // One RK stage (the P, F and U phases of try_advance_rk, src/scheme.rs:88-173)
 // over a strip of 64 columns x H rows inside one reference block (block_size
@@ -763,10 +763,14 @@ hipError_t launch_march_h(int nv, int cool, int visc, int read_base, const Stage
 
 }  // namespace
 
-// Strips of 16 rows (cdc_march16.hip).
+// Strips of 16 rows (cdc_march16.hip) and of 8 rows (cdc_march8.hip).
 hipError_t launch_march_rows16(int nv, int cool, int visc, int read_base, const StageArgs *args,
                                const int32_t *begin, int nruns, int ntiles, const StageArgs *fin,
                                const int32_t *fin_begin, int fin_nruns, int nblk, hipEvent_t ev0, hipEvent_t ev1,
                                hipStream_t s);
+hipError_t launch_march_rows8(int nv, int cool, int visc, int read_base, const StageArgs *args,
+                              const int32_t *begin, int nruns, int ntiles, const StageArgs *fin,
+                              const int32_t *fin_begin, int fin_nruns, int nblk, hipEvent_t ev0, hipEvent_t ev1,
+                              hipStream_t s);
 
 }  // namespace cdc

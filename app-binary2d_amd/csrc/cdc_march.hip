@@ -28,6 +28,9 @@ hipError_t launch_march_batch(int nv, int rows, int cool, int visc, int read_bas
     if (rows == 16)
         return launch_march_rows16(nv, cool, visc, read_base, args, begin, nruns, ntiles, fin, fin_begin, fin_nruns,
                                    nblk, ev0, ev1, s);
+    if (rows == 8)
+        return launch_march_rows8(nv, cool, visc, read_base, args, begin, nruns, ntiles, fin, fin_begin, fin_nruns,
+                                  nblk, ev0, ev1, s);
     return hipErrorInvalidValue;
 }
 

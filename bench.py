@@ -160,7 +160,7 @@ def run_ensemble(args):
         big.autotune(reps=3)
         cands = [big.get_kernel()]
         if all(g.mesh.block_size % 64 == 0 for g in solvers):
-            cands += [k for k in ((32, 64, 64), (16, 64, 64)) if k not in cands]
+            cands += [k for k in ((32, 64, 64), (16, 64, 64), (8, 64, 64)) if k not in cands]
         best = None
         for kern in cands:
             for g in solvers:

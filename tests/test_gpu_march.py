@@ -158,14 +158,14 @@ def test_corrected_advance_with_strips():
     net.close()
 
 
-@pytest.mark.parametrize("rows", [32, 16])
+@pytest.mark.parametrize("rows", [32, 16, 8])
 @pytest.mark.parametrize("kind,kw", [("iso", dict(rk_order="RK2")),
                                      ("euler", dict(rk_order="RK3", viscosity={"alpha": 0.1},
                                                     binary_eccentricity=0.5, binary_mass_ratio=0.5))],
                          ids=["iso", "euler"])
 def test_wide_blocks_alternating_segments(kind, kw, rows):
-    """Blocks of 128^2: two 64-column strips across each block and 4 (32-row
-    strips) or 8 (16-row strips) row segments that march down and up in
+    """Blocks of 128^2: two 64-column strips across each block and 4, 8 or 16
+    row segments (strips of 32, 16 or 8 rows) that march down and up in
     turn, against the oracle after every call."""
     mesh, hydro, ph = H.small(kind, nb=2, bs=128, R=4.0, **kw)
     u0 = H.initial(mesh, hydro, ph)

@@ -441,7 +441,7 @@ def test_every_tile_shape_matches_oracle(kind, shape):
 
 
 # the column-marching kernel's strips: 64 columns by 32 or 16 rows, one wave
-MARCH = [(32, 64, 64), (16, 64, 64)]
+MARCH = [(32, 64, 64), (16, 64, 64), (8, 64, 64)]
 MARCH_CASES = [
     ("iso", 2, dict(rk_order="RK2")),
     ("iso", 2, dict(rk_order="RK3", viscosity={"alpha": 0.1}, sink_model="torque_free", binary_eccentricity=0.4,
