@@ -105,6 +105,12 @@ struct TileShape {
 TileShape tile_shape_for(int bs, int nv);
 // the shapes of the column-marching kernel: 64-column strips, one wave each
 inline bool march_shape(const TileShape &ts) { return ts.ty == 64 && ts.nt == 64; }
+// whether the column-marching kernel can run an N x N grid of nv conserved
+// variables: it addresses its arrays through 32-bit buffer offsets, so each
+// (nv state planes, up to 3 aux planes) must hold under 2^32 bytes
+inline bool march_fits(int nv, int64_t N) {
+    return (uint64_t)(nv > 3 ? nv : 3) * (uint64_t)N * (uint64_t)N * 8u < (1ull << 32);
+}
 int tile_shapes(const TileShape **out);
 
 int nsum_for(int nv);

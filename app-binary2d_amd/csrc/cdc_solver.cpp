@@ -734,7 +734,8 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
         ALLOC(s->d_ist[i], (size_t)s->nblk * 7 * s->nv * sizeof(double));
         ALLOC(s->d_oec[i], (size_t)s->nblk * 7 * 4 * sizeof(double));
         ALLOC(s->d_tq[i], (size_t)s->nblk * 2 * sizeof(double));
-        if (s->bs % 64 == 0) ALLOC(s->d_edge[i], (size_t)s->nv * s->N * (s->N / 64) * 4 * sizeof(double));
+        if (s->bs % 64 == 0 && cdc::march_fits(s->nv, s->N))
+            ALLOC(s->d_edge[i], (size_t)s->nv * s->N * (s->N / 64) * 4 * sizeof(double));
     }
     s->ts_stride = (size_t)s->max_tiles * (s->nsum + 1);
     ALLOC(s->d_tile_sums, 2 * s->ts_stride * sizeof(double));
@@ -854,6 +855,10 @@ int32_t cdc_solver_set_kernel(cdc_solver_t *s, int32_t tx, int32_t ty, int32_t t
         if (!found) return set_msg(CDC_ERR_INVALID_ARGUMENT, "unsupported tile shape");
         if (s->bs % tx != 0 || s->bs % ty != 0)
             return set_msg(CDC_ERR_INVALID_ARGUMENT, "tile shape must divide block_size");
+        if (cdc::march_shape(t) && !cdc::march_fits(s->nv, s->N))
+            return set_msg(CDC_ERR_INVALID_ARGUMENT,
+                           "grid too large for the column-marching strips (arrays of 2^32 bytes or more): "
+                           "use a tile shape");
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->ts = t;
@@ -902,6 +907,7 @@ int32_t cdc_solver_autotune(cdc_solver_t *s, int32_t reps, int32_t *tx, int32_t 
     for (int i = 0; i < n && rc == CDC_OK; ++i) {
         const cdc::TileShape t = shapes[i];
         if (s->bs % t.tx != 0 || s->bs % t.ty != 0) continue;
+        if (cdc::march_shape(t) && !cdc::march_fits(s->nv, s->N)) continue;
         // stage launches only (no finalize) from the committed state into a
         // scratch buffer, in the costlier later-stage form
         cdc::StageArgs a;

@@ -294,10 +294,14 @@ int32_t cdc_max_signal_speed_noalloc(int32_t kind, double gamma_law_index,
 
 /* Tile shape (tx rows along x by ty columns along y, inside one block) of
  * the fused stage kernel.  Supported: 16x32, 32x16, 16x16, 8x32, 32x8 (256
- * threads), 8x16, 16x8 (128 threads), 8x8 (64 threads), each dividing
- * block_size; (0, 0) restores the default.  No reference counterpart: the reference has no tiles.
- * set_kernel / get_kernel also name the thread count (threads = 0: the
- * shape's only one). */
+ * threads), 8x16, 16x8 (128 threads), 8x8 (64 threads), and the
+ * column-marching strips 32x64, 16x64, 8x64 (64 threads), each dividing
+ * block_size; (0, 0) restores the default.  The strips need every array of
+ * the grid (conserved planes, aux planes) under 2^32 bytes
+ * (max(nv, 3) * N^2 * 8); on larger grids they are rejected with
+ * CDC_ERR_INVALID_ARGUMENT and the autotuner skips them.  No reference
+ * counterpart: the reference has no tiles.  set_kernel / get_kernel also
+ * name the thread count (threads = 0: the shape's only one). */
 int32_t cdc_solver_set_tile(cdc_solver_t *solver, int32_t tx, int32_t ty);
 int32_t cdc_solver_get_tile(cdc_solver_t *solver, int32_t *tx, int32_t *ty);
 int32_t cdc_solver_set_kernel(cdc_solver_t *solver, int32_t tx, int32_t ty, int32_t threads);

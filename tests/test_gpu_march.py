@@ -176,3 +176,30 @@ def test_wide_blocks_alternating_segments(kind, kw, rows):
         fold = 1 if step < 2 else 3
         assert g.advance(fold) == pytest.approx(o.advance(fold), rel=1e-12)
         check_states(g, o)
+
+
+def test_strips_rejected_on_grids_beyond_32_bit_offsets():
+    """The strips address their arrays through 32-bit buffer offsets: on a
+    grid whose arrays hold 2^32 bytes or more (gamma-law gas, 11,648^2 =
+    182 x 182 blocks of 64^2: 4.34 GB per array) set_kernel rejects them,
+    the autotuner picks a tile shape, and a tiled stage runs."""
+    from cdc_amd import CdcError, Euler, Mesh, workloads
+    mesh = Mesh(182, 64, 8.0)
+    hydro = Euler(gamma_law_index=5.0 / 3.0, cooling_coefficient=0.0)
+    ph = workloads.base_physics(sink_rate=0.0)   # gas at rest under the binary, no sinks
+    N = 182 * 64
+    assert 4 * N * N * 8 >= 2 ** 32
+    u0 = np.zeros((182 * 182, 64, 64, 4))
+    u0[..., 0] = 1.0
+    u0[..., 3] = 1.0 / (5.0 / 3.0 - 1.0)
+    g = Solver(mesh, hydro, ph, u0)
+    try:
+        with pytest.raises(CdcError):
+            g.set_kernel(32, 64, 64)
+        g.set_kernel(16, 32, 256)
+        assert g.get_kernel() == (16, 32, 256)
+        g.autotune(reps=1)
+        assert g.get_kernel()[1:] != (64, 64)
+        assert g.advance(1) > 0.0
+    finally:
+        g.close()
