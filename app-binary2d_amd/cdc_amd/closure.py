@@ -379,7 +379,8 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
                   train_steps=4000, lr=1e-3, seed=5, limiter=0.1, train_mask=0.05, pair_steps=1,
                   loop_rounds=0, loop_starts=8, loop_steps=200, loop_every=10, loop_train_steps=2000, burn_in=0,
                   rollout_rounds=0, rollout_steps=200, rollout_every=20, rollout_train_steps=2000, rollout_gain=0.1,
-                  rollout_smooth=8, pair_sinks=True, pair_start_orbits=0.0, log=print):
+                  rollout_smooth=8, pair_sinks=True, pair_start_orbits=0.0, extra_runs=(), main_pairs=True,
+                  log=print):
     """Train a closure for the coarse grid (coarse_blocks^2 blocks, 4x coarser
     than the fine workload) on pairs taken every sample_every fine steps along
     the fine run up to train_orbits: each pair compares pair_steps coarse steps
@@ -402,7 +403,12 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
     of the sink cells (or into them) instead of correcting the transport;
     the coarse run's own sinks then do the accreting.  Pairs are taken only
     from pair_start_orbits on (the fine run advances through the earlier
-    part without them).  Returns a dict with
+    part without them).  extra_runs (fine workloads on the same mesh, with
+    their own binary and gas parameters) add their one-step pairs first,
+    taken the same way along [pair_start_orbits, train_orbits] of each; with
+    main_pairs False the fine workload itself gives none (it is then only
+    advanced to train_orbits, and a closure evaluated on it has never seen
+    its parameters).  The sample capacity is shared equally.  Returns a dict with
     the fine solver at the end of the training window, the trained closure
     (limiter kappa and mask as given), the coarse mesh, hydro, physics, the
     coarse initial state, the trained parameters and output scale, and the
@@ -435,7 +441,7 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
     snaps = []
     snap_dt = t_snap = train_orbits * ORBIT / loop_starts if loop_rounds > 0 or rollout_rounds > 0 else None
 
-    def pair(base, plain, truth, fine):
+    def pair(base, plain, truth, fine, fine_pair=fine_pair):
         """Fill base, plain and truth for one training pair; returns its dt."""
         base.resample_from(fine)
         if burn_in > 0:
@@ -467,11 +473,50 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
         truth.resample_from(fp)
         return dtc * pair_steps
 
-    # training pairs along the fine run
     skipped = 0
     t_train = train_orbits * ORBIT
-    log(f"training pairs: fine {fm.num_blocks * fm.block_size}^2 to {train_orbits} orbits, coarse {N}^2")
     tp0 = time.perf_counter()
+    n_sources = len(extra_runs) + (1 if main_pairs else 0)
+    quota = max_samples // max(1, n_sources)
+    # one-step pairs from the other runs (their own gas and binary)
+    for w in extra_runs:
+        if w["mesh"].num_blocks * w["mesh"].block_size != fm.num_blocks * fm.block_size:
+            raise ValueError("extra training runs must share the fine mesh")
+        ew0 = workloads.initial_state(w)
+        ecu0, _, _ = resample_state(w["mesh"], cm, ew0)
+        eph_pair = w["physics"]
+        if not pair_sinks:
+            import copy
+            eph_pair = copy.deepcopy(w["physics"])
+            eph_pair.sink_rate = 0.0
+        ef = Solver(w["mesh"], w["hydro"], w["physics"], ew0)
+        eb, et = (Solver(cm, w["hydro"], w["physics"], ecu0) for _ in range(2))
+        ep = Solver(cm, w["hydro"], eph_pair, ecu0)
+        efp = None if pair_sinks else Solver(w["mesh"], w["hydro"], eph_pair, ew0)
+        start = net.samples()[0]
+        log(f"training pairs from a run with e {w['physics'].binary_eccentricity:.3f}, "
+            f"q {w['physics'].binary_mass_ratio:.3f}, Mach {w['hydro'].mach_number:.1f}")
+        try:
+            while ef.time() < t_train and net.samples()[0] - start < quota:
+                for _ in range(max(1, sample_every // 10)):
+                    ef.advance(10)
+                if ef.time() < pair_start_orbits * ORBIT:
+                    continue
+                try:
+                    dtc = pair(eb, ep, et, ef, fine_pair=efp)
+                except HydroError as e:
+                    skipped += 1
+                    log(f"  no pair at t = {ef.time() / ORBIT:.3f} orbits: {e}")
+                    continue
+                net.add_sample(eb, ep, et, dtc)
+        except HydroError as e:
+            log(f"  the run stopped at t = {ef.time() / ORBIT:.3f} orbits: {e}")
+        finally:
+            for sv in (ef, eb, et, ep) + ((efp,) if efp is not None else ()):
+                sv.close()
+    # training pairs along the fine run
+    log(f"training pairs: fine {fm.num_blocks * fm.block_size}^2 to {train_orbits} orbits, coarse {N}^2")
+    main_cap = net.samples()[0] + quota if main_pairs else 0
     while fine.time() < t_train:
         for _ in range(max(1, sample_every // 10)):
             fine.advance(10)
@@ -479,7 +524,7 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
             snaps.append(Snapshot(fine))
             fine.save(snaps[-1])
             t_snap += snap_dt
-        if net.samples()[0] >= max_samples or fine.time() < pair_start_orbits * ORBIT:
+        if net.samples()[0] >= min(max_samples, main_cap) or fine.time() < pair_start_orbits * ORBIT:
             continue
         try:
             dtc = pair(base, plain, truth, fine)
@@ -663,7 +708,7 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
                     loop_steps=200, loop_every=10, loop_train_steps=2000, burn_in=0, learned_fold=10,
                     rollout_rounds=0, rollout_steps=200, rollout_every=20, rollout_train_steps=2000,
                     rollout_gain=0.1, rollout_smooth=8, pair_sinks=True, pair_start_orbits=0.0, spread_runs=0,
-                    log=print):
+                    train_runs=(), log=print):
     """Train a closure on the fine workload (train_on_fine), measure it on the
     fine states that follow the training window (heldout_one_step, from a
     device snapshot of the fine run), then run plain fine, plain coarse and
@@ -683,7 +728,10 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
     their time-averaged accretion rates (`mdot_spread`) show how far a run's
     own rate moves when only the rounding of its start differs (the flow is
     chaotic over many orbits), the scale against which an error of a few
-    percent in a 20-orbit mean is to be read."""
+    percent in a 20-orbit mean is to be read.  With train_runs (fine
+    workloads on the same mesh) the closure is trained on their one-step
+    pairs only, and fine_w, whose parameters it has then never seen, serves
+    for the held-out states and the window."""
     from .solver import Snapshot, Solver, resample_state
 
     t = train_on_fine(fine_w, coarse_blocks=coarse_blocks, train_orbits=train_orbits, sample_every=sample_every,
@@ -694,7 +742,7 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
                       rollout_steps=rollout_steps, rollout_every=rollout_every,
                       rollout_train_steps=rollout_train_steps, rollout_gain=rollout_gain,
                       rollout_smooth=rollout_smooth, pair_sinks=pair_sinks, pair_start_orbits=pair_start_orbits,
-                      log=log)
+                      extra_runs=train_runs, main_pairs=not train_runs, log=log)
     fine, net, cm, hydro, ph, cu0 = t["fine"], t["net"], t["coarse_mesh"], t["hydro"], t["physics"], t["cu0"]
     fm = fine_w["mesh"]
     N = cm.num_blocks * cm.block_size

@@ -85,6 +85,27 @@ def ensemble_members(seed=4, counts=((24, 4), (24, 8), (16, 16)), block_size=64)
     return out
 
 
+def training_runs(count, seed=6, blocks=32, block_size=64):
+    """Fine isothermal runs for training the learned closure: the iso2048
+    mesh (32 x 32 blocks of 64^2) with binary and gas parameters drawn from
+    the 64-run study's ranges (e ~ U[0, 0.8], q ~ log-U[0.1, 1], Mach ~
+    U[10, 40], nu ~ log-U[1e-4, 1e-2]) by a seeded generator, each with its
+    own initial-state seed."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        e = float(rng.uniform(0.0, 0.8))
+        q = float(10.0 ** rng.uniform(-1.0, 0.0))
+        mach = float(rng.uniform(10.0, 40.0))
+        nu = float(10.0 ** rng.uniform(-4.0, -2.0))
+        out.append(dict(mesh=Mesh(blocks, block_size, 8.0), hydro=Isothermal(mach_number=mach),
+                        physics=base_physics(binary_eccentricity=e, binary_mass_ratio=q,
+                                             viscosity={"constant_nu": nu}),
+                        seed=1000 * seed + k, steps=None, orbits=None))
+    return out
+
+
 def get(name, **over):
     """Deep copy of a named workload; keyword overrides replace top-level keys."""
     w = copy.deepcopy(WORKLOADS[name])

@@ -257,7 +257,8 @@ def run_learned(args):
                         rollout_train_steps=args.rollout_train_steps, rollout_gain=args.rollout_gain,
                         rollout_smooth=args.rollout_smooth, pair_sinks=not args.pair_no_sinks,
                         seed=args.closure_seed, spread_runs=args.mdot_spread,
-                        pair_start_orbits=args.pair_start_orbits, log=log)
+                        pair_start_orbits=args.pair_start_orbits,
+                        train_runs=workloads.training_runs(args.train_runs) if args.train_runs else (), log=log)
     # the headline: the fastest learned variant within 2% accretion-rate error
     ok = [k for k, v in r.items() if k.startswith("learned") and isinstance(v, dict)
           and v.get("mdot_rel_err", 1.0) <= 0.02]
@@ -304,6 +305,8 @@ def main():
     ap.add_argument("--eval-orbits", type=float, default=2.0)
     ap.add_argument("--closure-seed", type=int, default=5, help="seed of the net's initial weights and batches")
     ap.add_argument("--pair-start-orbits", type=float, default=0.0, help="training pairs only from this time on")
+    ap.add_argument("--train-runs", type=int, default=0,
+                    help="train on this many 2048^2 runs with drawn parameters instead of the evaluated run")
     ap.add_argument("--mdot-spread", type=int, default=0,
                     help="plain fine and coarse runs from perturbed copies of the window's start")
     ap.add_argument("--pair-no-sinks", action="store_true",
