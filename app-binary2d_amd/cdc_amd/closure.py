@@ -380,7 +380,7 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
                   loop_rounds=0, loop_starts=8, loop_steps=200, loop_every=10, loop_train_steps=2000, burn_in=0,
                   rollout_rounds=0, rollout_steps=200, rollout_every=20, rollout_train_steps=2000, rollout_gain=0.1,
                   rollout_smooth=8, pair_sinks=True, pair_start_orbits=0.0, extra_runs=(), main_pairs=True,
-                  log=print):
+                  main_orbits=None, log=print):
     """Train a closure for the coarse grid (coarse_blocks^2 blocks, 4x coarser
     than the fine workload) on pairs taken every sample_every fine steps along
     the fine run up to train_orbits: each pair compares pair_steps coarse steps
@@ -407,8 +407,9 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
     their own binary and gas parameters) add their one-step pairs first,
     taken the same way along [pair_start_orbits, train_orbits] of each; with
     main_pairs False the fine workload itself gives none (it is then only
-    advanced to train_orbits, and a closure evaluated on it has never seen
-    its parameters).  The sample capacity is shared equally.  Returns a dict with
+    advanced, to main_orbits if given, else to train_orbits, and a closure
+    evaluated on it has never seen its parameters).  The sample capacity is
+    shared equally.  Returns a dict with
     the fine solver at the end of the training window, the trained closure
     (limiter kappa and mask as given), the coarse mesh, hydro, physics, the
     coarse initial state, the trained parameters and output scale, and the
@@ -475,6 +476,9 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
 
     skipped = 0
     t_train = train_orbits * ORBIT
+    if main_orbits is not None and main_pairs:
+        raise ValueError("main_orbits applies only when the fine workload gives no pairs")
+    t_main = t_train if main_orbits is None else main_orbits * ORBIT
     tp0 = time.perf_counter()
     n_sources = len(extra_runs) + (1 if main_pairs else 0)
     quota = max_samples // max(1, n_sources)
@@ -515,9 +519,12 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
             for sv in (ef, eb, et, ep) + ((efp,) if efp is not None else ()):
                 sv.close()
     # training pairs along the fine run
-    log(f"training pairs: fine {fm.num_blocks * fm.block_size}^2 to {train_orbits} orbits, coarse {N}^2")
+    if main_pairs:
+        log(f"training pairs: fine {fm.num_blocks * fm.block_size}^2 to {train_orbits} orbits, coarse {N}^2")
+    else:
+        log(f"the evaluated run to {t_main / ORBIT:g} orbits, without pairs")
     main_cap = net.samples()[0] + quota if main_pairs else 0
-    while fine.time() < t_train:
+    while fine.time() < t_main:
         for _ in range(max(1, sample_every // 10)):
             fine.advance(10)
         if snap_dt is not None and len(snaps) < loop_starts and fine.time() >= t_snap - snap_dt:
@@ -708,7 +715,7 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
                     loop_steps=200, loop_every=10, loop_train_steps=2000, burn_in=0, learned_fold=10,
                     rollout_rounds=0, rollout_steps=200, rollout_every=20, rollout_train_steps=2000,
                     rollout_gain=0.1, rollout_smooth=8, pair_sinks=True, pair_start_orbits=0.0, spread_runs=0,
-                    train_runs=(), log=print):
+                    train_runs=(), eval_start_orbits=None, net_seeds=(), log=print):
     """Train a closure on the fine workload (train_on_fine), measure it on the
     fine states that follow the training window (heldout_one_step, from a
     device snapshot of the fine run), then run plain fine, plain coarse and
@@ -731,7 +738,12 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
     percent in a 20-orbit mean is to be read.  With train_runs (fine
     workloads on the same mesh) the closure is trained on their one-step
     pairs only, and fine_w, whose parameters it has then never seen, serves
-    for the held-out states and the window."""
+    for the held-out states and the window, which then starts at
+    eval_start_orbits if given (the training runs may run longer).
+    net_seeds: more nets trained afresh from those seeds' initial weights on
+    the same one-step pairs (no loop or rollout rounds), each run over the
+    window in every variant (names suffixed _s<seed>); the plain runs are
+    shared."""
     from .solver import Snapshot, Solver, resample_state
 
     t = train_on_fine(fine_w, coarse_blocks=coarse_blocks, train_orbits=train_orbits, sample_every=sample_every,
@@ -742,8 +754,12 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
                       rollout_steps=rollout_steps, rollout_every=rollout_every,
                       rollout_train_steps=rollout_train_steps, rollout_gain=rollout_gain,
                       rollout_smooth=rollout_smooth, pair_sinks=pair_sinks, pair_start_orbits=pair_start_orbits,
-                      extra_runs=train_runs, main_pairs=not train_runs, log=log)
+                      extra_runs=train_runs, main_pairs=not train_runs,
+                      main_orbits=eval_start_orbits if train_runs else None, log=log)
+    if net_seeds and (loop_rounds or rollout_rounds):
+        raise ValueError("net_seeds retrain the one-step pairs only (no loop or rollout rounds)")
     fine, net, cm, hydro, ph, cu0 = t["fine"], t["net"], t["coarse_mesh"], t["hydro"], t["physics"], t["cu0"]
+    trainer = net
     fm = fine_w["mesh"]
     N = cm.num_blocks * cm.block_size
     params, trained_scale = t["params"], t["scale"]
@@ -774,17 +790,33 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
     # corrected steps per call (one replayed graph)
     runs = [("fine", fine, lambda s: 10 * s.advance(10), None),
             ("coarse", coarse, lambda s: 10 * s.advance(10), None)]
-    for v in variants:
-        kappa, damping, mask, mass_only = (tuple(v) + (0.0, 0.0))[:4]
-        name = f"learned_k{kappa:g}_d{damping:g}_m{mask:g}" + {0: "", 1: "_mass", 2: "_mom"}[int(mass_only)]
-        runs.append((name, None, lambda s: learned_fold * net.advance(s, learned_fold),
-                     (kappa, damping, mask, mass_only)))
+    seed_params = [(seed, "", params)]
+    for sd in net_seeds:
+        # a fresh net's initial weights for this seed, trained on the same pairs
+        init = Closure(N, cm.domain_radius, seed=sd, init_output_gain=0.1)
+        p0 = init.params()
+        init.close()
+        trainer.set_params(p0)
+        tt0 = time.perf_counter()
+        h = trainer.train(train_steps, lr=lr, seed=sd)
+        log(f"net seed {sd}: trained {train_steps} steps in {time.perf_counter() - tt0:.1f} s: "
+            f"loss {h[:50].mean():.4f} -> {h[-200:].mean():.4f}")
+        out[f"loss_last200_s{sd}"] = float(h[-200:].mean())
+        seed_params.append((sd, f"_s{sd}", trainer.params()))
+    for sd, suffix, prm in seed_params:
+        for v in variants:
+            kappa, damping, mask, mass_only = (tuple(v) + (0.0, 0.0))[:4]
+            name = (f"learned_k{kappa:g}_d{damping:g}_m{mask:g}" + {0: "", 1: "_mass", 2: "_mom"}[int(mass_only)]
+                    + suffix)
+            runs.append((name, None, lambda s: learned_fold * net.advance(s, learned_fold),
+                         (kappa, damping, mask, mass_only, prm)))
     sig_fine = None
     for name, solver, step, var in runs:
         if var is not None:
-            net.close()
+            if net is not trainer:
+                net.close()
             net = Closure(N, cm.domain_radius, limiter=var[0], mask_density=var[2])
-            net.set_params(params)
+            net.set_params(var[4])
             # the corrected flux components: all, the mass flux only (1) or
             # the momentum fluxes only (2)
             comp = {0: np.ones(3), 1: np.array([1.0, 0.0, 0.0]), 2: np.array([0.0, 1.0, 1.0])}[int(var[3])]
@@ -820,5 +852,7 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
                 out[name]["mdot_rel_err"] = abs(out[name]["mdot"] - ref) / abs(ref)
                 out[name]["torque_rel_err"] = abs(out[name]["torque"] - reft) / abs(reft)
                 out[name]["speedup_vs_fine"] = out["fine"]["wall_per_orbit_s"] / out[name]["wall_per_orbit_s"]
-    net.close()
+    if net is not trainer:
+        net.close()
+    trainer.close()
     return out

@@ -258,7 +258,9 @@ def run_learned(args):
                         rollout_smooth=args.rollout_smooth, pair_sinks=not args.pair_no_sinks,
                         seed=args.closure_seed, spread_runs=args.mdot_spread,
                         pair_start_orbits=args.pair_start_orbits,
-                        train_runs=workloads.training_runs(args.train_runs) if args.train_runs else (), log=log)
+                        train_runs=workloads.training_runs(args.train_runs) if args.train_runs else (),
+                        eval_start_orbits=args.eval_start_orbits,
+                        net_seeds=[int(x) for x in args.net_seeds.split(",")] if args.net_seeds else (), log=log)
     # the headline: the fastest learned variant within 2% accretion-rate error
     ok = [k for k, v in r.items() if k.startswith("learned") and isinstance(v, dict)
           and v.get("mdot_rel_err", 1.0) <= 0.02]
@@ -270,7 +272,8 @@ def run_learned(args):
         "n_gpus": 1, "steps": r[best]["advance_calls"] if best else None, "warmup": 0,
         "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 (net: bf16 operands, fp32 accumulation)",
-        "data": "synthetic (seeded cavity disk, seed %d); training pairs from the same run" % w["seed"],
+        "data": "synthetic (seeded cavity disk, seed %d); training pairs from %s" % (
+            w["seed"], "%d runs with drawn parameters" % args.train_runs if args.train_runs else "the same run"),
         "config": {"workload": "learned512", "fine": "iso2048 (isothermal, 32x32 blocks of 64^2, RK2)",
                    **r},
     }
@@ -307,6 +310,10 @@ def main():
     ap.add_argument("--pair-start-orbits", type=float, default=0.0, help="training pairs only from this time on")
     ap.add_argument("--train-runs", type=int, default=0,
                     help="train on this many 2048^2 runs with drawn parameters instead of the evaluated run")
+    ap.add_argument("--eval-start-orbits", type=float, default=None,
+                    help="with --train-runs: the evaluated run's window starts here (default: --train-orbits)")
+    ap.add_argument("--net-seeds", default="",
+                    help="more net seeds trained on the same one-step pairs, comma-separated")
     ap.add_argument("--mdot-spread", type=int, default=0,
                     help="plain fine and coarse runs from perturbed copies of the window's start")
     ap.add_argument("--pair-no-sinks", action="store_true",
