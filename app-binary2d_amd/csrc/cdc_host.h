@@ -115,6 +115,10 @@ struct cdc_solver {
     // current
     double *d_edge[cdc::host::NBUF] = {};
     uint64_t edge_version = UINT64_MAX;
+    // the column-marching kernel's static operands (StageArgs::stat; with
+    // the edge arrays only), nstat pair planes
+    double *d_stat = nullptr;
+    int nstat = 0;
 
     bool timing = false;
     int64_t launches = 0;

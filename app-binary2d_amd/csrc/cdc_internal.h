@@ -79,6 +79,11 @@ struct StageArgs {
     // ec and writes eo
     const double *ec;
     double *eo;
+    // the column-marching kernel's static operands in 16-byte pairs, one
+    // pair plane [N][N][2] after another: (u0[0], u0[1]), (u0[2], u0[3] or,
+    // for 3 variables, the buffer rate) and, with beta cooling, (buffer
+    // rate, the advance's cooling factor); null when no strip shape fits
+    const double *stat;
     int32_t N, nb, bs, ntiles, tpb;  // tpb: tiles per block
     int32_t stage_id, read_base, want_amax, reuse_prim;
     int32_t cool_factor;   // beta cooling reads exp(-dt Omega / beta) from aux plane 2
@@ -139,6 +144,11 @@ hipError_t launch_march_batch(int nv, int rows, int cool, int visc, int read_bas
 // The strip-edge columns of the state u (SoA [nv][N][N]) into E (the layout
 // of StageArgs::ec), N a multiple of 64.
 hipError_t launch_edge_build(int nv, int N, const double *u, double *E, hipStream_t s);
+// StageArgs::stat from the initial state u0 (SoA [nv][nn]) and the aux
+// planes (buffer rate in plane 0); npair = 3 leaves the cooling-factor slot
+// of the third pair plane to launch_cooling_factor.
+hipError_t launch_stat_build(int nv, int npair, const double *u0, const double *aux, double *stat, int64_t nn,
+                             hipStream_t s);
 hipError_t launch_finalize_batch(int nv, const StageArgs *args, const int32_t *block_begin, int nruns,
                                  int nblocks, hipStream_t s);
 hipError_t launch_batch_reset(const StageArgs *args, int nruns, hipStream_t s);
@@ -152,8 +162,10 @@ hipError_t launch_set_int(int32_t *p, int32_t v, hipStream_t s);
 // Zeroes an advance's error flag and max-speed slot (a kernel, so that a
 // captured graph holds no memset node).
 hipError_t launch_reset_advance(int32_t *flag, double *amax, hipStream_t s);
-// aux plane 2 = exp(-dt Omega / beta) from plane 1 and a->dt (nn cells)
-hipError_t launch_cooling_factor(const StageArgs *a, double *aux, int64_t nn, hipStream_t s);
+// aux plane 2 = exp(-dt Omega / beta) from plane 1 and a->dt (nn cells),
+// and the same factor into the cooling slot of stat's third pair plane
+// when stat is not null
+hipError_t launch_cooling_factor(const StageArgs *a, double *aux, double *stat, int64_t nn, hipStream_t s);
 // out[3 i .. 3 i + 2] = the stage kernel's 1/x, 1/sqrt(x), sqrt(x) of x[i]
 hipError_t launch_math_probe(const double *x, double *out, int n, hipStream_t s);
 // r in 1..16 (a power of two), down = 1 for averaging, 0 for injection

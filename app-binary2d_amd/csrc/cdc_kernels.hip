@@ -509,13 +509,16 @@ __global__ void set_int_kernel(int32_t *p, int32_t v) { *p = v; }
 
 // The beta-cooling relaxation factor exp(-dt Omega / beta) of every cell for
 // an advance (one dt for all its stages): aux plane 2 from plane 1 (Omega),
-// the same expression the stage kernel evaluates when it is not given.
+// the same expression the stage kernel evaluates when it is not given; also
+// the cooling slot of the march kernel's third pair plane.
 __global__ void __launch_bounds__(256) cooling_factor_kernel(const StageArgs *__restrict__ a, double *aux,
-                                                              int64_t nn) {
+                                                              double *stat, int64_t nn) {
     const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (c >= nn) return;
     const double dt = a->dt, cool = aux[nn + c];
-    aux[2 * nn + c] = exp(-dt * cool * a->k.cooling_inv_beta);
+    const double f = exp(-dt * cool * a->k.cooling_inv_beta);
+    aux[2 * nn + c] = f;
+    if (stat) stat[4 * nn + 2 * c + 1] = f;
 }
 
 // the refined estimates, for measuring their error
@@ -680,8 +683,9 @@ hipError_t launch_set_int(int32_t *p, int32_t v, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_cooling_factor(const StageArgs *a, double *aux, int64_t nn, hipStream_t s) {
-    hipLaunchKernelGGL(cooling_factor_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, a, aux, nn);
+hipError_t launch_cooling_factor(const StageArgs *a, double *aux, double *stat, int64_t nn, hipStream_t s) {
+    hipLaunchKernelGGL(cooling_factor_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, a, aux, stat,
+                       nn);
     return hipGetLastError();
 }
 

@@ -18,6 +18,28 @@ hipError_t launch_edge_build(int nv, int N, const double *u, double *E, hipStrea
     return hipGetLastError();
 }
 
+// StageArgs::stat, one thread per cell
+__global__ void __launch_bounds__(256) stat_build_kernel(const double *__restrict__ u0, const double *__restrict__ aux,
+                                                         double *__restrict__ stat, int64_t nn, int nv, int npair) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= nn) return;
+    stat[2 * c] = u0[c];
+    stat[2 * c + 1] = u0[nn + c];
+    stat[2 * nn + 2 * c] = u0[2 * nn + c];
+    stat[2 * nn + 2 * c + 1] = nv == 4 ? u0[3 * nn + c] : aux[c];
+    if (npair == 3) {
+        stat[4 * nn + 2 * c] = aux[c];
+        stat[4 * nn + 2 * c + 1] = 0.0;
+    }
+}
+
+hipError_t launch_stat_build(int nv, int npair, const double *u0, const double *aux, double *stat, int64_t nn,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(stat_build_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, u0, aux, stat, nn,
+                       nv, npair);
+    return hipGetLastError();
+}
+
 hipError_t launch_march_batch(int nv, int rows, int cool, int visc, int read_base, const StageArgs *args,
                               const int32_t *begin, int nruns, int ntiles, const StageArgs *fin,
                               const int32_t *fin_begin, int fin_nruns, int nblk, hipEvent_t ev0, hipEvent_t ev1,

@@ -165,6 +165,7 @@ void free_all(cdc_solver *s) {
     (void)hipFree(s->d_perr);
     (void)hipFree(s->d_err_flag);
     (void)hipFree(s->d_aux);
+    (void)hipFree(s->d_stat);
     (void)hipFree(s->d_stage);
     (void)hipFree(s->d_amax);
     (void)hipFree(s->d_args);
@@ -223,6 +224,9 @@ int compute_aux(cdc_solver *s) {
     a.rs2 = softening_length(s->phys) * softening_length(s->phys);
     a.mtot = el[1];
     HIP_TRY(cdc::launch_aux(a, s->stream));
+    if (s->d_stat)
+        HIP_TRY(cdc::launch_stat_build(s->nv, s->nstat, s->d_init, s->d_aux, s->d_stat, (int64_t)s->ncell(),
+                                       s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return CDC_OK;
 }
@@ -275,7 +279,7 @@ int upload_stages(cdc_solver *s, size_t n) {
     HIP_TRY(hipMemcpyAsync(s->d_begin, s->h_begin, 4 * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
     // every stage of an advance has the first one's dt
     if (s->h_args[0].cool_factor)
-        HIP_TRY(cdc::launch_cooling_factor(s->d_args, s->d_aux, (int64_t)s->ncell(), s->stream));
+        HIP_TRY(cdc::launch_cooling_factor(s->d_args, s->d_aux, s->d_stat, (int64_t)s->ncell(), s->stream));
     return CDC_OK;
 }
 
@@ -457,6 +461,7 @@ void plan_steps(const cdc_solver *s, int fold, double dt, StepPlan &plan) {
         memset(&a, 0, sizeof(a));
         a.uc = s->d_u[in.idx];
         a.u0 = s->d_init;
+        a.stat = s->d_stat;
         a.ub = s->d_u[base.idx];
         a.uo = s->d_u[out.idx];
         a.aux = s->d_aux;
@@ -747,6 +752,10 @@ int32_t cdc_solver_create(const cdc_mesh_t *mesh, const cdc_hydro_t *hydro, cons
     // relaxation factor exp(-dt Omega / beta)
     s->naux = (hydro->kind == CDC_EULER && hydro->cooling_model == CDC_COOLING_BETA) ? 3 : 1;
     ALLOC(s->d_aux, (size_t)s->naux * s->ncell() * sizeof(double));
+    if (s->d_edge[0]) {
+        s->nstat = s->naux > 2 ? 3 : 2;
+        ALLOC(s->d_stat, (size_t)s->nstat * 2 * s->ncell() * sizeof(double));
+    }
     s->namax = 1024;
     ALLOC(s->d_amax, (size_t)s->namax * sizeof(double));
     ALLOC(s->d_begin, 4 * sizeof(int32_t));
@@ -914,6 +923,7 @@ int32_t cdc_solver_autotune(cdc_solver_t *s, int32_t reps, int32_t *tx, int32_t 
         memset(&a, 0, sizeof(a));
         a.uc = s->d_u[s->cur];
         a.u0 = s->d_init;
+        a.stat = s->d_stat;
         a.ub = s->d_u[s->cur];
         a.aux = s->d_aux;
         a.uo = s->d_u[scratch];
