@@ -506,7 +506,7 @@ __global__ void __launch_bounds__(256) adam_kernel(const AdamArgs a) {
 }
 
 struct FeatArgs {
-    const double *u;     // SoA [nv][N][N]
+    const double *u;     // device state (sidx layout)
     uint16_t *x;         // [N*N][32]
     int N, nv;
     double gm1, a0, a1, a2;
@@ -519,7 +519,8 @@ __device__ __forceinline__ void features_at(const FeatArgs &a) {
     const int64_t NN = (int64_t)a.N * a.N;
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (p >= NN) return;
-    const double sig = a.u[p], sx = a.u[NN + p], sy = a.u[2 * NN + p];
+    const double sig = a.u[cdc::sidx(a.nv, 0, NN, p)], sx = a.u[cdc::sidx(a.nv, 1, NN, p)],
+                 sy = a.u[cdc::sidx(a.nv, 2, NN, p)];
     const double vx = sx / sig, vy = sy / sig;
     float f[4];
     f[0] = (float)(a.a0 * log(fmax(sig, 1e-30)));
@@ -527,7 +528,7 @@ __device__ __forceinline__ void features_at(const FeatArgs &a) {
     f[2] = (float)(a.a1 * vy);
     f[3] = 0.0f;
     if (a.nv == 4) {
-        const double pr = (a.u[3 * NN + p] - 0.5 * sig * (vx * vx + vy * vy)) * a.gm1;
+        const double pr = (a.u[cdc::sidx(a.nv, 3, NN, p)] - 0.5 * sig * (vx * vx + vy * vy)) * a.gm1;
         f[3] = (float)(a.a2 * log(fmax(pr / sig, 1e-30)));
     } else {
         // ln(-phi), phi the softened two-body potential at the cell centre
@@ -560,7 +561,7 @@ struct StepIO {
 __global__ void __launch_bounds__(256) edge_step_kernel(const StepIO *io, int f, int N, int nv) {
     const int64_t total = (int64_t)nv * N * (N / 64) * 4;
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e < total) io[f].e_out[e] = cdc::edge_entry(io[f].u_out, e, N);
+    if (e < total) io[f].e_out[e] = cdc::edge_entry(io[f].u_out, e, N, nv);
 }
 
 // Per-workgroup maxima of the signal speed of the last step's corrected
@@ -574,7 +575,7 @@ __global__ void __launch_bounds__(256) max_speed_step_kernel(const StepIO *io, i
     for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * 256) {
         double w[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) w[v] = u[v * ncell + c];
+        for (int v = 0; v < NV; ++v) w[v] = u[cdc::sidx(NV, v, ncell, c)];
         m = fmax(m, cdc::dev::max_signal_speed<NV>(w, gm));
     }
     red[threadIdx.x] = m;
@@ -595,11 +596,11 @@ __global__ void __launch_bounds__(256) features_step_kernel(FeatArgs a, const St
 }
 
 struct TargetArgs {
-    const double *plain, *truth;   // SoA [nv][N][N]
-    const double *base;            // SoA state the features come from
+    const double *plain, *truth;   // device states (sidx layout)
+    const double *base;            // the state the features come from
     float *t;                      // [N*N][4]: 3 targets and the cell weight
     float *part;                   // [nblocks][3] weighted sums of t_v^2
-    int N;
+    int N, nv;
     double inv_dt, mask;           // cell weight 1 where base Sigma >= mask, else 0
 };
 
@@ -609,9 +610,10 @@ __global__ void __launch_bounds__(256) target_kernel(const TargetArgs a) {
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
     float sq[3] = {0.0f, 0.0f, 0.0f};
     if (p < NN) {
-        const float wc = a.base[p] >= a.mask ? 1.0f : 0.0f;
+        const float wc = a.base[cdc::sidx(a.nv, 0, NN, p)] >= a.mask ? 1.0f : 0.0f;
         for (int v = 0; v < 3; ++v) {
-            const float d = (float)((a.truth[v * NN + p] - a.plain[v * NN + p]) * a.inv_dt);
+            const int64_t q = cdc::sidx(a.nv, v, NN, p);
+            const float d = (float)((a.truth[q] - a.plain[q]) * a.inv_dt);
             a.t[p * 4 + v] = d;
             sq[v] = wc * d * d;
         }
@@ -628,7 +630,7 @@ __global__ void __launch_bounds__(256) target_kernel(const TargetArgs a) {
 }
 
 struct ApplyArgs {
-    double *u;            // SoA [nv][N][N], corrected in place
+    double *u;            // device state (sidx layout), corrected in place
     const float *y;       // [N*N][YS]
     double *lam;          // [N*N][2] face limiter factors (x+, y+ face of each cell)
     int N, nv;
@@ -646,10 +648,11 @@ struct ApplyArgs {
 // mass times the local signal speed in momentum.  kappa <= 0 disables it.
 // Faces where either density is below mask_density get no correction.
 __device__ inline double signal_speed(const ApplyArgs &a, int64_t NN, int64_t q) {
-    const double sig = a.u[q];
-    const double vx = a.u[NN + q] / sig, vy = a.u[2 * NN + q] / sig;
+    const double sig = a.u[cdc::sidx(a.nv, 0, NN, q)];
+    const double vx = a.u[cdc::sidx(a.nv, 1, NN, q)] / sig, vy = a.u[cdc::sidx(a.nv, 2, NN, q)] / sig;
     double c2 = 0.0;
-    if (a.nv == 4) c2 = a.gm * (a.u[3 * NN + q] - 0.5 * sig * (vx * vx + vy * vy)) * (a.gm - 1.0) / sig;
+    if (a.nv == 4)
+        c2 = a.gm * (a.u[cdc::sidx(a.nv, 3, NN, q)] - 0.5 * sig * (vx * vx + vy * vy)) * (a.gm - 1.0) / sig;
     return sqrt(vx * vx + vy * vy) + sqrt(fmax(c2, 0.0));
 }
 
@@ -662,13 +665,13 @@ __device__ __forceinline__ void limit_at(const ApplyArgs &a) {
     const double id[2] = {a.idx, a.idy};
     double l[2] = {1.0, 1.0};
     for (int d = 0; d < 2; ++d)
-        if (fmin(a.u[p], a.u[nb[d]]) < a.mask) l[d] = 0.0;
+        if (fmin(a.u[2 * p], a.u[2 * nb[d]]) < a.mask) l[d] = 0.0;   // Sigma: sidx(nv, 0, NN, c) = 2 c
     if (a.kappa > 0.0) {
-        const double sig = a.u[p];
+        const double sig = a.u[2 * p];
         const double ap = signal_speed(a, NN, p);
         for (int d = 0; d < 2; ++d) {
             if (l[d] == 0.0) continue;
-            const double cap = a.kappa * fmax(fmin(sig, a.u[nb[d]]), 0.0);
+            const double cap = a.kappa * fmax(fmin(sig, a.u[2 * nb[d]]), 0.0);
             const double capm = cap * fmax(ap, signal_speed(a, NN, nb[d]));
             double lam = 1.0;
             for (int v = 0; v < 3; ++v) {
@@ -694,7 +697,7 @@ __device__ __forceinline__ void apply_at(const ApplyArgs &a) {
         const double fx = lxp * (double)a.y[p * YS + v] - lxm * (double)a.y[xm * YS + v];
         const double fy = lyp * (double)a.y[p * YS + 3 + v] - lym * (double)a.y[ym * YS + 3 + v];
         const double r = -a.s[v] * (fx * a.idx + fy * a.idy);
-        a.u[v * NN + p] += a.dt * r;
+        a.u[cdc::sidx(a.nv, v, NN, p)] += a.dt * r;
     }
 }
 
@@ -1160,6 +1163,7 @@ int32_t cdc_closure_add_sample(cdc_closure_t *c, const cdc_solver_t *base, const
     a.t = c->d_tgt + (size_t)k * c->P * 4;
     a.part = c->d_part_l;
     a.N = c->n;
+    a.nv = base->nv;
     a.inv_dt = 1.0 / dt;
     hipLaunchKernelGGL(target_kernel, dim3(c->nlb), dim3(256), 0, c->stream, a);
     HIP_TRY(hipGetLastError());

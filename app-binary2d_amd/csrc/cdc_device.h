@@ -14,6 +14,15 @@
 
 namespace cdc {
 
+// Element (v, c) of a device state of nv conserved variables over nn cells
+// (c = gi * N + gj): variables 0 and 1 interleaved in a pair plane [nn][2],
+// then variables 2 and 3 in a second pair plane (gamma-law gas) or variable
+// 2 in a plane [nn] of its own (isothermal gas), so that a lane loads a
+// cell's pair of values in one 16-byte access.
+__host__ __device__ __forceinline__ int64_t sidx(int nv, int v, int64_t nn, int64_t c) {
+    return v < 2 ? 2 * c + v : 2 * nn + (nv == 4 ? 2 * c + (v - 2) : c);
+}
+
 // ---------------------------------------------------------------------------
 // Compact list of the itemized-source components that can be nonzero; the
 // others (grav mass, iso cooling, Euler fake_mass) are identically zero.
@@ -448,17 +457,19 @@ __device__ __forceinline__ void cell_sources(const StageArgs &A, const Orbit &o,
 
 }  // namespace dev
 
-// Entry e of the strip-edge columns of state u (StageArgs::ec layout
-// [nv][N][N / 64][4]: per plane and row, the columns 64 k - 2 .. 64 k + 1 of
-// boundary k, periodic).
-__device__ __forceinline__ double edge_entry(const double *u, int64_t e, int N) {
+// Entry e of the strip-edge columns of state u of nv variables (StageArgs::ec
+// layout [nv][N][N / 64][4]: per variable and row, the columns
+// 64 k - 2 .. 64 k + 1 of boundary k, periodic).
+__device__ __forceinline__ double edge_entry(const double *u, int64_t e, int N, int nv) {
     const int NK = N / 64;
     const int c = (int)(e & 3);
     const int64_t rest = e >> 2;
     const int k = (int)(rest % NK);
     int col = 64 * k - 2 + c;
     col = col < 0 ? col + N : (col >= N ? col - N : col);
-    return u[(rest / NK) * N + col];
+    const int64_t vr = rest / NK;   // v * N + row
+    const int v = (int)(vr / N);
+    return u[sidx(nv, v, (int64_t)N * N, (vr - (int64_t)v * N) * N + col)];
 }
 
 // ---------------------------------------------------------------------------

@@ -53,11 +53,11 @@ struct AuxArgs {
 };
 
 struct StageArgs {
-    const double *uc;      // stage input, SoA [nv][N][N]
-    const double *u0;      // initial (background) state, SoA
-    const double *ub;      // RK step base, SoA (read when read_base)
+    const double *uc;      // stage input (device state layout, sidx)
+    const double *u0;      // initial (background) state, the same layout
+    const double *ub;      // RK step base (read when read_base)
     const double *aux;     // static per-cell planes
-    double *uo;            // stage output, SoA
+    double *uo;            // stage output
     double *tile_sums;     // [ntiles][NSUM + 1] (last slot: max signal speed)
     TileErr *tile_err;     // [ntiles]
     int32_t *err_flag;     // stage id of the first failing stage, 0 if none
@@ -141,10 +141,10 @@ hipError_t launch_march_batch(int nv, int rows, int cool, int visc, int read_bas
                               const int32_t *begin, int nruns, int ntiles, const StageArgs *fin,
                               const int32_t *fin_begin, int fin_nruns, int nblk, hipEvent_t ev0, hipEvent_t ev1,
                               hipStream_t s);
-// The strip-edge columns of the state u (SoA [nv][N][N]) into E (the layout
+// The strip-edge columns of the state u (the device state layout, sidx) into E (the layout
 // of StageArgs::ec), N a multiple of 64.
 hipError_t launch_edge_build(int nv, int N, const double *u, double *E, hipStream_t s);
-// StageArgs::stat from the initial state u0 (SoA [nv][nn]) and the aux
+// StageArgs::stat from the initial state u0 (the device state layout) and the aux
 // planes (buffer rate in plane 0); npair = 3 leaves the cooling-factor slot
 // of the third pair plane to launch_cooling_factor.
 hipError_t launch_stat_build(int nv, int npair, const double *u0, const double *aux, double *stat, int64_t nn,

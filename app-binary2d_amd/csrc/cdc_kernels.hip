@@ -121,7 +121,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile, d
             gj = gj < 0 ? gj + N : (gj >= N ? gj - N : gj);
             const size_t g = (size_t)gi * N + gj;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) ue[q][v] = guc[v * NN + g];
+            for (int v = 0; v < NV; ++v) ue[q][v] = guc[sidx(NV, v, NN, g)];
         }
     }
     double pu0[CPT][NV], pub[CPT][NV], pbr[CPT], pom[CPT];
@@ -134,8 +134,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile, d
         // neighbouring tiles re-read
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            pu0[q][v] = __builtin_nontemporal_load(&gu0[v * NN + g]);
-            pub[q][v] = A.read_base ? __builtin_nontemporal_load(&gub[v * NN + g]) : 0.0;
+            pu0[q][v] = __builtin_nontemporal_load(&gu0[sidx(NV, v, NN, g)]);
+            pub[q][v] = A.read_base ? __builtin_nontemporal_load(&gub[sidx(NV, v, NN, g)]) : 0.0;
         }
         pbr[q] = __builtin_nontemporal_load(&gaux[g]);
         pom[q] = A.k.cooling_model == CDC_COOLING_BETA
@@ -241,7 +241,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile, d
         double uc[NV], du[NV];
         const double *u0 = pu0[q], *ub = pub[q];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) uc[v] = guc[v * NN + g];
+        for (int v = 0; v < NV; ++v) uc[v] = guc[sidx(NV, v, NN, g)];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const double *fx = FX + v * NFX + ci * TY + cj;
@@ -277,7 +277,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile, d
             out[v] = A.read_base ? u1 * A.c1 + ub[v] * A.c0 : u1;
             // streamed past L2: the next stage reads it back through its halo
             // loads, and dirty lines left in L2 cost more than they save
-            __builtin_nontemporal_store(out[v], &guo[v * NN + g]);
+            __builtin_nontemporal_store(out[v], &guo[sidx(NV, v, NN, g)]);
         }
 #pragma unroll
         for (int s = 0; s < NS; ++s) acc[s] += it[SumMap<NV>::item(s)][SumMap<NV>::comp(s)];
@@ -439,7 +439,7 @@ __global__ void __launch_bounds__(256) max_speed_kernel(const double *u, int64_t
     for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * 256) {
         double w[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) w[v] = u[v * ncell + c];
+        for (int v = 0; v < NV; ++v) w[v] = u[sidx(NV, v, ncell, c)];
         m = fmax(m, dev::max_signal_speed<NV>(w, gm));
     }
     red[threadIdx.x] = m;
@@ -451,7 +451,7 @@ __global__ void __launch_bounds__(256) max_speed_kernel(const double *u, int64_t
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// Boundary layout [block][i][j][v] <-> device layout [v][gi][gj]
+// Boundary layout [block][i][j][v] <-> device layout (sidx)
 __global__ void __launch_bounds__(256) aos_to_soa_kernel(int nv, int nb, int bs, const double *aos,
                                                          double *soa) {
     const int64_t N = (int64_t)nb * bs, NN = N * N;
@@ -461,7 +461,7 @@ __global__ void __launch_bounds__(256) aos_to_soa_kernel(int nv, int nb, int bs,
     const int64_t b = c / bsq, l = c - b * bsq;
     const int64_t i = l / bs, j = l - i * bs;
     const int64_t gi = (b / nb) * bs + i, gj = (b % nb) * bs + j;
-    for (int v = 0; v < nv; ++v) soa[v * NN + gi * N + gj] = aos[c * nv + v];
+    for (int v = 0; v < nv; ++v) soa[sidx(nv, v, NN, gi * N + gj)] = aos[c * nv + v];
 }
 
 __global__ void __launch_bounds__(256) soa_to_aos_kernel(int nv, int nb, int bs, const double *soa,
@@ -473,10 +473,10 @@ __global__ void __launch_bounds__(256) soa_to_aos_kernel(int nv, int nb, int bs,
     const int64_t b = c / bsq, l = c - b * bsq;
     const int64_t i = l / bs, j = l - i * bs;
     const int64_t gi = (b / nb) * bs + i, gj = (b % nb) * bs + j;
-    for (int v = 0; v < nv; ++v) aos[c * nv + v] = soa[v * NN + gi * N + gj];
+    for (int v = 0; v < nv; ++v) aos[c * nv + v] = soa[sidx(nv, v, NN, gi * N + gj)];
 }
 
-// Re-gridding between two SoA states of one domain, r a power of two:
+// Re-gridding between two device states (sidx layout) of one domain, r a power of two:
 // injection (up) or the mean of r x r fine cells as a pairwise tree over each
 // row, then over the row means (down), the same order as the host resampler.
 __global__ void __launch_bounds__(256) resample_kernel(int nv, const double *src, int64_t Ns, double *dst,
@@ -484,24 +484,24 @@ __global__ void __launch_bounds__(256) resample_kernel(int nv, const double *src
     const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (c >= Nd * Nd) return;
     const int64_t gi = c / Nd, gj = c - (c / Nd) * Nd;
+    const int64_t ns = Ns * Ns, nd = Nd * Nd;
     for (int v = 0; v < nv; ++v) {
-        const double *sp = src + v * Ns * Ns;
         if (!down) {
-            dst[v * Nd * Nd + c] = sp[(gi / r) * Ns + gj / r];
+            dst[sidx(nv, v, nd, c)] = src[sidx(nv, v, ns, (gi / r) * Ns + gj / r)];
             continue;
         }
         double rows[16];
         for (int a = 0; a < r; ++a) {
             double t[16];
-            const double *q = sp + (gi * r + a) * Ns + gj * r;
-            for (int k = 0; k < r; ++k) t[k] = q[k];
+            const int64_t q = (gi * r + a) * Ns + gj * r;
+            for (int k = 0; k < r; ++k) t[k] = src[sidx(nv, v, ns, q + k)];
             for (int w = r; w > 1; w /= 2)
                 for (int k = 0; k < w / 2; ++k) t[k] = t[2 * k] + t[2 * k + 1];
             rows[a] = t[0] / (double)r;
         }
         for (int w = r; w > 1; w /= 2)
             for (int k = 0; k < w / 2; ++k) rows[k] = rows[2 * k] + rows[2 * k + 1];
-        dst[v * Nd * Nd + c] = rows[0] / (double)r;
+        dst[sidx(nv, v, nd, c)] = rows[0] / (double)r;
     }
 }
 

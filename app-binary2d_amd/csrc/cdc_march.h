@@ -205,10 +205,10 @@ __device__ __forceinline__ double bload(Rsrc r, unsigned voff, unsigned soff) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, AUX));
 }
 typedef double d2_t __attribute__((ext_vector_type(2)));
-// two doubles at byte offset voff (16-byte aligned): one 128-bit load
+// two doubles at byte offset voff + soff (16-byte aligned): one 128-bit load
 template <int AUX = 0>
-__device__ __forceinline__ d2_t bload2(Rsrc r, unsigned voff) {
-    return __builtin_bit_cast(d2_t, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, 0, AUX));
+__device__ __forceinline__ d2_t bload2(Rsrc r, unsigned voff, unsigned soff = 0u) {
+    return __builtin_bit_cast(d2_t, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, AUX));
 }
 // The stage's output stores take the default cache policy: the next stage
 // reads them back (on the 2048^2 gamma-law workload 1.2% faster per launch
@@ -220,11 +220,39 @@ __device__ __forceinline__ void bstore(double x, Rsrc r, unsigned voff, unsigned
                                           (int)voff, (int)soff, 0);
 }
 
-// cell at byte offset voff, planes pl bytes apart
+__device__ __forceinline__ void bstore2(double x, double y, Rsrc r, unsigned voff, unsigned soff) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned,
+                                                              d2_t{x, y}),
+                                           r, (int)voff, (int)soff, 0);
+}
+
+// cell at byte offset voff, planes pl bytes apart (the strip-edge arrays)
 template <int NV>
 __device__ __forceinline__ void load_cell(Rsrc r, unsigned voff, unsigned pl, double *out) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) out[v] = bload(r, voff, (unsigned)v * pl);
+}
+
+// Cell c of a state in the sidx layout, g = 8 c and PB = 8 N^2 bytes: the
+// pair (v0, v1) at 16 c, then (v2, v3) at 2 PB + 16 c, or v2 at 2 PB + 8 c
+template <int NV, int AUX = 0>
+__device__ __forceinline__ void load_state(Rsrc r, unsigned g, unsigned PB, double *out) {
+    const d2_t a = bload2<AUX>(r, 2u * g);
+    out[0] = a[0];
+    out[1] = a[1];
+    if constexpr (NV == 4) {
+        const d2_t b = bload2<AUX>(r, 2u * g, 2u * PB);
+        out[2] = b[0];
+        out[3] = b[1];
+    } else {
+        out[2] = bload<AUX>(r, g, 2u * PB);
+    }
+}
+template <int NV>
+__device__ __forceinline__ void store_state(const double *x, Rsrc r, unsigned g, unsigned PB) {
+    bstore2(x[0], x[1], r, 2u * g, 0u);
+    if constexpr (NV == 4) bstore2(x[2], x[3], r, 2u * g, 2u * PB);
+    else bstore(x[2], r, g, 2u * PB);
 }
 
 // PLM theta of the three cells a, b, c (plm_density_trigger); PDT = false:
@@ -382,10 +410,10 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
     const int gj = gj0 + j;                   // global column (inside the block)
     const double yc = axy.c(j), yv1 = axy.v(j + 1);
     double Um2[NV], Um1[NV], U0r[NV], Uz[NV];
-    load_cell<NV>(ruc, ((unsigned)grow(mrow(0)) * N + gj) * 8u, PB, Um2);
-    load_cell<NV>(ruc, ((unsigned)grow(mrow(1)) * N + gj) * 8u, PB, Um1);
-    load_cell<NV>(ruc, ((unsigned)grow(mrow(2)) * N + gj) * 8u, PB, U0r);
-    load_cell<NV>(ruc, ((unsigned)grow(mrow(3)) * N + gj) * 8u, PB, Uz);
+    load_state<NV>(ruc, ((unsigned)grow(mrow(0)) * N + gj) * 8u, PB, Um2);
+    load_state<NV>(ruc, ((unsigned)grow(mrow(1)) * N + gj) * 8u, PB, Um1);
+    load_state<NV>(ruc, ((unsigned)grow(mrow(2)) * N + gj) * 8u, PB, U0r);
+    load_state<NV>(ruc, ((unsigned)grow(mrow(3)) * N + gj) * 8u, PB, Uz);
 #pragma unroll
     for (int s = 0; s < NS; ++s) Acc[s * 64 + lane] = 0.0;
     // the rows' x coordinates, read by every step instead of evaluated
@@ -463,9 +491,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
         S0[2] = q1[0];
         if constexpr (NV == 4) S0[3] = q1[1];
         else Sr = q1[1];
-#pragma unroll
-        for (int v = 0; v < NV; ++v)
-            if constexpr (RB != 0) Sb[v] = bload<kNt>(rub, g, (unsigned)v * PB);
+        if constexpr (RB != 0) load_state<NV, kNt>(rub, g, PB, Sb);
         if constexpr (kStat3) {
             const d2_t q2 = bload2<kNt>(rs2, 2u * g);
             Sr = q2[0];
@@ -484,7 +510,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
         const int ha = mrow(t + 1) - ri0;   // row a in the strip
         // next row's conserved state, one step ahead
         double Un[NV];
-        if constexpr (next) load_cell<NV>(ruc, ((unsigned)grow(mrow(t + 4)) * N + gj) * 8u, PB, Un);
+        if constexpr (next) load_state<NV>(ruc, ((unsigned)grow(mrow(t + 4)) * N + gj) * 8u, PB, Un);
         // primitives of row z
         double Pz[NV], idz, prz, val;
         const int code = dev::try_to_primitive<NV, !PLAIN>(k, Uz, Pz, val, idz, prz);
@@ -620,8 +646,7 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
             }
             // the next update's operands, then this row's stores
             if constexpr (next) load_update_operands(mrow(t + 2));
-#pragma unroll
-            for (int v = 0; v < NV; ++v) bstore(out[v], ruo, g, (unsigned)v * PB);
+            store_state<NV>(out, ruo, g, PB);
             // into the lane's own LDS slots (no return value: no registers held)
             if (A.want_amax) amax = fmax(amax, dev::max_signal_speed<NV>(out, k.gm));
         }

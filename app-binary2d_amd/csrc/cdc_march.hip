@@ -9,7 +9,7 @@ __global__ void __launch_bounds__(256) edge_build_kernel(const double *__restric
                                                          int nv) {
     const int64_t total = (int64_t)nv * N * (N / 64) * 4;
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e < total) E[e] = edge_entry(u, e, N);
+    if (e < total) E[e] = edge_entry(u, e, N, nv);
 }
 
 hipError_t launch_edge_build(int nv, int N, const double *u, double *E, hipStream_t s) {
@@ -23,10 +23,10 @@ __global__ void __launch_bounds__(256) stat_build_kernel(const double *__restric
                                                          double *__restrict__ stat, int64_t nn, int nv, int npair) {
     const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (c >= nn) return;
-    stat[2 * c] = u0[c];
-    stat[2 * c + 1] = u0[nn + c];
-    stat[2 * nn + 2 * c] = u0[2 * nn + c];
-    stat[2 * nn + 2 * c + 1] = nv == 4 ? u0[3 * nn + c] : aux[c];
+    stat[2 * c] = u0[sidx(nv, 0, nn, c)];
+    stat[2 * c + 1] = u0[sidx(nv, 1, nn, c)];
+    stat[2 * nn + 2 * c] = u0[sidx(nv, 2, nn, c)];
+    stat[2 * nn + 2 * c + 1] = nv == 4 ? u0[sidx(nv, 3, nn, c)] : aux[c];
     if (npair == 3) {
         stat[4 * nn + 2 * c] = aux[c];
         stat[4 * nn + 2 * c + 1] = 0.0;
