@@ -162,9 +162,9 @@ hipError_t launch_set_int(int32_t *p, int32_t v, hipStream_t s);
 // Zeroes an advance's error flag and max-speed slot (a kernel, so that a
 // captured graph holds no memset node).
 hipError_t launch_reset_advance(int32_t *flag, double *amax, hipStream_t s);
-// aux plane 2 = exp(-dt Omega / beta) from plane 1 and a->dt (nn cells),
-// and the same factor into the cooling slot of stat's third pair plane
-// when stat is not null
+// exp(-dt Omega / beta) from aux plane 1 and a->dt (nn cells): into the
+// cooling slot of stat's third pair plane when stat is not null (the stages
+// march), else into aux plane 2
 hipError_t launch_cooling_factor(const StageArgs *a, double *aux, double *stat, int64_t nn, hipStream_t s);
 // out[3 i .. 3 i + 2] = the stage kernel's 1/x, 1/sqrt(x), sqrt(x) of x[i]
 hipError_t launch_math_probe(const double *x, double *out, int n, hipStream_t s);

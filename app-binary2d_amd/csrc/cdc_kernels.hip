@@ -508,17 +508,19 @@ __global__ void __launch_bounds__(256) resample_kernel(int nv, const double *src
 __global__ void set_int_kernel(int32_t *p, int32_t v) { *p = v; }
 
 // The beta-cooling relaxation factor exp(-dt Omega / beta) of every cell for
-// an advance (one dt for all its stages): aux plane 2 from plane 1 (Omega),
-// the same expression the stage kernel evaluates when it is not given; also
-// the cooling slot of the march kernel's third pair plane.
+// an advance (one dt for all its stages), from aux plane 1 (Omega), the same
+// expression the stage kernel evaluates when it is not given: into the march
+// kernel's third pair plane (buffer rate, factor) when stat is given, stored
+// whole (16 B per lane: a store of half of each pair leaves partly written
+// lines), else into aux plane 2 (the tiled kernel's).
 __global__ void __launch_bounds__(256) cooling_factor_kernel(const StageArgs *__restrict__ a, double *aux,
                                                               double *stat, int64_t nn) {
     const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (c >= nn) return;
     const double dt = a->dt, cool = aux[nn + c];
     const double f = exp(-dt * cool * a->k.cooling_inv_beta);
-    aux[2 * nn + c] = f;
-    if (stat) stat[4 * nn + 2 * c + 1] = f;
+    if (stat) reinterpret_cast<double2 *>(stat + 4 * nn)[c] = make_double2(aux[c], f);
+    else aux[2 * nn + c] = f;
 }
 
 // the refined estimates, for measuring their error

@@ -277,9 +277,11 @@ int stage_args(cdc_solver *s, const StageArgs *args, size_t n) {
 int upload_stages(cdc_solver *s, size_t n) {
     HIP_TRY(hipMemcpyAsync(s->d_args, s->h_args, n * sizeof(StageArgs), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_begin, s->h_begin, 4 * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
-    // every stage of an advance has the first one's dt
+    // every stage of an advance has the first one's dt; the factor goes where
+    // the kernel that runs the stages reads it
     if (s->h_args[0].cool_factor)
-        HIP_TRY(cdc::launch_cooling_factor(s->d_args, s->d_aux, s->d_stat, (int64_t)s->ncell(), s->stream));
+        HIP_TRY(cdc::launch_cooling_factor(s->d_args, s->d_aux, marching(s) ? s->d_stat : nullptr,
+                                           (int64_t)s->ncell(), s->stream));
     return CDC_OK;
 }
 

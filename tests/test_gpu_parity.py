@@ -507,6 +507,24 @@ def test_march_kernel_errors_match_oracle(shape):
     np.testing.assert_array_equal(g.get_state()["conserved"], bad)
 
 
+def test_beta_cooling_factor_follows_the_kernel_switch():
+    """The per-advance beta-cooling factor is written where the kernel that
+    runs the advance reads it (the strips' pair plane or the tiles' aux
+    plane): switching between a strip shape and a tile shape from call to
+    call (graphs of both kept and replayed) matches the oracle after every
+    call, so no kernel reads a factor left by the other's advance at an
+    older dt."""
+    mesh, hydro, ph = H.small("euler_beta", nb=2, bs=64, R=4.0, rk_order="RK3", viscosity={"alpha": 0.05})
+    u0 = H.initial(mesh, hydro, ph)
+    g, o = pair(mesh, hydro, ph, u0)
+    shapes = [(32, 64, 64), (8, 32, 256), (32, 64, 64), (16, 16, 256), (8, 64, 64), (8, 32, 256)]
+    for call, shape in enumerate(shapes * 2):
+        g.set_kernel(*shape)
+        fold = 2 if call % 3 else 1
+        assert g.advance(fold) == pytest.approx(o.advance(fold), rel=1e-12)
+        check_states(g, o)
+
+
 def test_autotune_keeps_state_and_picks_a_valid_shape():
     mesh, hydro, ph = H.small("euler", nb=2, bs=64, R=4.0)
     u0 = H.initial(mesh, hydro, ph)
