@@ -746,6 +746,11 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
     shared."""
     from .solver import Snapshot, Solver, resample_state
 
+    if net_seeds and (loop_rounds or rollout_rounds):
+        raise ValueError("net_seeds retrain the one-step pairs only (no loop or rollout rounds)")
+    if train_runs and (loop_rounds or rollout_rounds):
+        raise ValueError("loop and rollout rounds start from states of the evaluated run: with train_runs "
+                         "they would train on the run they are evaluated on")
     t = train_on_fine(fine_w, coarse_blocks=coarse_blocks, train_orbits=train_orbits, sample_every=sample_every,
                       max_samples=max_samples, train_steps=train_steps, lr=lr, seed=seed, limiter=limiter,
                       train_mask=train_mask, pair_steps=pair_steps, loop_rounds=loop_rounds,
@@ -756,8 +761,6 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
                       rollout_smooth=rollout_smooth, pair_sinks=pair_sinks, pair_start_orbits=pair_start_orbits,
                       extra_runs=train_runs, main_pairs=not train_runs,
                       main_orbits=eval_start_orbits if train_runs else None, log=log)
-    if net_seeds and (loop_rounds or rollout_rounds):
-        raise ValueError("net_seeds retrain the one-step pairs only (no loop or rollout rounds)")
     fine, net, cm, hydro, ph, cu0 = t["fine"], t["net"], t["coarse_mesh"], t["hydro"], t["physics"], t["cu0"]
     trainer = net
     fm = fine_w["mesh"]

@@ -248,6 +248,8 @@ def run_learned(args):
     w = workloads.get("iso2048")
     log = lambda m: print(m, file=sys.stderr, flush=True)
     variants = [tuple(float(x) for x in v.split(":")) for v in args.variants.split(",")]
+    if args.rollout_rounds is None:
+        args.rollout_rounds = 0 if args.train_runs else 3
     r = learned_vs_fine(w, train_orbits=args.train_orbits, eval_orbits=args.eval_orbits,
                         sample_every=args.sample_every, max_samples=args.max_samples,
                         train_steps=args.train_steps, pair_steps=args.pair_steps, variants=variants,
@@ -296,8 +298,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tile", default=None, help="TXxTY stage-kernel tile (skips the autotuner)")
     ap.add_argument("--train-orbits", type=float, default=2.0)
-    ap.add_argument("--rollout-rounds", type=int, default=3,
-                    help="rounds of rollout-objective training pairs after the one-step training")
+    ap.add_argument("--rollout-rounds", type=int, default=None,
+                    help="rounds of rollout-objective training pairs after the one-step training "
+                         "(default 3; 0 with --train-runs, whose rounds would start from the evaluated run)")
     ap.add_argument("--rollout-steps", type=int, default=200, help="corrected coarse steps per rollout")
     ap.add_argument("--rollout-every", type=int, default=20, help="rollout steps between training pairs")
     ap.add_argument("--rollout-train-steps", type=int, default=2000, help="training steps per rollout round")

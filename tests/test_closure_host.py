@@ -91,3 +91,17 @@ def test_sink_rates_match_the_oracle_sink_kernel():
                           O.sink_kernel(pc, x[i] - st[1, 1], x[j] - st[1, 2]) for j in range(N)] for i in range(N)])
         assert np.count_nonzero(want) > 8
         np.testing.assert_allclose(k, want, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("kw", [dict(rollout_rounds=1), dict(loop_rounds=1), dict(net_seeds=(6,), rollout_rounds=1)])
+def test_closure_study_refuses_rounds_it_cannot_run(kw):
+    """Loop and rollout rounds start from states of the evaluated run, so with
+    training runs of their own they would train on it; extra net seeds
+    retrain the one-step pairs only.  Both are refused before any run is
+    built."""
+    from cdc_amd import workloads
+    from cdc_amd.closure import learned_vs_fine
+    w = workloads.get("iso2048")
+    train = () if "net_seeds" in kw else (w,)
+    with pytest.raises(ValueError):
+        learned_vs_fine(w, train_runs=train, **kw)
