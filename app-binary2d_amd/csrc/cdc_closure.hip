@@ -319,7 +319,10 @@ __device__ __forceinline__ void fused_layer(const FusedArgs &a, int layer, const
     }
 }
 
-__global__ void __launch_bounds__(256) net_forward_fused_kernel(const FusedArgs a) {
+// The fused forward pass of one 16 x 16 output tile; feat(F, q, gi, gj)
+// writes the 4 bf16 features of grid cell (gi, gj) to F + 4 q.
+template <typename Feat>
+__device__ __forceinline__ void fused_forward(const FusedArgs &a, Feat feat) {
     __shared__ __attribute__((aligned(16))) uint16_t lds[FUSED_LDS];
     uint16_t *F = lds;                           // features, 24 x 24 x 4
     uint16_t *A1 = F + FR0 * FR0 * 4;            // layer 1 (22 x 22), then layer 3 (18 x 18)
@@ -333,8 +336,7 @@ __global__ void __launch_bounds__(256) net_forward_fused_kernel(const FusedArgs 
     load_weights<false>(a, 1, lane, wb);
     for (int q = tid; q < FR0 * FR0; q += 256) {
         const int fi = q / FR0, fj = q - fi * FR0;
-        const int gi = wrap(i0 - 4 + fi, n), gj = wrap(j0 - 4 + fj, n);
-        *(uint2 *)(F + q * 4) = *(const uint2 *)(a.x + ((size_t)gi * n + gj) * C);
+        feat(F, q, wrap(i0 - 4 + fi, n), wrap(j0 - 4 + fj, n));
     }
     __syncthreads();
     fused_layer<true, false, FR0, FR1>(a, 0, wa, F, A1, i0, j0, wave, lane);
@@ -346,6 +348,13 @@ __global__ void __launch_bounds__(256) net_forward_fused_kernel(const FusedArgs 
     fused_layer<false, false, FR2, FR3>(a, 2, wa, A2, A1, i0, j0, wave, lane);
     __syncthreads();
     fused_layer<false, true, FR3, TT>(a, 3, wb, A1, nullptr, i0, j0, wave, lane);
+}
+
+// the features from the feature buffer a.x
+__global__ void __launch_bounds__(256) net_forward_fused_kernel(const FusedArgs a) {
+    fused_forward(a, [&](uint16_t *F, int q, int gi, int gj) {
+        *(uint2 *)(F + q * 4) = *(const uint2 *)(a.x + ((size_t)gi * a.n + gj) * C);
+    });
 }
 
 struct WgradArgs {
@@ -515,10 +524,9 @@ struct FeatArgs {
     double x1, y1, m1, x2, y2, m2, rs2, R, dx;
 };
 
-__device__ __forceinline__ void features_at(const FeatArgs &a) {
+// the 4 features of cell p, as stored (bf16, clamped to [-16, 16])
+__device__ __forceinline__ void feature_values(const FeatArgs &a, int64_t p, uint16_t *o) {
     const int64_t NN = (int64_t)a.N * a.N;
-    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (p >= NN) return;
     const double sig = a.u[cdc::sidx(a.nv, 0, NN, p)], sx = a.u[cdc::sidx(a.nv, 1, NN, p)],
                  sy = a.u[cdc::sidx(a.nv, 2, NN, p)];
     const double vx = sx / sig, vy = sy / sig;
@@ -538,9 +546,16 @@ __device__ __forceinline__ void features_at(const FeatArgs &a) {
         const double mphi = a.m1 / sqrt(d1x * d1x + d1y * d1y + a.rs2) + a.m2 / sqrt(d2x * d2x + d2y * d2y + a.rs2);
         f[3] = (float)(a.a2 * log(fmax(mphi, 1e-30)));
     }
-    uint16_t *o = a.x + p * C;
 #pragma unroll
     for (int c = 0; c < 4; ++c) o[c] = f2bf(fminf(fmaxf(f[c], -16.0f), 16.0f));
+}
+
+__device__ __forceinline__ void features_at(const FeatArgs &a) {
+    const int64_t NN = (int64_t)a.N * a.N;
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= NN) return;
+    uint16_t *o = a.x + p * C;
+    feature_values(a, p, o);
     for (int c = 4; c < C; ++c) o[c] = 0;
 }
 
@@ -593,6 +608,24 @@ __global__ void __launch_bounds__(256) features_step_kernel(FeatArgs a, const St
     a.x1 = o.x1; a.y1 = o.y1; a.m1 = o.m1;
     a.x2 = o.x2; a.y2 = o.y2; a.m2 = o.m2;
     features_at(a);
+}
+
+// The fused forward pass of step f of a corrected advance with the features
+// computed from the step's starting state in the kernel (the same values as
+// features_step_kernel's, for the tile and its halo), so that no feature
+// buffer is written and read back.
+__global__ void __launch_bounds__(256) net_forward_fused_step_kernel(const FusedArgs a, FeatArgs fa,
+                                                                     const StepIO *io, int f) {
+    const StepIO &o = io[f];
+    fa.u = o.u_in;
+    fa.x1 = o.x1; fa.y1 = o.y1; fa.m1 = o.m1;
+    fa.x2 = o.x2; fa.y2 = o.y2; fa.m2 = o.m2;
+    fused_forward(a, [&](uint16_t *F, int q, int gi, int gj) {
+        uint16_t v[4];
+        feature_values(fa, (int64_t)gi * a.n + gj, v);
+        *(uint2 *)(F + q * 4) = make_uint2((unsigned)v[0] | ((unsigned)v[1] << 16),
+                                           (unsigned)v[2] | ((unsigned)v[3] << 16));
+    });
 }
 
 struct TargetArgs {
@@ -827,15 +860,22 @@ int forward_layers(cdc_closure *c, hipStream_t s) {
 
 // the net's outputs only (inference): fused over 16 x 16 tiles where the
 // grid allows, layer by layer otherwise
-int forward(cdc_closure *c, hipStream_t s) {
-    if (c->n % TT != 0 || c->P != c->n * c->n) return forward_layers(c, s);
+bool fused_grid(const cdc_closure *c) { return c->n % TT == 0 && c->P == c->n * c->n; }
+
+FusedArgs fused_args(const cdc_closure *c) {
     FusedArgs a{};
     a.x = c->d_act[0];
     a.w = c->d_wf;
     a.par = c->d_par;
     a.y = c->d_y;
     a.n = c->n;
-    hipLaunchKernelGGL(net_forward_fused_kernel, dim3((unsigned)(c->P / (TT * TT))), dim3(256), 0, s, a);
+    return a;
+}
+
+int forward(cdc_closure *c, hipStream_t s) {
+    if (!fused_grid(c)) return forward_layers(c, s);
+    hipLaunchKernelGGL(net_forward_fused_kernel, dim3((unsigned)(c->P / (TT * TT))), dim3(256), 0, s,
+                       fused_args(c));
     HIP_TRY(hipGetLastError());
     return CDC_OK;
 }
@@ -1279,9 +1319,15 @@ int enqueue_fold(cdc_closure *c, cdc_solver *s, int fold, int rk, bool fuse) {
     feature_geometry(s, fa);
     const ApplyArgs ab = apply_base(c, s);
     for (int f = 0; f < fold; ++f) {
-        hipLaunchKernelGGL(features_step_kernel, dim3(grid), dim3(256), 0, s->stream, fa, c->d_io, f);
-        HIP_TRY(hipGetLastError());
-        if ((rc = forward(c, s->stream))) return rc;
+        if (fused_grid(c)) {
+            hipLaunchKernelGGL(net_forward_fused_step_kernel, dim3((unsigned)(c->P / (TT * TT))), dim3(256), 0,
+                               s->stream, fused_args(c), fa, c->d_io, f);
+            HIP_TRY(hipGetLastError());
+        } else {
+            hipLaunchKernelGGL(features_step_kernel, dim3(grid), dim3(256), 0, s->stream, fa, c->d_io, f);
+            HIP_TRY(hipGetLastError());
+            if ((rc = forward_layers(c, s->stream))) return rc;
+        }
         if ((rc = cdc::host::enqueue_stage_range(s, n, (size_t)f * rk, (size_t)(f + 1) * rk, fuse))) return rc;
         hipLaunchKernelGGL(limit_step_kernel, dim3(grid), dim3(256), 0, s->stream, ab, c->d_io, f);
         HIP_TRY(hipGetLastError());
