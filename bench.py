@@ -423,7 +423,17 @@ def main():
                      "bytes_per_launch": bytes_total / max(launches, 1),
                      "valu_issue": valu_issue(prof, kernel_ms / max(launches, 1)),
                      "timing": "HIP events bound to each stage dispatch, %d steps after the timed region"
-                               % timing_steps},
+                               % timing_steps,
+                     # the timed region's own figure: its wall time per stage
+                     # launch, which also holds the other launches of an
+                     # advance and the gaps between launches, so a lower
+                     # bound on the kernel's rate (graph replays run the
+                     # launches back to back, at a lower shader clock)
+                     "timed_region": {
+                         "ms_per_launch": wall * 1e3 / (steps * rk),
+                         "achieved": algorithmic_bytes_per_step(hydro.nv, rk) * zones * steps / wall / 1e9,
+                         "frac": algorithmic_bytes_per_step(hydro.nv, rk) * zones * steps / wall / 1e9
+                                 / HBM_PEAK_GBS}},
     }
     g.close()
     if not args.no_cpu:
