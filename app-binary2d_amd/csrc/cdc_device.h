@@ -93,16 +93,13 @@ __device__ __forceinline__ double fsqrt(double x) {
 
 // sqrt(x + 1e-300), x >= 0, without the zero test: the tiny addend changes
 // no x above 2e-284 (it is below half an ulp there) and keeps x = 0 finite
-// (sqrt(1e-300) = 1e-150), so a squared distance needs no select
+// (sqrt(1e-300) = 1e-150), so a squared distance needs no select.  Formed
+// as x times its refined 1/sqrt: a few ulp from the rounded root, where
+// fsqrt's residual correction costs two operations more (the alpha face
+// factor, whose distances enter the viscosity only)
 __device__ __forceinline__ double fsqrt_nn(double x) {
     x += 1e-300;
-    const double y = __builtin_amdgcn_rsq(x);
-    double h = 0.5 * y, g = x * y;
-    const double r = __builtin_fma(-h, g, 0.5);
-    h = __builtin_fma(h, r, h);
-    g = __builtin_fma(g, r, g);
-    const double d = __builtin_fma(-g, g, x);
-    return __builtin_fma(d, h, g);
+    return x * frsq(x);
 }
 
 // Block geometry exactly as Mesh::block_vertices / cell_centers
