@@ -380,7 +380,7 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
                   loop_rounds=0, loop_starts=8, loop_steps=200, loop_every=10, loop_train_steps=2000, burn_in=0,
                   rollout_rounds=0, rollout_steps=200, rollout_every=20, rollout_train_steps=2000, rollout_gain=0.1,
                   rollout_smooth=8, pair_sinks=True, pair_start_orbits=0.0, extra_runs=(), main_pairs=True,
-                  main_orbits=None, log=print):
+                  main_orbits=None, autotune=True, log=print):
     """Train a closure for the coarse grid (coarse_blocks^2 blocks, 4x coarser
     than the fine workload) on pairs taken every sample_every fine steps along
     the fine run up to train_orbits: each pair compares pair_steps coarse steps
@@ -409,7 +409,8 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
     main_pairs False the fine workload itself gives none (it is then only
     advanced, to main_orbits if given, else to train_orbits, and a closure
     evaluated on it has never seen its parameters).  The sample capacity is
-    shared equally.  Returns a dict with
+    shared equally.  autotune: every run on the stage kernel the autotuner
+    picks for its grid (else the default tile).  Returns a dict with
     the fine solver at the end of the training window, the trained closure
     (limiter kappa and mask as given), the coarse mesh, hydro, physics, the
     coarse initial state, the trained parameters and output scale, and the
@@ -432,6 +433,17 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
         ph_pair.sink_rate = 0.0
     plain = Solver(cm, hydro, ph_pair, cu0)
     fine_pair = None if pair_sinks else Solver(fm, hydro, ph_pair, u0)
+    # every run of the study on the stage kernel the autotuner picks for its
+    # grid (the plain fine run is the learned runs' wall-time reference)
+    if autotune:
+        fine.autotune(reps=3)
+        base.autotune(reps=3)
+    kf, kc = fine.get_kernel(), base.get_kernel()
+    for sv in (truth, plain):
+        sv.set_kernel(*kc)
+    if fine_pair is not None:
+        fine_pair.set_kernel(*kf)
+    log(f"stage kernels: fine {kf}, coarse {kc}")
     N = cm.num_blocks * cm.block_size
     dxc = 2.0 * cm.domain_radius / N
     per_round = loop_starts * (-(-loop_steps // loop_every))
@@ -497,6 +509,10 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
         eb, et = (Solver(cm, w["hydro"], w["physics"], ecu0) for _ in range(2))
         ep = Solver(cm, w["hydro"], eph_pair, ecu0)
         efp = None if pair_sinks else Solver(w["mesh"], w["hydro"], eph_pair, ew0)
+        for sv in (ef,) + ((efp,) if efp is not None else ()):
+            sv.set_kernel(*kf)
+        for sv in (eb, et, ep):
+            sv.set_kernel(*kc)
         start = net.samples()[0]
         log(f"training pairs from a run with e {w['physics'].binary_eccentricity:.3f}, "
             f"q {w['physics'].binary_mass_ratio:.3f}, Mach {w['hydro'].mach_number:.1f}")
@@ -598,6 +614,7 @@ def train_on_fine(fine_w, coarse_blocks=8, train_orbits=2.0, sample_every=100, m
         for sn in snaps:
             sn.close()
     return {"fine": fine, "net": net, "coarse_mesh": cm, "hydro": hydro, "physics": ph, "cu0": cu0,
+            "kernel_fine": kf, "kernel_coarse": kc,
             "params": net.params(), "scale": scale,
             "info": {"fine_grid": fm.num_blocks * fm.block_size, "coarse_grid": N, "samples": nsamp,
                      "train_steps": train_steps, "loss_first50": float(hist[:50].mean()),
@@ -616,16 +633,18 @@ def _perturbed(solver, rel, seed):
                      torque=solver.torque())
 
 
-def mdot_spread(fine, snap, coarse_mesh, hydro, physics, cu0, t_end, runs, rel=1e-10, log=print):
+def mdot_spread(fine, snap, coarse_mesh, hydro, physics, cu0, t_end, runs, rel=1e-10, kernel=None, log=print):
     """Time-averaged accretion rates of `runs` plain fine runs (the fine
     solver restored from snap, its state perturbed by _perturbed with seeds
     1..runs) and of plain coarse runs from each perturbed state
     down-sampled, over [snap's time, t_end], stepped as learned_vs_fine steps
     its plain runs.  Returns the rates and their relative spread (max - min
-    over the mean) for both grids."""
+    over the mean) for both grids; kernel: the coarse runs' stage kernel."""
     from .solver import Solver
     fr, cr = [], []
     coarse = Solver(coarse_mesh, hydro, physics, cu0)
+    if kernel is not None:
+        coarse.set_kernel(*kernel)
     try:
         for k in range(1, runs + 1):
             fine.restore(snap)
@@ -779,10 +798,13 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
     # held-out window from the fine state at the end of training
     t_start = fine.time()
     t_end = t_start + eval_orbits * ORBIT
+    kc = t["kernel_coarse"]
     start = Solver(cm, hydro, ph, cu0)
     start.resample_from(fine)
     coarse = Solver(cm, hydro, ph, cu0)
+    coarse.set_kernel(*kc)
     coarse.resample_from(fine)
+    out["stage_kernels"] = {"fine": list(t["kernel_fine"]), "coarse": list(kc)}
     out["window_orbits"] = [t_start / ORBIT, t_end / ORBIT]
     snap0 = None
     if spread_runs > 0:
@@ -825,6 +847,7 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
             comp = {0: np.ones(3), 1: np.array([1.0, 0.0, 0.0]), 2: np.array([0.0, 1.0, 1.0])}[int(var[3])]
             net.set_scales(output_scale=trained_scale * var[1] * comp)
             solver = Solver(cm, hydro, ph, cu0)
+            solver.set_kernel(*kc)
             solver.resample_from(start)
         m0, ta, tq0 = accreted(solver), solver.time(), solver.torque().sum()
         try:
@@ -846,7 +869,7 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
             out[name]["sigma_l1_rel"] = float(np.abs(u[..., 0] - sig_fine).sum() / np.abs(sig_fine).sum())
         log(f"{name}: {n} calls, {wall:.2f} s, mdot {mdot:.6e}, torque {torque:.6e}")
     if snap0 is not None:
-        out["mdot_spread"] = mdot_spread(fine, snap0, cm, hydro, ph, cu0, t_end, spread_runs, log=log)
+        out["mdot_spread"] = mdot_spread(fine, snap0, cm, hydro, ph, cu0, t_end, spread_runs, kernel=kc, log=log)
         snap0.close()
     if "mdot" in out.get("fine", {}):
         ref, reft = out["fine"]["mdot"], out["fine"]["torque"]

@@ -239,6 +239,61 @@ def run_sink_quadrature(args):
                       "config": {"workload": "sinkq", "fine": "iso2048", **r}}), flush=True)
 
 
+def run_closure_step(args):
+    """Wall time of corrected and plain steps on the learned closure's 512^2
+    isothermal coarse grid (the 2048^2 initial state down-sampled 4x), the
+    net with seeded random weights: --steps steps of each after --warmup,
+    --learned-fold corrected steps per call and --fold plain steps per call,
+    as one JSON line."""
+    import numpy as np
+    from cdc_amd import Solver, workloads
+    from cdc_amd.closure import Closure, num_params
+    from cdc_amd.config import Mesh
+    from cdc_amd.solver import resample_state
+    w = workloads.get("iso2048")
+    fm = w["mesh"]
+    cm = Mesh(8, 64, fm.domain_radius)
+    cu0, _, _ = resample_state(fm, cm, workloads.initial_state(w))
+    n = cm.num_blocks * cm.block_size
+    net = Closure(n, cm.domain_radius, limiter=0.02, mask_density=0.05)
+    rng = np.random.default_rng(args.closure_seed)
+    net.set_params((0.05 * rng.standard_normal(num_params())).astype(np.float32))
+    # small output scale: the corrections act, the flow stays the plain one's
+    net.set_scales(output_scale=[1e-6, 1e-6, 1e-6])
+    lf, pf = max(1, args.learned_fold), max(1, args.fold)
+
+    def timed(call, per_call):
+        for _ in range(max(1, args.warmup // per_call)):
+            call()
+        k = max(1, args.steps // per_call)
+        t0 = time.perf_counter()
+        for _ in range(k):
+            call()
+        return (time.perf_counter() - t0) / (k * per_call), k * per_call
+
+    s = Solver(cm, w["hydro"], w["physics"], cu0)
+    p = Solver(cm, w["hydro"], w["physics"], cu0)
+    # the stage kernel: the given tile, else the autotuner's pick, both runs
+    if args.tile:
+        s.set_tile(*(int(v) for v in args.tile.lower().split("x")))
+    elif not args.no_autotune:
+        s.autotune(reps=3)
+    p.set_kernel(*s.get_kernel())
+    t_corr, n_corr = timed(lambda: net.advance(s, lf), lf)
+    t_plain, n_plain = timed(lambda: p.advance(pf), pf)
+    tile = "%dx%d" % p.get_tile()
+    net.close(); s.close(); p.close()
+    print(json.dumps({"metric": "corrected 512\u00b2 step wall time", "value": t_corr * 1e6, "unit": "us",
+                      "n_gpus": 1, "steps": n_corr, "warmup": args.warmup, "higher_is_better": False,
+                      "dtype": "f64 (net: bf16 operands, fp32 accumulation)",
+                      "data": "synthetic (seeded cavity disk, seed %d, down-sampled 4x)" % w["seed"],
+                      "config": {"workload": "closure512", "grid": "512^2", "blocks": "8x8 of 64^2",
+                                 "hydro": "isothermal", "rk": w["physics"].rk_order, "learned_fold": lf,
+                                 "plain_fold": pf, "plain_us_per_step": t_plain * 1e6,
+                                 "tile": tile,
+                                 "plain_steps": n_plain, "net_seed": args.closure_seed}}), flush=True)
+
+
 def run_learned(args):
     """Learned 512^2 closure vs plain 2048^2 (and plain 512^2), isothermal:
     wall time per orbit, accretion rate, torque and Sigma error over a
@@ -348,6 +403,9 @@ def main():
         return
     if args.workload == "sinkq":
         run_sink_quadrature(args)
+        return
+    if args.workload == "closure512":
+        run_closure_step(args)
         return
     from cdc_amd import HydroError, Solver, workloads
     w = workloads.get(args.workload)

@@ -339,7 +339,7 @@ def test_trained_closure_gains_on_heldout_states():
     from cdc_amd import workloads
     from cdc_amd.closure import heldout_one_step, train_on_fine
     t = train_on_fine(workloads.get("iso256"), coarse_blocks=2, train_orbits=1.0, sample_every=20,
-                      max_samples=80, train_steps=2000, log=lambda m: None)
+                      max_samples=80, train_steps=2000, autotune=False, log=lambda m: None)
     try:
         assert t["info"]["loss_last200"] < 0.2 * t["info"]["loss_first50"]
         h = heldout_one_step(t["fine"], t["net"], t["coarse_mesh"], t["hydro"], t["physics"], t["cu0"],
@@ -461,11 +461,15 @@ def test_trained_closure_gains_on_heldout_states_2048_to_512():
     the isothermal 2048^2 run, 4000 Adam steps): the corrected 512^2 step
     lands closer to the down-sampled fine run than the plain 512^2 step on
     the 12 fine states that follow the training window.  (With 1 orbit and
-    2000 steps the ratios were 0.90, 0.86, 0.96.)"""
+    2000 steps the ratios were 0.90, 0.86, 0.96.)  Every run on the default
+    tiles (autotune=False): the autotuner's pick follows its timings, and the
+    pairs' rounding, and with it the trained net, follow the pick (measured
+    0.73, 0.87, 0.78 on the default tiles; 0.73, 0.83-0.91, 0.80-0.83 with
+    the fine run on the picked strips)."""
     from cdc_amd import workloads
     from cdc_amd.closure import heldout_one_step, train_on_fine
     t = train_on_fine(workloads.get("iso2048"), coarse_blocks=8, train_orbits=2.0, sample_every=100,
-                      max_samples=200, train_steps=4000, log=lambda m: None)
+                      max_samples=200, train_steps=4000, autotune=False, log=lambda m: None)
     try:
         assert t["coarse_mesh"].num_blocks * t["coarse_mesh"].block_size == 512
         assert t["info"]["loss_last200"] < 0.5 * t["info"]["loss_first50"]
@@ -487,7 +491,7 @@ def test_sink_free_pairs_gain_on_transport():
     from cdc_amd import workloads
     from cdc_amd.closure import heldout_one_step, train_on_fine
     t = train_on_fine(workloads.get("iso256"), coarse_blocks=2, train_orbits=1.0, sample_every=20,
-                      max_samples=80, train_steps=2000, pair_sinks=False, log=lambda m: None)
+                      max_samples=80, train_steps=2000, pair_sinks=False, autotune=False, log=lambda m: None)
     try:
         assert t["info"]["samples"] > 0
         assert t["info"]["loss_last200"] < 0.2 * t["info"]["loss_first50"]
