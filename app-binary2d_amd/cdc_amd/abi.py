@@ -184,6 +184,7 @@ SIGNATURES = {
     "cdc_closure_set_params": (c_int32, [c_void_p, c_void_p]),
     "cdc_closure_get_config": (c_int32, [c_void_p, POINTER(CClosureConfig)]),
     "cdc_closure_set_scales": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "cdc_closure_set_interval": (c_int32, [c_void_p, c_int32]),
     "cdc_closure_features": (c_int32, [c_void_p, c_void_p, c_void_p]),
     "cdc_closure_forward": (c_int32, [c_void_p, c_void_p, c_void_p]),
     "cdc_closure_add_sample": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_double]),

@@ -73,6 +73,11 @@ class Closure:
         _check(self.lib, self.lib.cdc_closure_set_scales(self.handle, None if o is None else _ptr(o),
                                                          None if w is None else _ptr(w)))
 
+    def set_interval(self, k):
+        """Correct every k-th step of an advance: the net on a group's
+        starting state, the correction over the group's k steps."""
+        _check(self.lib, self.lib.cdc_closure_set_interval(self.handle, int(k)))
+
     def features(self, solver):
         out = np.empty((self.grid * self.grid, 4), dtype=np.float32)
         _check(self.lib, self.lib.cdc_closure_features(self.handle, solver.handle, _ptr(out)))
@@ -741,9 +746,10 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
     learned coarse runs over the held-out window
     [train_orbits, train_orbits + eval_orbits] from the same fine state; one
     learned run per (limiter kappa, damping, mask density[, components])
-    variant, the damping scaling the trained output and the fourth entry
+    variant, the damping scaling the trained output, the fourth entry
     keeping only the mass-flux correction (1) or only the momentum-flux
-    corrections (2).  Every run ends exactly at the
+    corrections (2) and the fifth correcting every that many steps
+    (Closure.set_interval).  Every run ends exactly at the
     window's end.  Reported per run: wall time per orbit, the time-averaged
     accretion rate and torque on the binary over the window, and for the
     coarse runs the L1 error of Sigma at the window's end against the
@@ -830,11 +836,11 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
         seed_params.append((sd, f"_s{sd}", trainer.params()))
     for sd, suffix, prm in seed_params:
         for v in variants:
-            kappa, damping, mask, mass_only = (tuple(v) + (0.0, 0.0))[:4]
+            kappa, damping, mask, mass_only, every = (tuple(v) + (0.0, 0.0, 1.0))[:5]
             name = (f"learned_k{kappa:g}_d{damping:g}_m{mask:g}" + {0: "", 1: "_mass", 2: "_mom"}[int(mass_only)]
-                    + suffix)
+                    + (f"_i{int(every)}" if int(every) > 1 else "") + suffix)
             runs.append((name, None, lambda s: learned_fold * net.advance(s, learned_fold),
-                         (kappa, damping, mask, mass_only, prm)))
+                         (kappa, damping, mask, mass_only, prm, int(every))))
     sig_fine = None
     for name, solver, step, var in runs:
         if var is not None:
@@ -846,6 +852,7 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
             # the momentum fluxes only (2)
             comp = {0: np.ones(3), 1: np.array([1.0, 0.0, 0.0]), 2: np.array([0.0, 1.0, 1.0])}[int(var[3])]
             net.set_scales(output_scale=trained_scale * var[1] * comp)
+            net.set_interval(var[5])
             solver = Solver(cm, hydro, ph, cu0)
             solver.set_kernel(*kc)
             solver.resample_from(start)

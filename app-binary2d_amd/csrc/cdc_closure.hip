@@ -566,7 +566,7 @@ __global__ void __launch_bounds__(256) features_kernel(const FeatArgs a) { featu
 struct StepIO {
     const double *u_in;   // the state the step starts from (the net's input)
     double *u_out;        // the step's result, corrected in place
-    double dt;
+    double dt;            // the correction's time: the group's steps x dt
     double x1, y1, m1, x2, y2, m2;   // the bodies at the step's start
     double *e_out;                   // the strip-edge columns of u_out (null: not marching)
 };
@@ -787,6 +787,7 @@ struct cdc_closure {
     };
     std::vector<FoldGraph> graphs;
     bool use_graph = true;
+    int interval = 1;   // steps per correction (cdc_closure_set_interval)
 };
 
 namespace {
@@ -1138,6 +1139,17 @@ int32_t cdc_closure_get_config(cdc_closure_t *c, cdc_closure_config_t *out) {
     return CDC_OK;
 }
 
+int32_t cdc_closure_set_interval(cdc_closure_t *c, int32_t interval) {
+    if (!c) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null closure");
+    if (interval < 1) return set_msg(CDC_ERR_INVALID_ARGUMENT, "interval must be >= 1");
+    if (interval != c->interval) {
+        if (c->stream) HIP_TRY(hipStreamSynchronize(c->stream));
+        drop_fold_graphs(c);
+        c->interval = interval;
+    }
+    return CDC_OK;
+}
+
 int32_t cdc_closure_set_scales(cdc_closure_t *c, const double *output_scale, const double *loss_weight) {
     if (!c) return set_msg(CDC_ERR_INVALID_ARGUMENT, "null closure");
     for (int v = 0; v < 3; ++v) {
@@ -1319,16 +1331,21 @@ int enqueue_fold(cdc_closure *c, cdc_solver *s, int fold, int rk, bool fuse) {
     feature_geometry(s, fa);
     const ApplyArgs ab = apply_base(c, s);
     for (int f = 0; f < fold; ++f) {
-        if (fused_grid(c)) {
+        // steps in groups of c->interval: the net on a group's starting
+        // state, the correction after its last step
+        const bool first = f % c->interval == 0;
+        const bool last = f % c->interval == c->interval - 1 || f == fold - 1;
+        if (first && fused_grid(c)) {
             hipLaunchKernelGGL(net_forward_fused_step_kernel, dim3((unsigned)(c->P / (TT * TT))), dim3(256), 0,
                                s->stream, fused_args(c), fa, c->d_io, f);
             HIP_TRY(hipGetLastError());
-        } else {
+        } else if (first) {
             hipLaunchKernelGGL(features_step_kernel, dim3(grid), dim3(256), 0, s->stream, fa, c->d_io, f);
             HIP_TRY(hipGetLastError());
             if ((rc = forward_layers(c, s->stream))) return rc;
         }
         if ((rc = cdc::host::enqueue_stage_range(s, n, (size_t)f * rk, (size_t)(f + 1) * rk, fuse))) return rc;
+        if (!last) continue;   // the stages wrote the result's edge columns themselves
         hipLaunchKernelGGL(limit_step_kernel, dim3(grid), dim3(256), 0, s->stream, ab, c->d_io, f);
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(apply_step_kernel, dim3(grid), dim3(256), 0, s->stream, ab, c->d_io, f);
@@ -1399,7 +1416,10 @@ int32_t cdc_closure_advance_fold(cdc_closure_t *c, cdc_solver_t *s, int32_t fold
     for (int f = 0; f < fold; ++f) {
         c->h_io[f].u_in = plan.args[(size_t)f * rk].uc;
         c->h_io[f].u_out = plan.args[(size_t)f * rk + rk - 1].uo;
-        c->h_io[f].dt = dt;
+        // a group's last step corrects over the group's steps
+        const int g0 = f - f % c->interval;
+        const bool last = f % c->interval == c->interval - 1 || f == fold - 1;
+        c->h_io[f].dt = last ? (double)(f - g0 + 1) * dt : dt;
         const cdc::Orbit &o = plan.args[(size_t)f * rk].orb;
         c->h_io[f].x1 = o.x1; c->h_io[f].y1 = o.y1; c->h_io[f].m1 = o.m1;
         c->h_io[f].x2 = o.x2; c->h_io[f].y2 = o.y2; c->h_io[f].m2 = o.m2;

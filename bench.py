@@ -260,6 +260,7 @@ def run_closure_step(args):
     net.set_params((0.05 * rng.standard_normal(num_params())).astype(np.float32))
     # small output scale: the corrections act, the flow stays the plain one's
     net.set_scales(output_scale=[1e-6, 1e-6, 1e-6])
+    net.set_interval(args.correction_interval)
     lf, pf = max(1, args.learned_fold), max(1, args.fold)
 
     def timed(call, per_call):
@@ -289,6 +290,7 @@ def run_closure_step(args):
                       "data": "synthetic (seeded cavity disk, seed %d, down-sampled 4x)" % w["seed"],
                       "config": {"workload": "closure512", "grid": "512^2", "blocks": "8x8 of 64^2",
                                  "hydro": "isothermal", "rk": w["physics"].rk_order, "learned_fold": lf,
+                                 "correction_interval": args.correction_interval,
                                  "plain_fold": pf, "plain_us_per_step": t_plain * 1e6,
                                  "tile": tile,
                                  "plain_steps": n_plain, "net_seed": args.closure_seed}}), flush=True)
@@ -365,6 +367,8 @@ def main():
                     help="corrected coarse steps per call of the learned runs")
     ap.add_argument("--eval-orbits", type=float, default=2.0)
     ap.add_argument("--closure-seed", type=int, default=5, help="seed of the net's initial weights and batches")
+    ap.add_argument("--correction-interval", type=int, default=1,
+                    help="closure512: correct every that many steps")
     ap.add_argument("--pair-start-orbits", type=float, default=0.0, help="training pairs only from this time on")
     ap.add_argument("--train-runs", type=int, default=0,
                     help="train on this many 2048^2 runs with drawn parameters instead of the evaluated run")
@@ -386,8 +390,9 @@ def main():
     ap.add_argument("--burn-in", type=int, default=0,
                     help="coarse steps beside the fine run before each pair (tendency targets)")
     ap.add_argument("--variants", default="0.02:0.25:0.05,0.02:0.5:0.05,0.05:0.5:0.05",
-                    help="learned runs as kappa:damping:mask_density[:components],...; components 1: the "
-                         "mass flux only, 2: the momentum fluxes only")
+                    help="learned runs as kappa:damping:mask_density[:components[:interval]],...; components "
+                         "1: the mass flux only, 2: the momentum fluxes only; interval: correct every that many "
+                         "steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU only; --gpus {args.gpus} is not supported (the mesh would "

@@ -431,6 +431,13 @@ int32_t cdc_closure_advance(cdc_closure_t *closure, cdc_solver_t *solver, double
  * cdc_solver_advance(solver, fold).  cdc_closure_advance is fold 1. */
 int32_t cdc_closure_advance_fold(cdc_closure_t *closure, cdc_solver_t *solver, int32_t fold,
                                  double *dt_out, cdc_error_t *err);
+/* Correct every `interval`-th step only (default 1): the steps of a call go
+ * in groups of `interval` (the last group of a call may be shorter); the
+ * net is evaluated on a group's starting state and its limited correction
+ * applied to the group's plain result over the group's time (steps x dt).
+ * No counterpart in the reference (the learned closure is an extension);
+ * for callers that trade the correction's time resolution for wall time. */
+int32_t cdc_closure_set_interval(cdc_closure_t *closure, int32_t interval);
 
 /* ---- host utilities (no GPU) -------------------------------------------- */
 

@@ -212,16 +212,19 @@ def to_blocks(ug, nb, bs):
     return ug.reshape(nb, bs, nb, bs, nv).transpose(0, 2, 1, 3, 4).reshape(nb * nb, bs, bs, nv)
 
 
-def test_zero_net_fold_is_the_plain_run():
+@pytest.mark.parametrize("interval", [1, 2])
+def test_zero_net_fold_is_the_plain_run(interval):
     """An all-zero net through the corrected advance of 3 steps per call,
     twice (the first call captures its graph, the second replays it): bitwise
     the plain cdc_solver_advance(3), and the next dt (the corrected result's
-    max signal speed) is the plain run's.  (Two calls of 4 steps at one dt
-    lose this configuration to a negative pressure, oracle and GPU alike.)"""
+    max signal speed) is the plain run's; also with a correction every
+    second step.  (Two calls of 4 steps at one dt lose this configuration to
+    a negative pressure, oracle and GPU alike.)"""
     mesh, hydro, ph, u0 = setup()
     N = mesh.num_blocks * mesh.block_size
     net = Closure(N, mesh.domain_radius)
     net.set_params(np.zeros(num_params(), dtype=np.float32))
+    net.set_interval(interval)
     a, b = Solver(mesh, hydro, ph, u0), Solver(mesh, hydro, ph, u0)
     for _ in range(2):
         assert net.advance(a, fold=3) == b.advance(3)
@@ -233,7 +236,8 @@ def test_zero_net_fold_is_the_plain_run():
     assert net.advance(a, fold=0) == b.advance(0)
 
 
-def test_fold_matches_host_composition():
+@pytest.mark.parametrize("interval", [1, 2])
+def test_fold_matches_host_composition(interval):
     """Two calls of 3 corrected steps each (capture, then replay), limiter
     engaged, isothermal gas (the study's), against the same steps composed one at a time: the net's output
     on each step's starting state (cdc_closure_forward), a plain step at the
@@ -241,7 +245,9 @@ def test_fold_matches_host_composition():
     again.  The corrections act (they move the state by far more than the
     bar) and the states agree to 1e-3 of the largest correction applied (the
     host's correction rounds differently, and a bf16 feature can then round
-    the other way in a later step)."""
+    the other way in a later step).  With a correction every second step
+    the calls' steps go in groups of 2 and 1: the net on a group's starting
+    state, the group's plain steps, the correction over the group's time."""
     mesh, hydro, ph = H.small("iso", nb=2, bs=16, R=4.0)
     u0 = H.initial(mesh, hydro, ph)
     nb, bs = mesh.num_blocks, mesh.block_size
@@ -250,6 +256,7 @@ def test_fold_matches_host_composition():
     net = Closure(N, mesh.domain_radius, limiter=kappa)
     net.set_params(rand_params(8))
     net.set_scales(output_scale=sc)
+    net.set_interval(interval)
     a, b = Solver(mesh, hydro, ph, u0), Solver(mesh, hydro, ph, u0)
     plain = Solver(mesh, hydro, ph, u0)
     dx = 2 * mesh.domain_radius / N
@@ -258,13 +265,16 @@ def test_fold_matches_host_composition():
         dt = net.advance(a, fold=3)
         assert dt == pytest.approx(b.advance(0), rel=1e-12)
         plain.advance_dt(dt, 3)
-        for _ in range(3):
+        done = 0
+        while done < 3:
+            m = min(interval, 3 - done)
+            done += m
             Y = net.forward(b).astype(np.float64)
-            b.advance_dt(dt, 1)
+            b.advance_dt(dt, m)
             st, tq = b.get_state(), b.torque()
             ug = to_global(st["conserved"], nb, bs)
-            lam = C.limiter(Y, sc, dx, dx, dt, ug, kappa, 3)
-            step = dt * C.correction(Y, sc, dx, dx, lam)
+            lam = C.limiter(Y, sc, dx, dx, m * dt, ug, kappa, 3)
+            step = (m * dt) * C.correction(Y, sc, dx, dx, lam)
             biggest = max(biggest, float(np.max(np.abs(step))))
             ug[..., :3] += step
             b.set_state(to_blocks(ug, nb, bs), time=st["time"], iteration=st["iteration"], ist=st["ist"],
