@@ -733,6 +733,16 @@ def sink_quadrature(fine_w, coarse_blocks=8, t0_orbits=2.0, t1_orbits=22.0, ever
             "ratio_of_means": float(a[:, 2].mean() / a[:, 1].mean())}
 
 
+def variant_fields(v):
+    """(kappa, damping, mask density, components, interval) of a learned-run
+    variant given as 3 to 5 numbers: components default 0 (all), interval 1
+    (a correction every step)."""
+    v = tuple(float(x) for x in v)
+    if not 3 <= len(v) <= 5:
+        raise ValueError(f"a variant has 3 to 5 fields, not {len(v)}")
+    return v[0], v[1], v[2], int(v[3]) if len(v) > 3 else 0, int(v[4]) if len(v) > 4 else 1
+
+
 def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, sample_every=100,
                     max_samples=200, train_steps=4000, lr=1e-3, seed=5, limiter=0.1, train_mask=0.05,
                     pair_steps=1, variants=((0.1, 1.0, 0.05),), heldout_pairs=16, loop_rounds=0, loop_starts=8,
@@ -836,11 +846,11 @@ def learned_vs_fine(fine_w, coarse_blocks=8, train_orbits=2.0, eval_orbits=1.0, 
         seed_params.append((sd, f"_s{sd}", trainer.params()))
     for sd, suffix, prm in seed_params:
         for v in variants:
-            kappa, damping, mask, mass_only, every = (tuple(v) + (0.0, 0.0, 1.0))[:5]
-            name = (f"learned_k{kappa:g}_d{damping:g}_m{mask:g}" + {0: "", 1: "_mass", 2: "_mom"}[int(mass_only)]
-                    + (f"_i{int(every)}" if int(every) > 1 else "") + suffix)
+            kappa, damping, mask, mass_only, every = variant_fields(v)
+            name = (f"learned_k{kappa:g}_d{damping:g}_m{mask:g}" + {0: "", 1: "_mass", 2: "_mom"}[mass_only]
+                    + (f"_i{every}" if every > 1 else "") + suffix)
             runs.append((name, None, lambda s: learned_fold * net.advance(s, learned_fold),
-                         (kappa, damping, mask, mass_only, prm, int(every))))
+                         (kappa, damping, mask, mass_only, prm, every)))
     sig_fine = None
     for name, solver, step, var in runs:
         if var is not None:

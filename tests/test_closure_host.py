@@ -105,3 +105,14 @@ def test_closure_study_refuses_rounds_it_cannot_run(kw):
     train = () if "net_seeds" in kw else (w,)
     with pytest.raises(ValueError):
         learned_vs_fine(w, train_runs=train, **kw)
+
+
+def test_variant_fields_defaults():
+    """A learned-run variant of 3 fields corrects all components every step;
+    4 and 5 fields set the components and the correction interval."""
+    from cdc_amd.closure import variant_fields
+    assert variant_fields((0.02, 0.25, 0.05)) == (0.02, 0.25, 0.05, 0, 1)
+    assert variant_fields((0.02, 0.25, 0.05, 2)) == (0.02, 0.25, 0.05, 2, 1)
+    assert variant_fields((0.02, 0.25, 0.05, 0, 2)) == (0.02, 0.25, 0.05, 0, 2)
+    with pytest.raises(ValueError):
+        variant_fields((0.02, 0.25))
