@@ -550,3 +550,24 @@ def test_mdot_spread_of_perturbed_starts():
         assert r["fine_rel_spread"] < 1e-6, r
     finally:
         fine.close()
+
+
+def test_study_runs_end_to_end():
+    """learned_vs_fine on the 256^2 run and its 64^2 coarse grid with a few
+    pairs and Adam steps: the autotuned plain runs and a learned run
+    correcting every step and one correcting every second step (3- and
+    5-field variants) each complete or stop on a hydro error, and the
+    completed ones report finite rates and wall times over the window."""
+    from cdc_amd import workloads
+    from cdc_amd.closure import learned_vs_fine
+    r = learned_vs_fine(workloads.get("iso256"), coarse_blocks=2, train_orbits=0.2, eval_orbits=0.1,
+                        sample_every=20, max_samples=8, train_steps=50, heldout_pairs=2,
+                        variants=((0.02, 0.25, 0.05), (0.02, 0.25, 0.05, 0, 2)), learned_fold=4,
+                        log=lambda m: None)
+    assert len(r["stage_kernels"]["fine"]) == 3 and len(r["stage_kernels"]["coarse"]) == 3
+    assert r["window_orbits"][1] - r["window_orbits"][0] == pytest.approx(0.1)
+    for name in ("fine", "coarse", "learned_k0.02_d0.25_m0.05", "learned_k0.02_d0.25_m0.05_i2"):
+        assert name in r, name
+        if "failed" not in r[name]:
+            assert r[name]["wall_per_orbit_s"] > 0 and np.isfinite(r[name]["mdot"]), (name, r[name])
+    assert "failed" not in r["fine"] and "failed" not in r["coarse"]
