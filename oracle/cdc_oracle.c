@@ -38,6 +38,10 @@
  *   linspace       x_k = a + k (b - a)/(n - 1) (ndarray 0.14)
  *   adjacent_mean  (x_k + x_{k+1}) / 2
  * What pins each of them is listed in the tests (tests/test_oracle_pins.py).
+ * Parity unpinned (only the absent crates could pin it; the HIP path shares
+ * the convention, so GPU parity is unaffected): the order of the additions
+ * inside riemann_hlle and plm_gradient (the pins fix their values to
+ * rounding), and the Kepler phase at t = 0 and the body order above.
  * The Euler primitive recovery p = (E - Sigma |v|^2 / 2)(gamma - 1) and the
  * Euler max signal speed max(|vx|, |vy|) + c_s are chosen conventions whose
  * exact operation order only the absent crates could pin.
