@@ -667,14 +667,9 @@ __device__ __forceinline__ void march_strip(const StageArgs &A, const int tile, 
         prb = prz;
         axa = axb;
         ba = bb;
-        // the next row's state is complete at the end of every step (its
-        // loads were issued at the step's start): the loop is then entered
-        // from the first step and from its own end with nothing of that row
-        // in flight, and its first wait covers only that row's own loads
-        if constexpr (next) {
-#pragma unroll
-            for (int v = 0; v < NV; ++v) __asm__ volatile("" ::"v"(Uz[v]));
-        }
+        // (the next row's state is left in flight across the loop's back
+        // edge: forcing it complete at the end of every step measured the
+        // same on 2048^2 Euler and 1.5% slower on 1024^2 isothermal)
     };
 
     const std::true_type yes;
